@@ -1,0 +1,836 @@
+// pcg.hip - CSR SpMV/SpMM and device-resident Jacobi-PCG for gfx950.
+//
+// One HIP stream, no host synchronisation inside an iteration: every scalar of
+// the CG recurrence (alpha, beta, ||r||, the stopping and breakdown decisions)
+// lives on the device. Each iteration is three launches:
+//   S  q = A p                      (+ per-block partials of p.q)
+//   U  x += a p, r -= a q, z = r/d  (+ per-block partials of r.r, r.z)
+//   P  p = z + b p                  (stopping test, beta)
+// Cross-block sums never use atomics: the consumer kernel re-reduces the
+// producer's per-block partials in a fixed order, so results are bitwise
+// reproducible run to run, and every block takes the same decisions from the
+// same sums. The host polls the per-column status every `check_every`
+// iterations; kernels launched after every column has stopped return at once.
+//
+// The recurrence and the stopping / breakdown rules follow the reference GPU
+// loop, code/fast_solver_backends.py:233-274 (x0 = 0, r = b, z = r/d with
+// zero diagonal -> identity, rel = ||r||/||b|| <= tol non-strict, breakdown
+// when p.Ap <= 0 or non-finite, iterations = k at exit).
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "../../include/cutcell_hip.h"
+#include "common.h"
+
+namespace cc {
+
+// ---------------------------------------------------------------------------
+// Reductions.
+// ---------------------------------------------------------------------------
+
+// Sum a thread's value over all threads of the block that share its column
+// c = threadIdx.x % K; thread c < K writes the block total to out[c].
+template <int K>
+__device__ inline void block_col_sum(double v, double* out, double* smem) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int o = K; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+  if (lane < K) smem[wave * K + lane] = v;
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += smem[w * K + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+  __syncthreads();
+}
+
+// Reduce partial[nparts][PAIRS] over the parts, in a fixed order, into
+// out[PAIRS] (shared memory). PAIRS must divide blockDim.x.
+template <int PAIRS>
+__device__ inline void reduce_partials(const double* __restrict__ partial, int nparts, double* out, double* smem) {
+  const int pair = threadIdx.x % PAIRS;
+  const int sub = threadIdx.x / PAIRS;
+  const int nsub = blockDim.x / PAIRS;
+  double s = 0.0;
+  for (int b = sub; b < nparts; b += nsub) s += partial[(size_t)b * PAIRS + pair];
+  smem[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < PAIRS) {
+    double t = 0.0;
+    for (int j = 0; j < nsub; ++j) t += smem[j * PAIRS + threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+template <int K>
+__device__ inline bool all_stopped(const ColState* st) {
+  if (st == nullptr) return false;
+#pragma unroll
+  for (int c = 0; c < K; ++c)
+    if (st[c].status == CC_RUNNING) return false;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Kernels.
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_diag(int n, const int* __restrict__ indptr, const int* __restrict__ indices,
+                                              const double* __restrict__ vals, double* __restrict__ diag) {
+  int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  double d = 0.0;
+  for (int j = indptr[row]; j < indptr[row + 1]; ++j)
+    if (indices[j] == row) d += vals[j];
+  diag[row] = d;
+}
+
+// Sparse product over row teams of T = K*NZ lanes: lane (z, c) with
+// c = lane % K takes nonzeros z, z+NZ, ... of the row for right-hand side c;
+// the NZ partial sums are combined with xor shuffles inside the team.
+// MODE 0: Y = A X, partial[block][c] = sum_rows X.Y      (p.q of CG)
+// MODE 1: Y = B - A X, partial[block][c] = sum_rows Y.Y  (true residual)
+template <typename MT, typename VT, int K, int NZ, int MODE>
+__global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ indptr, const int* __restrict__ indices,
+                                              const MT* __restrict__ vals, const VT* __restrict__ X,
+                                              VT* __restrict__ Y, const double* __restrict__ B,
+                                              double* __restrict__ partial, const ColState* __restrict__ st) {
+  constexpr int T = K * NZ;
+  constexpr int TPW = 64 / T;
+  __shared__ double smem[16 * 16];
+  if (all_stopped<K>(st)) return;
+  using AT = typename Accum<VT>::type;
+  const int lane = threadIdx.x & 63;
+  const int c = lane % K;
+  const int zs = (lane / K) % NZ;
+  const int wave = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int team0 = (blockIdx.x * wpb + wave) * TPW + lane / T;
+  const int nteams = gridDim.x * wpb * TPW;
+  double dot = 0.0;
+  for (int row = team0; row < n; row += nteams) {
+    const int beg = indptr[row];
+    const int end = indptr[row + 1];
+    AT acc = 0;
+    for (int j = beg + zs; j < end; j += NZ) acc += (AT)vals[j] * (AT)X[(size_t)indices[j] * K + c];
+#pragma unroll
+    for (int o = K; o < T; o <<= 1) acc += __shfl_xor(acc, o, 64);
+    if (zs == 0) {
+      const size_t e = (size_t)row * K + c;
+      if (MODE == 0) {
+        const VT y = (VT)acc;
+        Y[e] = y;
+        if (partial) dot += (double)X[e] * (double)y;
+      } else {
+        const double y = B[e] - (double)acc;
+        Y[e] = (VT)y;
+        dot += y * y;
+      }
+    }
+  }
+  if (partial) block_col_sum<K>(dot, partial + (size_t)blockIdx.x * K, smem);
+}
+
+// x = 0, r = src * inv_scale[c], z = r/d, p = z; partials (r.z, r.r).
+template <typename VT, int K>
+__global__ __launch_bounds__(256) void k_cg_setup(int n, const double* __restrict__ src,
+                                                  const double* __restrict__ inv_scale, VT* __restrict__ X,
+                                                  VT* __restrict__ R, VT* __restrict__ Z, VT* __restrict__ P,
+                                                  const double* __restrict__ diag, int precond,
+                                                  double* __restrict__ partial) {
+  __shared__ double smem[16 * 16];
+  const size_t total = (size_t)n * K;
+  const int c = threadIdx.x % K;
+  const double s = inv_scale ? inv_scale[c] : 1.0;
+  double rz = 0.0, rr = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const VT r = (VT)(src[e] * s);
+    const double d = precond ? diag[e / K] : 0.0;
+    const VT z = (precond && fabs(d) > 0.0) ? (VT)((double)r / d) : r;
+    X[e] = (VT)0;
+    R[e] = r;
+    Z[e] = z;
+    P[e] = z;
+    rz += (double)r * (double)z;
+    rr += (double)r * (double)r;
+  }
+  block_col_sum<K>(rz, partial + ((size_t)blockIdx.x * 2 + 0) * K, smem);
+  block_col_sum<K>(rr, partial + ((size_t)blockIdx.x * 2 + 1) * K, smem);
+}
+
+// One block: ||b|| (0 -> 1), rho_0 = r.z, status. run_mask[c] == 0 marks a
+// column that is not to be solved (already converged outside).
+template <int K>
+__global__ __launch_bounds__(256) void k_cg_setup_finish(const double* __restrict__ partial, int nparts,
+                                                         ColState* __restrict__ st, const int* __restrict__ run_mask,
+                                                         double* __restrict__ hist) {
+  __shared__ double smem[256];
+  __shared__ double sums[2 * K];
+  reduce_partials<2 * K>(partial, nparts, sums, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    const double rz = sums[c];
+    const double bb = sums[K + c];
+    double bn = sqrt(bb);
+    if (bn == 0.0) bn = 1.0;
+    ColState s;
+    s.rho[0] = rz;
+    s.rho[1] = rz;
+    s.bnorm = bn;
+    s.rel = sqrt(bb) / bn;
+    s.status = (run_mask == nullptr || run_mask[c]) ? CC_RUNNING : CC_CONVERGED;
+    s.iterations = 0;
+    st[c] = s;
+    if (hist) hist[c] = s.rel;
+  }
+}
+
+// alpha = rho / p.q per column (breakdown when p.q <= 0 or non-finite);
+// x += alpha p, r -= alpha q, z = r/d; partials (r.r, r.z).
+template <typename VT, int K>
+__global__ __launch_bounds__(256) void k_cg_update(int n, VT* __restrict__ X, VT* __restrict__ R, VT* __restrict__ Z,
+                                                   const VT* __restrict__ P, const VT* __restrict__ Q,
+                                                   const double* __restrict__ diag, int precond,
+                                                   const double* __restrict__ partial_in, int nparts_in,
+                                                   ColState* __restrict__ st, int iter,
+                                                   double* __restrict__ partial_out) {
+  __shared__ double smem[256];
+  __shared__ double den[K];
+  __shared__ double alpha[K];
+  __shared__ int active[K];
+  if (all_stopped<K>(st)) return;
+  reduce_partials<K>(partial_in, nparts_in, den, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    bool act = st[c].status == CC_RUNNING;
+    const double dn = den[c];
+    if (act && !(dn > 0.0 && isfinite(dn))) {
+      act = false;
+      if (blockIdx.x == 0) {
+        st[c].status = CC_BREAKDOWN;
+        st[c].iterations = iter;
+      }
+    }
+    alpha[c] = act ? st[c].rho[iter & 1] / dn : 0.0;
+    active[c] = act;
+  }
+  __syncthreads();
+  const size_t total = (size_t)n * K;
+  const int c = threadIdx.x % K;  // fixed per thread: the stride is a multiple of K
+  const bool act = active[c];
+  const double a = alpha[c];
+  double rr = 0.0, rz = 0.0;
+  if (act) {
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+      const VT p = P[e];
+      const VT q = Q[e];
+      const VT x = (VT)((double)X[e] + a * (double)p);
+      const VT r = (VT)((double)R[e] - a * (double)q);
+      const double d = precond ? diag[e / K] : 0.0;
+      const VT z = (precond && fabs(d) > 0.0) ? (VT)((double)r / d) : r;
+      X[e] = x;
+      R[e] = r;
+      Z[e] = z;
+      rr += (double)r * (double)r;
+      rz += (double)r * (double)z;
+    }
+  }
+  block_col_sum<K>(rr, partial_out + ((size_t)blockIdx.x * 2 + 0) * K, smem);
+  block_col_sum<K>(rz, partial_out + ((size_t)blockIdx.x * 2 + 1) * K, smem);
+}
+
+// rel = ||r||/||b||: stop when rel <= tol (or iter == maxiter); otherwise
+// beta = (r.z)_new / rho and p = z + beta p.
+template <typename VT, int K>
+__global__ __launch_bounds__(256) void k_cg_direction(int n, VT* __restrict__ P, const VT* __restrict__ Z,
+                                                      const double* __restrict__ partial_in, int nparts_in,
+                                                      ColState* __restrict__ st, int iter, double tol, int maxiter,
+                                                      double* __restrict__ hist) {
+  __shared__ double smem[256];
+  __shared__ double sums[2 * K];
+  __shared__ double beta[K];
+  __shared__ int active[K];
+  if (all_stopped<K>(st)) return;
+  reduce_partials<2 * K>(partial_in, nparts_in, sums, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    bool act = st[c].status == CC_RUNNING;
+    double b = 0.0;
+    if (act) {
+      const double rr = sums[c];
+      const double rz = sums[K + c];
+      const double rel = sqrt(rr) / st[c].bnorm;
+      const double rho = st[c].rho[iter & 1];
+      if (blockIdx.x == 0) {
+        st[c].rel = rel;
+        st[c].iterations = iter;
+        if (hist) hist[(size_t)iter * K + c] = rel;
+      }
+      if (rel <= tol) {
+        act = false;
+        if (blockIdx.x == 0) st[c].status = CC_CONVERGED;
+      } else if (iter >= maxiter) {
+        act = false;
+        if (blockIdx.x == 0) st[c].status = CC_MAXITER;
+      } else {
+        b = rz / rho;
+        if (blockIdx.x == 0) st[c].rho[(iter + 1) & 1] = rz;
+      }
+    }
+    beta[c] = b;
+    active[c] = act;
+  }
+  __syncthreads();
+  const int c = threadIdx.x % K;
+  if (!active[c]) return;
+  const double b = beta[c];
+  const size_t total = (size_t)n * K;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    P[e] = (VT)((double)Z[e] + b * (double)P[e]);
+}
+
+// Outer refinement bookkeeping (one block): rel = ||B - A X|| / ||B||.
+// first != 0: record ||B|| (0 -> 1). Sets per-column run flags and the scale
+// of the next inner right-hand side (1 / ||residual||).
+template <int K>
+__global__ __launch_bounds__(256) void k_outer_finish(const double* __restrict__ partial, int nparts,
+                                                      OuterState* __restrict__ ost, int* __restrict__ run_mask,
+                                                      double* __restrict__ inv_scale, double tol, int first) {
+  __shared__ double smem[256];
+  __shared__ double sums[K];
+  reduce_partials<K>(partial, nparts, sums, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    const double rn = sqrt(sums[c]);
+    OuterState o = ost[c];
+    if (first) {
+      o.bnorm = rn == 0.0 ? 1.0 : rn;
+      o.status = CC_RUNNING;
+      o.outer = 0;
+      o.inner_total = 0;
+      o.best_rel = INFINITY;
+    }
+    o.rel = rn / o.bnorm;
+    if (o.status == CC_RUNNING && o.rel <= tol) o.status = CC_CONVERGED;
+    const bool run = o.status == CC_RUNNING;
+    run_mask[c] = run;
+    o.scale = rn;
+    inv_scale[c] = (run && rn > 0.0) ? 1.0 / rn : 0.0;
+    ost[c] = o;
+  }
+}
+
+// X64 += scale[c] * D32 for the columns of this sweep; bookkeeping of the
+// inner iteration counts.
+template <int K>
+__global__ __launch_bounds__(256) void k_outer_correct(int n, double* __restrict__ X, const float* __restrict__ D,
+                                                       const OuterState* __restrict__ ost,
+                                                       const int* __restrict__ run_mask) {
+  const size_t total = (size_t)n * K;
+  const int c = threadIdx.x % K;
+  if (!run_mask[c]) return;
+  const double s = ost[c].scale;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    X[e] += s * (double)D[e];
+}
+
+template <int K>
+__global__ void k_outer_account(OuterState* __restrict__ ost, const ColState* __restrict__ ist,
+                                const int* __restrict__ run_mask) {
+  const int c = threadIdx.x;
+  if (c >= K || !run_mask[c]) return;
+  ost[c].outer += 1;
+  ost[c].inner_total += ist[c].iterations;
+}
+
+__global__ void k_zero(double* __restrict__ p, size_t count) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch helpers.
+// ---------------------------------------------------------------------------
+
+constexpr int kBlock = 256;
+
+// Optional per-kernel-class event timing (process-wide, not thread-safe).
+// Each timed launch is bracketed by two events on the launch stream; the
+// elapsed times are collected whenever the host has synchronised the stream.
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> pending;
+  double ms[kProfClasses] = {0};
+  long long count[kProfClasses] = {0};
+};
+static Profiler g_prof;
+
+static void prof_mark(hipStream_t s) {
+  if (g_prof.used >= g_prof.pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    g_prof.pool.push_back(e);
+  }
+  hipEventRecord(g_prof.pool[g_prof.used], s);
+  ++g_prof.used;
+}
+
+static void prof_begin(int cls, hipStream_t s) {
+  if (!g_prof.on) return;
+  g_prof.pending.push_back({cls, g_prof.used});
+  prof_mark(s);
+}
+
+static void prof_end(hipStream_t s) {
+  if (!g_prof.on) return;
+  prof_mark(s);
+}
+
+// Call only after the stream has been synchronised.
+static void prof_collect() {
+  if (!g_prof.on) return;
+  for (auto& pr : g_prof.pending) {
+    float t = 0.f;
+    if (pr.second + 1 < g_prof.used && hipEventElapsedTime(&t, g_prof.pool[pr.second], g_prof.pool[pr.second + 1]) == hipSuccess) {
+      g_prof.ms[pr.first] += t;
+      g_prof.count[pr.first] += 1;
+    }
+  }
+  g_prof.pending.clear();
+  g_prof.used = 0;
+}
+
+static int stream_grid(size_t work) {
+  size_t g = (work + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > (size_t)kMaxParts) g = kMaxParts;
+  return (int)g;
+}
+
+template <int K>
+static int nz_lanes(double avg_row) {
+  if (K > 1) return 64 / K;
+  if (avg_row >= 96.0) return 64;
+  if (avg_row >= 40.0) return 32;
+  if (avg_row >= 20.0) return 16;
+  return 8;
+}
+
+template <typename MT, typename VT, int K, int NZ, int MODE>
+static int spmm_launch_nz(int n, const int* indptr, const int* indices, const MT* vals, const VT* X, VT* Y,
+                          const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  constexpr int T = K * NZ;
+  const int teams_per_block = (kBlock / 64) * (64 / T);
+  int grid = (n + teams_per_block - 1) / teams_per_block;
+  if (grid < 1) grid = 1;
+  if (grid > kMaxParts) grid = kMaxParts;
+  hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals, X, Y, B,
+                     partial, st);
+  if (grid_out) *grid_out = grid;
+  return (int)hipGetLastError();
+}
+
+template <typename MT, typename VT, int K, int MODE>
+static int spmm_launch(int n, int nz, const int* indptr, const int* indices, const MT* vals, const VT* X, VT* Y,
+                       const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  if constexpr (K == 1) {
+    switch (nz) {
+      case 64: return spmm_launch_nz<MT, VT, 1, 64, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
+      case 32: return spmm_launch_nz<MT, VT, 1, 32, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
+      case 16: return spmm_launch_nz<MT, VT, 1, 16, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
+      default: return spmm_launch_nz<MT, VT, 1, 8, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
+    }
+  } else {
+    return spmm_launch_nz<MT, VT, K, 64 / K, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
+  }
+}
+
+static double avg_row_len(int n, const int* indptr, hipStream_t s) {
+  // Only the two end values are needed: nnz = indptr[n] - indptr[0].
+  int ends[2] = {0, 0};
+  if (hipMemcpyAsync(&ends[0], indptr, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return 64.0;
+  if (hipMemcpyAsync(&ends[1], indptr + n, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return 64.0;
+  if (hipStreamSynchronize(s) != hipSuccess) return 64.0;
+  return n > 0 ? double(ends[1] - ends[0]) / n : 0.0;
+}
+
+// Workspace carving.
+struct Carve {
+  char* base;
+  size_t off = 0;
+  size_t cap;
+  template <typename T>
+  T* take(size_t count) {
+    off = (off + 255) & ~size_t(255);
+    T* p = reinterpret_cast<T*>(base + off);
+    off += count * sizeof(T);
+    return p;
+  }
+  bool ok() const { return off <= cap; }
+};
+
+template <typename VT>
+struct CgBuffers {
+  VT *X, *R, *Z, *P, *Q;
+  double *diag, *pa, *pb;
+  ColState* st;
+};
+
+template <typename VT>
+static size_t cg_bytes(size_t n, int k) {
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
+  for (int i = 0; i < 5; ++i) add(n * k * sizeof(VT));
+  add(n * sizeof(double));
+  add((size_t)kMaxParts * 2 * k * sizeof(double));
+  add((size_t)kMaxParts * 2 * k * sizeof(double));
+  add(k * sizeof(ColState));
+  return b + 256;
+}
+
+template <typename VT>
+static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external) {
+  CgBuffers<VT> b;
+  b.X = cv.take<VT>(n * k);
+  if (X_external) b.X = X_external;
+  b.R = cv.take<VT>(n * k);
+  b.Z = cv.take<VT>(n * k);
+  b.P = cv.take<VT>(n * k);
+  b.Q = cv.take<VT>(n * k);
+  b.diag = cv.take<double>(n);
+  b.pa = cv.take<double>((size_t)kMaxParts * 2 * k);
+  b.pb = cv.take<double>((size_t)kMaxParts * 2 * k);
+  b.st = cv.take<ColState>(k);
+  return b;
+}
+
+// The device-resident CG loop for k columns. src is the float64 right-hand
+// side block; inv_scale (device, may be null) scales it into VT. Returns a HIP
+// status; the per-column state is left in b.st.
+template <typename MT, typename VT, int K>
+static int cg_run(int n, int nz, const int* indptr, const int* indices, const MT* vals, const double* src,
+                  const double* inv_scale, const int* run_mask, CgBuffers<VT>& b, int precond, double tol,
+                  int maxiter, int check_every, hipStream_t s, std::vector<ColState>& host_st,
+                  double* hist = nullptr) {
+  const int sg = stream_grid((size_t)n * K);
+  hipLaunchKernelGGL((k_cg_setup<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, src, inv_scale, b.X, b.R, b.Z, b.P, b.diag,
+                     precond, b.pa);
+  hipLaunchKernelGGL((k_cg_setup_finish<K>), dim3(1), dim3(kBlock), 0, s, b.pa, sg, b.st, run_mask, hist);
+  CC_CHECK(hipGetLastError());
+  if (check_every < 1) check_every = 1;
+  host_st.resize(K);
+  for (int it = 1; it <= maxiter; ++it) {
+    int gs = 0;
+    prof_begin(kProfSpmm, s);
+    CC_CHECK((spmm_launch<MT, VT, K, 0>(n, nz, indptr, indices, vals, b.P, b.Q, nullptr, b.pa, b.st, s, &gs)));
+    prof_end(s);
+    prof_begin(kProfUpdate, s);
+    hipLaunchKernelGGL((k_cg_update<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.X, b.R, b.Z, b.P, b.Q, b.diag,
+                       precond, b.pa, gs, b.st, it, b.pb);
+    prof_end(s);
+    prof_begin(kProfDirection, s);
+    hipLaunchKernelGGL((k_cg_direction<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.P, b.Z, b.pb, sg, b.st, it, tol,
+                       maxiter, hist);
+    prof_end(s);
+    CC_CHECK(hipGetLastError());
+    if (it % check_every == 0 || it == maxiter) {
+      CC_CHECK(hipMemcpyAsync(host_st.data(), b.st, K * sizeof(ColState), hipMemcpyDeviceToHost, s));
+      CC_CHECK(hipStreamSynchronize(s));
+      prof_collect();
+      bool any = false;
+      for (int c = 0; c < K; ++c) any |= host_st[c].status == CC_RUNNING;
+      if (!any) return 0;
+    }
+  }
+  CC_CHECK(hipMemcpyAsync(host_st.data(), b.st, K * sizeof(ColState), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  prof_collect();
+  return 0;
+}
+
+// Column sums of squares of a row-major [n][K] block: partial[block][c].
+template <int K>
+__global__ __launch_bounds__(256) void k_colsq(int n, const double* __restrict__ V, double* __restrict__ partial) {
+  __shared__ double smem[16 * 16];
+  const size_t total = (size_t)n * K;
+  double s = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    s += V[e] * V[e];
+  block_col_sum<K>(s, partial + (size_t)blockIdx.x * K, smem);
+}
+
+// True relative residuals ||B - A X|| / ||B|| (||B|| = 0 -> 1) of the float64
+// block X, written to rel[K] (host). The per-block partial sums are combined
+// on the host in block order.
+template <int K>
+static int true_residual(int n, int nz, const int* indptr, const int* indices, const double* vals, const double* B,
+                         const double* X, double* R, double* partial, hipStream_t s, double* rel) {
+  int gs = 0;
+  CC_CHECK((spmm_launch<double, double, K, 1>(n, nz, indptr, indices, vals, X, R, B, partial, nullptr, s, &gs)));
+  const int sg = stream_grid((size_t)n * K);
+  hipLaunchKernelGGL((k_colsq<K>), dim3(sg), dim3(kBlock), 0, s, n, B, partial + (size_t)kMaxParts * K);
+  CC_CHECK(hipGetLastError());
+  std::vector<double> hr((size_t)gs * K), hb((size_t)sg * K);
+  CC_CHECK(hipMemcpyAsync(hr.data(), partial, hr.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipMemcpyAsync(hb.data(), partial + (size_t)kMaxParts * K, hb.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c) {
+    double rr = 0.0, bb = 0.0;
+    for (int g = 0; g < gs; ++g) rr += hr[(size_t)g * K + c];
+    for (int g = 0; g < sg; ++g) bb += hb[(size_t)g * K + c];
+    double bn = std::sqrt(bb);
+    if (bn == 0.0) bn = 1.0;
+    rel[c] = std::sqrt(rr) / bn;
+  }
+  return 0;
+}
+
+template <int K>
+static int pcg_f64(int n, const int* indptr, const int* indices, const double* vals, const double* B, double* X,
+                   double tol, int maxiter, int precond, int check_every, void* ws, size_t wsb,
+                   cc_pcg_result* results, double* hist, hipStream_t s) {
+  Carve cv{(char*)ws, 0, wsb};
+  CgBuffers<double> b = carve_cg<double>(cv, n, K, X);
+  if (!cv.ok()) return CC_E_WORKSPACE;
+  const double avg = avg_row_len(n, indptr, s);
+  const int nz = nz_lanes<K>(avg);
+  if (precond) {
+    hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, indptr, indices, vals, b.diag);
+    CC_CHECK(hipGetLastError());
+  }
+  std::vector<ColState> hs;
+  CC_CHECK((cg_run<double, double, K>(n, nz, indptr, indices, vals, B, nullptr, nullptr, b, precond, tol, maxiter,
+                                      check_every, s, hs, hist)));
+  std::vector<double> rel(K);
+  CC_CHECK(true_residual<K>(n, nz, indptr, indices, vals, B, X, b.R, b.pa, s, rel.data()));
+  for (int c = 0; c < K; ++c) {
+    results[c].iterations = hs[c].iterations;
+    results[c].status = hs[c].status == CC_RUNNING ? CC_MAXITER : hs[c].status;
+    results[c].outer = 1;
+    results[c].inner_total = hs[c].iterations;
+    results[c].recursive_rel = hs[c].rel;
+    results[c].true_rel = rel[c];
+  }
+  return 0;
+}
+
+template <int K>
+static size_t mixed_bytes(size_t n) {
+  size_t b = cg_bytes<float>(n, K);
+  auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
+  add(n * K * sizeof(double));                    // R64
+  add((size_t)kMaxParts * K * sizeof(double));    // residual partials
+  add(K * sizeof(OuterState));
+  add(K * sizeof(int));
+  add(K * sizeof(double));
+  return b + 256;
+}
+
+template <int K>
+static int pcg_mixed(int n, const int* indptr, const int* indices, const double* vals, const float* vals32,
+                     const double* B, double* X, double tol, int maxiter, double inner_tol, int max_outer,
+                     int precond, int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s) {
+  Carve cv{(char*)ws, 0, wsb};
+  CgBuffers<float> b = carve_cg<float>(cv, n, K, nullptr);
+  double* R64 = cv.take<double>((size_t)n * K);
+  double* pr = cv.take<double>((size_t)kMaxParts * K);
+  OuterState* ost = cv.take<OuterState>(K);
+  int* run_mask = cv.take<int>(K);
+  double* inv_scale = cv.take<double>(K);
+  if (!cv.ok()) return CC_E_WORKSPACE;
+  const double avg = avg_row_len(n, indptr, s);
+  const int nz = nz_lanes<K>(avg);
+  if (precond) {
+    hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, indptr, indices, vals, b.diag);
+  }
+  const int sg = stream_grid((size_t)n * K);
+  hipLaunchKernelGGL(k_zero, dim3(sg), dim3(kBlock), 0, s, X, (size_t)n * K);
+  int gs = 0;
+  CC_CHECK((spmm_launch<double, double, K, 1>(n, nz, indptr, indices, vals, X, R64, B, pr, nullptr, s, &gs)));
+  hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 1);
+  CC_CHECK(hipGetLastError());
+  std::vector<OuterState> ho(K);
+  std::vector<ColState> hs;
+  int used = 0;
+  for (int sweep = 0; sweep < max_outer; ++sweep) {
+    CC_CHECK(hipMemcpyAsync(ho.data(), ost, K * sizeof(OuterState), hipMemcpyDeviceToHost, s));
+    CC_CHECK(hipStreamSynchronize(s));
+    bool any = false;
+    for (int c = 0; c < K; ++c) any |= ho[c].status == CC_RUNNING;
+    if (!any || used >= maxiter) break;
+    CC_CHECK((cg_run<float, float, K>(n, nz, indptr, indices, vals32, R64, inv_scale, run_mask, b, precond, inner_tol,
+                                      maxiter - used, check_every, s, hs)));
+    int most = 0;
+    for (int c = 0; c < K; ++c)
+      if (ho[c].status == CC_RUNNING && hs[c].iterations > most) most = hs[c].iterations;
+    used += most > 0 ? most : 1;
+    hipLaunchKernelGGL((k_outer_correct<K>), dim3(sg), dim3(kBlock), 0, s, n, X, b.X, ost, run_mask);
+    hipLaunchKernelGGL((k_outer_account<K>), dim3(1), dim3(64), 0, s, ost, b.st, run_mask);
+    CC_CHECK((spmm_launch<double, double, K, 1>(n, nz, indptr, indices, vals, X, R64, B, pr, nullptr, s, &gs)));
+    hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 0);
+    CC_CHECK(hipGetLastError());
+  }
+  CC_CHECK(hipMemcpyAsync(ho.data(), ost, K * sizeof(OuterState), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c) {
+    results[c].iterations = ho[c].inner_total;
+    results[c].status = ho[c].status == CC_CONVERGED ? CC_CONVERGED : CC_MAXITER;
+    results[c].outer = ho[c].outer;
+    results[c].inner_total = ho[c].inner_total;
+    results[c].recursive_rel = ho[c].rel;
+    results[c].true_rel = ho[c].rel;
+  }
+  return 0;
+}
+
+}  // namespace cc
+
+using namespace cc;
+
+// ---------------------------------------------------------------------------
+// C ABI.
+// ---------------------------------------------------------------------------
+
+#define CC_DISPATCH_K(k, CALL)        \
+  switch (k) {                        \
+    case 1: { constexpr int KK = 1; return CALL; }  \
+    case 2: { constexpr int KK = 2; return CALL; }  \
+    case 4: { constexpr int KK = 4; return CALL; }  \
+    case 8: { constexpr int KK = 8; return CALL; }  \
+    case 16: { constexpr int KK = 16; return CALL; } \
+    default: return CC_E_ARG;         \
+  }
+
+extern "C" {
+
+int cc_version(void) { return 10000; }
+
+int cc_profile_enable(int32_t on) {
+  g_prof.on = on != 0;
+  g_prof.pending.clear();
+  g_prof.used = 0;
+  return 0;
+}
+
+int cc_profile_reset(void) {
+  for (int i = 0; i < kProfClasses; ++i) {
+    g_prof.ms[i] = 0.0;
+    g_prof.count[i] = 0;
+  }
+  return 0;
+}
+
+int cc_profile_read(int32_t which, double* total_ms, int64_t* count) {
+  if (which < 0 || which >= kProfClasses || !total_ms || !count) return CC_E_ARG;
+  *total_ms = g_prof.ms[which];
+  *count = g_prof.count[which];
+  return 0;
+}
+
+int cc_device_count(int* count) {
+  if (!count) return CC_E_ARG;
+  return (int)hipGetDeviceCount(count);
+}
+
+int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals, const double* x,
+                    double* y, cc_stream_t stream) {
+  if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !x || !y))) return CC_E_ARG;
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const int nz = nz_lanes<1>(avg_row_len(n, indptr, s));
+  return spmm_launch<double, double, 1, 0>(n, nz, indptr, indices, vals, x, y, nullptr, nullptr, nullptr, s, nullptr);
+}
+
+int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, const float* vals, const float* x,
+                    float* y, cc_stream_t stream) {
+  if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !x || !y))) return CC_E_ARG;
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const int nz = nz_lanes<1>(avg_row_len(n, indptr, s));
+  return spmm_launch<float, float, 1, 0>(n, nz, indptr, indices, vals, x, y, nullptr, nullptr, nullptr, s, nullptr);
+}
+
+int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                    const double* X, double* Y, cc_stream_t stream) {
+  if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !X || !Y))) return CC_E_ARG;
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const double avg = avg_row_len(n, indptr, s);
+  CC_DISPATCH_K(k, (spmm_launch<double, double, KK, 0>(n, nz_lanes<KK>(avg), indptr, indices, vals, X, Y, nullptr,
+                                                       nullptr, nullptr, s, nullptr)));
+}
+
+int cc_jacobi_diag_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals, double* diag,
+                       cc_stream_t stream) {
+  if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !diag))) return CC_E_ARG;
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, indptr, indices, vals,
+                     diag);
+  return (int)hipGetLastError();
+}
+
+int cc_residual_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                    const double* B, const double* X, double* rel, void* workspace, size_t workspace_bytes,
+                    cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !B || !X || !rel || !workspace) return CC_E_ARG;
+  const size_t need = (size_t)n * k * sizeof(double) + (size_t)kMaxParts * 2 * k * sizeof(double) + 512;
+  if (workspace_bytes < need) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  Carve cv{(char*)workspace, 0, workspace_bytes};
+  double* R = cv.take<double>((size_t)n * k);
+  double* part = cv.take<double>((size_t)kMaxParts * 2 * k);
+  const double avg = avg_row_len(n, indptr, s);
+  CC_DISPATCH_K(k, (true_residual<KK>(n, nz_lanes<KK>(avg), indptr, indices, vals, B, X, R, part, s, rel)));
+}
+
+size_t cc_pcg_workspace_bytes(int32_t n, int32_t k, int32_t precision) {
+  if (n <= 0 || k <= 0) return 0;
+  if (precision == 1) {
+    switch (k) {
+      case 1: return mixed_bytes<1>(n);
+      case 2: return mixed_bytes<2>(n);
+      case 4: return mixed_bytes<4>(n);
+      case 8: return mixed_bytes<8>(n);
+      case 16: return mixed_bytes<16>(n);
+      default: return 0;
+    }
+  }
+  return cg_bytes<double>(n, k);
+}
+
+int cc_pcg_jacobi_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                      const double* B, double* X, double tol, int32_t maxiter, int32_t preconditioner,
+                      int32_t check_every, void* workspace, size_t workspace_bytes, cc_pcg_result* results,
+                      double* history, cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !B || !X || !results || !workspace || maxiter < 0) return CC_E_ARG;
+  if (workspace_bytes < cc_pcg_workspace_bytes(n, k, 0)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  CC_DISPATCH_K(k, (pcg_f64<KK>(n, indptr, indices, vals, B, X, tol, maxiter, preconditioner, check_every, workspace,
+                                workspace_bytes, results, history, s)));
+}
+
+int cc_pcg_jacobi_mixed(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                        const float* vals32, const double* B, double* X, double tol, int32_t maxiter,
+                        double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
+                        void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !vals32 || !B || !X || !results || !workspace || maxiter < 0)
+    return CC_E_ARG;
+  if (workspace_bytes < cc_pcg_workspace_bytes(n, k, 1)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  CC_DISPATCH_K(k, (pcg_mixed<KK>(n, indptr, indices, vals, vals32, B, X, tol, maxiter, inner_tol, max_outer,
+                                  preconditioner, check_every, workspace, workspace_bytes, results, s)));
+}
+
+}  // extern "C"

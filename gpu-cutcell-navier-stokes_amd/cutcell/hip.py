@@ -1,0 +1,309 @@
+"""ctypes binding of the gfx950 HIP library (``libcutcell_hip.so``).
+
+PyTorch-ROCm is used only to own device memory and to name the current HIP
+stream; every computation runs in the hand-written kernels behind the C ABI
+declared in ``include/cutcell_hip.h``. There is no CPU code path here: when
+the library or a GPU is missing the calls raise ``HipUnavailable``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "libcutcell_hip.so"
+
+CC_RUNNING, CC_CONVERGED, CC_BREAKDOWN, CC_MAXITER = 0, 1, 2, 3
+STATUS_MESSAGE = {
+    CC_CONVERGED: "converged",
+    CC_BREAKDOWN: "CG breakdown: non-positive curvature",
+    CC_MAXITER: "maximum iterations reached",
+    CC_RUNNING: "maximum iterations reached",
+}
+SUPPORTED_K = (1, 2, 4, 8, 16)
+
+
+class HipUnavailable(RuntimeError):
+    """The HIP library or a GPU is not available."""
+
+
+class HipError(RuntimeError):
+    """A library call returned a nonzero status."""
+
+
+class PcgResult(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("outer", ctypes.c_int32),
+        ("inner_total", ctypes.c_int32),
+        ("recursive_rel", ctypes.c_double),
+        ("true_rel", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_F64 = ctypes.c_double
+_SZ = ctypes.c_size_t
+
+_SIGNATURES = {
+    "cc_version": ([], ctypes.c_int),
+    "cc_device_count": ([_P], ctypes.c_int),
+    "cc_profile_enable": ([_I32], ctypes.c_int),
+    "cc_profile_reset": ([], ctypes.c_int),
+    "cc_profile_read": ([_I32, _P, _P], ctypes.c_int),
+    "cc_spmv_csr_f64": ([_I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_spmv_csr_f32": ([_I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_spmm_csr_f64": ([_I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_jacobi_diag_f64": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_residual_f64": ([_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _SZ, _P], ctypes.c_int),
+    "cc_pcg_workspace_bytes": ([_I32, _I32, _I32], _SZ),
+    "cc_pcg_jacobi_f64": ([_I32, _I32, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int),
+    "cc_pcg_jacobi_mixed": (
+        [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P],
+        ctypes.c_int,
+    ),
+}
+
+
+def declared_symbols() -> list[str]:
+    """Every entry point the Python side binds (all declared in the C header)."""
+    return sorted(_SIGNATURES)
+
+
+def load_library(path: Path | str | None = None) -> ctypes.CDLL:
+    """Load the library and declare its signatures (no GPU needed to load)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path is not None else LIB_PATH
+    if not p.exists():
+        raise HipUnavailable(f"HIP library not built: {p} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(str(p))
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _torch():
+    import torch
+
+    return torch
+
+
+def require_gpu():
+    torch = _torch()
+    if not torch.cuda.is_available():
+        raise HipUnavailable("no ROCm GPU is visible to this process")
+    return load_library()
+
+
+def _check(status: int, what: str) -> None:
+    if status != 0:
+        raise HipError(f"{what} failed with status {status}")
+
+
+def _ptr(t) -> int:
+    return int(t.data_ptr())
+
+
+def _stream() -> int:
+    return int(_torch().cuda.current_stream().cuda_stream)
+
+
+@dataclass
+class DeviceCSR:
+    """A CSR matrix resident in HBM (int32 index arrays, float64 values)."""
+
+    n: int
+    nnz: int
+    indptr: object
+    indices: object
+    vals: object
+    vals32: object | None = None
+
+    @staticmethod
+    def from_scipy(A, device: str = "cuda", with_f32: bool = False) -> "DeviceCSR":
+        torch = _torch()
+        A = A.tocsr()
+        if A.shape[0] != A.shape[1]:
+            raise ValueError("a square matrix is required")
+        if A.nnz >= 2**31:
+            raise ValueError("nnz must fit in int32")
+        indptr = torch.from_numpy(np.ascontiguousarray(A.indptr, dtype=np.int32)).to(device)
+        indices = torch.from_numpy(np.ascontiguousarray(A.indices, dtype=np.int32)).to(device)
+        vals = torch.from_numpy(np.ascontiguousarray(A.data, dtype=np.float64)).to(device)
+        d = DeviceCSR(int(A.shape[0]), int(A.nnz), indptr, indices, vals)
+        if with_f32:
+            d.ensure_f32()
+        return d
+
+    def ensure_f32(self):
+        if self.vals32 is None:
+            self.vals32 = self.vals.to(_torch().float32)
+        return self.vals32
+
+    def bytes(self, value_bytes: int = 8) -> int:
+        return int(self.nnz * (value_bytes + 4) + 4 * (self.n + 1))
+
+
+def spmv(A: DeviceCSR, x, y=None):
+    lib = require_gpu()
+    torch = _torch()
+    if x.dtype == torch.float64:
+        y = torch.empty_like(x) if y is None else y
+        _check(lib.cc_spmv_csr_f64(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(x), _ptr(y), _stream()), "spmv")
+    elif x.dtype == torch.float32:
+        y = torch.empty_like(x) if y is None else y
+        v32 = A.ensure_f32()
+        _check(lib.cc_spmv_csr_f32(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(v32), _ptr(x), _ptr(y), _stream()), "spmv")
+    else:
+        raise TypeError("x must be float64 or float32")
+    return y
+
+
+def spmm(A: DeviceCSR, X, Y=None):
+    """``Y = A X`` for a row-major ``[n, k]`` float64 block, k in {1,2,4,8,16}."""
+    lib = require_gpu()
+    torch = _torch()
+    k = X.shape[1]
+    if k not in SUPPORTED_K:
+        raise ValueError(f"k must be one of {SUPPORTED_K}")
+    X = X.contiguous()
+    Y = torch.empty_like(X) if Y is None else Y
+    _check(
+        lib.cc_spmm_csr_f64(A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(X), _ptr(Y), _stream()), "spmm"
+    )
+    return Y
+
+
+def jacobi_diag(A: DeviceCSR):
+    lib = require_gpu()
+    d = _torch().empty(A.n, dtype=_torch().float64, device=A.vals.device)
+    _check(lib.cc_jacobi_diag_f64(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(d), _stream()), "diag")
+    return d
+
+
+def residual(A: DeviceCSR, B, X) -> np.ndarray:
+    """FP64 ``||B - A X|| / ||B||`` per column (device computation)."""
+    lib = require_gpu()
+    torch = _torch()
+    B2 = B.reshape(A.n, -1).contiguous()
+    X2 = X.reshape(A.n, -1).contiguous()
+    k = B2.shape[1]
+    if k not in SUPPORTED_K:
+        raise ValueError(f"k must be one of {SUPPORTED_K}")
+    ws_bytes = A.n * k * 8 + 1024 * 2 * k * 8 + 4096
+    ws = torch.empty(ws_bytes, dtype=torch.uint8, device=A.vals.device)
+    rel = np.zeros(k)
+    _check(
+        lib.cc_residual_f64(
+            A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(B2), _ptr(X2),
+            rel.ctypes.data, _ptr(ws), ws_bytes, _stream(),
+        ),
+        "residual",
+    )
+    return rel
+
+
+PROFILE_CLASSES = {"spmm": 0, "update": 1, "direction": 2, "batched": 3}
+
+
+def profile_enable(on: bool = True) -> None:
+    lib = load_library()
+    lib.cc_profile_enable(1 if on else 0)
+    lib.cc_profile_reset()
+
+
+def profile_read(which: str) -> tuple[float, int]:
+    """(summed milliseconds, launch count) of a timed kernel class."""
+    lib = load_library()
+    ms = ctypes.c_double()
+    cnt = ctypes.c_int64()
+    _check(lib.cc_profile_read(PROFILE_CLASSES[which], ctypes.addressof(ms), ctypes.addressof(cnt)), "profile_read")
+    return float(ms.value), int(cnt.value)
+
+
+class Workspace:
+    """A reusable device workspace grown on demand."""
+
+    def __init__(self):
+        self.buf = None
+
+    def get(self, nbytes: int, device):
+        torch = _torch()
+        if self.buf is None or self.buf.numel() < nbytes or self.buf.device != device:
+            self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+        return self.buf
+
+
+_default_ws = Workspace()
+
+
+def pcg(
+    A: DeviceCSR,
+    B,
+    *,
+    tol: float = 1.0e-8,
+    maxiter: int = 5000,
+    preconditioner: str = "jacobi",
+    precision: str = "fp64",
+    inner_tol: float = 1.0e-4,
+    max_outer: int = 12,
+    check_every: int = 16,
+    X=None,
+    workspace: Workspace | None = None,
+    history: bool = False,
+):
+    """Device Jacobi-PCG on ``k`` right-hand sides (``B`` is ``[n]`` or ``[n, k]``).
+
+    Returns ``(X, results)`` where ``results`` is a list of ``PcgResult``
+    (and, with ``history=True`` for ``precision="fp64"``, a ``[maxiter+1, k]``
+    array of ``||r_j||/||b||`` per iteration, NaN past each column's stop).
+    ``precision="mixed"`` runs float32 inner PCG under float64 refinement.
+    """
+    lib = require_gpu()
+    torch = _torch()
+    squeeze = B.dim() == 1
+    B2 = B.reshape(A.n, -1).to(torch.float64).contiguous()
+    k = B2.shape[1]
+    if k not in SUPPORTED_K:
+        raise ValueError(f"k must be one of {SUPPORTED_K}")
+    X2 = torch.empty_like(B2) if X is None else X.reshape(A.n, k)
+    prec = {"none": 0, "jacobi": 1}[preconditioner]
+    mode = {"fp64": 0, "mixed": 1}[precision]
+    nbytes = int(lib.cc_pcg_workspace_bytes(A.n, k, mode))
+    ws = (workspace or _default_ws).get(nbytes, A.vals.device)
+    res = (PcgResult * k)()
+    hist = None
+    if mode == 0:
+        if history:
+            hist = torch.full((int(maxiter) + 1, k), float("nan"), dtype=torch.float64, device=A.vals.device)
+        st = lib.cc_pcg_jacobi_f64(
+            A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(B2), _ptr(X2),
+            float(tol), int(maxiter), prec, int(check_every), _ptr(ws), nbytes, ctypes.addressof(res),
+            _ptr(hist) if hist is not None else None, _stream(),
+        )
+    else:
+        v32 = A.ensure_f32()
+        st = lib.cc_pcg_jacobi_mixed(
+            A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(v32), _ptr(B2), _ptr(X2),
+            float(tol), int(maxiter), float(inner_tol), int(max_outer), prec, int(check_every),
+            _ptr(ws), nbytes, ctypes.addressof(res), _stream(),
+        )
+    _check(st, "pcg")
+    out = X2.reshape(-1) if squeeze else X2
+    if history:
+        return out, list(res), (hist.cpu().numpy() if hist is not None else None)
+    return out, list(res)

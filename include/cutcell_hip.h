@@ -1,0 +1,129 @@
+/*
+ * cutcell_hip.h - C ABI of the MI355X (gfx950) cut-cell PCG library.
+ *
+ * All entry points are extern "C", take plain device pointers and sizes, run
+ * asynchronously on the given HIP stream unless stated otherwise, and return
+ * an int status: 0 on success, a negative CC_E* code on a bad argument, or
+ * the positive hipError_t of a failed HIP call. Nothing throws across the ABI.
+ *
+ * Matrix format: CSR with int32 row pointers (n+1) and int32 column indices,
+ * values in float64 (or float32 where the name says so). Multi-RHS blocks are
+ * row-major [n][k] (the k right-hand sides of one row are contiguous), k in
+ * {1, 2, 4, 8, 16}.
+ *
+ * Reference interfaces replaced (reference repository, file:line):
+ *  - cc_pcg_jacobi_f64 / cc_pcg_jacobi_mixed: the Jacobi-PCG loop of
+ *    solve_gpu_cg, code/fast_solver_backends.py:194-289 (its CuPy/cuSPARSE
+ *    loop, lines 233-274), reached through solve_spd(backend="gpu"),
+ *    code/fast_solver_backends.py:292-325;
+ *  - cc_spmv_csr_* / cc_spmm_csr_*: the `A_gpu @ p` product of that loop,
+ *    code/fast_solver_backends.py:250 (cuSPARSE csrmv through cupyx);
+ *  - cc_residual_f64: relative_residual, code/fast_solver_backends.py:63-71;
+ *  - cc_jacobi_diag_f64: the `A_gpu.diagonal()` of line 237 and the
+ *    zero-diagonal-to-identity rule of line 238.
+ */
+#ifndef CUTCELL_HIP_H
+#define CUTCELL_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* cc_stream_t; /* a hipStream_t; NULL is the null stream */
+
+enum {
+  CC_OK = 0,
+  CC_E_ARG = -1,       /* bad size / null pointer / unsupported k */
+  CC_E_WORKSPACE = -2, /* workspace too small */
+  CC_E_NODEVICE = -3
+};
+
+/* Per-column solve status (the `message` of the reference report). */
+enum {
+  CC_RUNNING = 0,
+  CC_CONVERGED = 1,   /* "converged" */
+  CC_BREAKDOWN = 2,   /* "CG breakdown: non-positive curvature" */
+  CC_MAXITER = 3      /* "maximum iterations reached" */
+};
+
+/* One right-hand side's outcome, written to host memory by the solvers. */
+typedef struct cc_pcg_result {
+  int32_t iterations;      /* CG iterations, counted as the reference GPU loop does */
+  int32_t status;          /* CC_CONVERGED / CC_BREAKDOWN / CC_MAXITER */
+  int32_t outer;           /* refinement sweeps (mixed precision), else 1 */
+  int32_t inner_total;     /* total inner CG iterations (mixed precision) */
+  double recursive_rel;    /* ||r_k|| / ||b|| of the recurrence at exit */
+  double true_rel;         /* ||b - A x|| / ||b|| recomputed in FP64 on the device */
+} cc_pcg_result;
+
+/* Library / device information. */
+int cc_version(void);
+int cc_device_count(int* count);
+
+/* Optional event timing of the solver kernels (no reference counterpart: it
+ * serves the benchmark's live roofline figure). When enabled, every launch of
+ * a timed kernel class inside the solvers is bracketed by two hipEvents on its
+ * stream; cc_profile_read returns the summed milliseconds and launch count of
+ * a class: 0 = SpMV/SpMM of the CG loop, 1 = CG update, 2 = CG direction,
+ * 3 = batched persistent solve. Process-wide and not thread-safe. */
+int cc_profile_enable(int32_t on);
+int cc_profile_reset(void);
+int cc_profile_read(int32_t which, double* total_ms, int64_t* count);
+
+/* y = A x, float64 (replaces the cuSPARSE product A_gpu @ p). */
+int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                    const double* x, double* y, cc_stream_t stream);
+/* y = A x with float32 values and vectors (the inner product of the mixed solve). */
+int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, const float* vals,
+                    const float* x, float* y, cc_stream_t stream);
+/* Y = A X for k right-hand sides, X and Y row-major [n][k]. */
+int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                    const double* X, double* Y, cc_stream_t stream);
+
+/* d[i] = A[i,i] (0 where the row stores no diagonal entry). */
+int cc_jacobi_diag_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                       double* diag, cc_stream_t stream);
+
+/* Relative residuals ||B[:,c] - A X[:,c]|| / ||B[:,c]|| (||B[:,c]|| = 0 -> 1) for
+ * the k columns, written to host memory rel[k]. Synchronises the stream. */
+int cc_residual_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                    const double* B, const double* X, double* rel, void* workspace, size_t workspace_bytes,
+                    cc_stream_t stream);
+
+/* Workspace needed by the solvers for n rows and k right-hand sides.
+ * precision: 0 = float64 solve, 1 = mixed (float32 inner, float64 outer). */
+size_t cc_pcg_workspace_bytes(int32_t n, int32_t k, int32_t precision);
+
+/* Jacobi-preconditioned CG, float64 throughout, k independent right-hand
+ * sides sharing A (replaces solve_gpu_cg's loop). X (row-major [n][k]) is
+ * overwritten with the solutions; x0 = 0. Per column the recurrence, the
+ * stopping test (||r_k||/||b|| <= tol, ||b|| = 0 -> 1), the breakdown test
+ * (p.Ap <= 0 or non-finite) and the iteration count follow
+ * code/fast_solver_backends.py:233-274. preconditioner: 0 = none, 1 = Jacobi
+ * (zero diagonal -> identity). check_every: iterations between host polls of
+ * the device status. history (device memory, may be NULL) receives
+ * ||r_j||/||b|| of every iteration j = 0..iterations as history[j*k + c]
+ * (size (maxiter+1)*k). Synchronises the stream before returning; results[k]
+ * is host memory. */
+int cc_pcg_jacobi_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                      const double* B, double* X, double tol, int32_t maxiter, int32_t preconditioner,
+                      int32_t check_every, void* workspace, size_t workspace_bytes, cc_pcg_result* results,
+                      double* history, cc_stream_t stream);
+
+/* Mixed precision: float32 values (vals32, same pattern) and vectors in the
+ * inner Jacobi-PCG, float64 residual recomputation, dot products and outer
+ * correction (iterative refinement). inner_tol is the inner solve's relative
+ * tolerance; max_outer bounds the refinement sweeps. */
+int cc_pcg_jacobi_mixed(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                        const float* vals32, const double* B, double* X, double tol, int32_t maxiter,
+                        double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
+                        void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUTCELL_HIP_H */

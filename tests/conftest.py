@@ -1,0 +1,19 @@
+"""Test configuration: import paths and the ``gpu`` marker.
+
+Like the reference's ``code/tests/conftest.py`` this puts the solver package
+directory on ``sys.path`` so tests import ``fast_solver_backends`` and
+``cutcell`` directly; the repository root is added for the ``oracle`` package.
+"""
+
+import sys
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "gpu-cutcell-navier-stokes_amd"
+for p in (str(PKG), str(ROOT)):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (MI355X, gfx950)")

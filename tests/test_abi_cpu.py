@@ -1,0 +1,63 @@
+"""CPU checks of the C ABI library: it loads and exports every declared symbol."""
+
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _header_symbols():
+    text = (ROOT / "include" / "cutcell_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cc_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from cutcell import hip
+
+    lib = hip.load_library()
+    declared = _header_symbols()
+    assert declared, "no entry points parsed from the header"
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert set(hip.declared_symbols()) <= set(declared)
+    assert lib.cc_version() >= 10000
+
+
+def test_workspace_sizes_are_monotone():
+    from cutcell import hip
+
+    lib = hip.load_library()
+    a = lib.cc_pcg_workspace_bytes(1000, 1, 0)
+    b = lib.cc_pcg_workspace_bytes(2000, 1, 0)
+    c = lib.cc_pcg_workspace_bytes(1000, 16, 0)
+    m = lib.cc_pcg_workspace_bytes(1000, 16, 1)
+    assert 0 < a < b and a < c and m > 0
+    assert lib.cc_pcg_workspace_bytes(1000, 3, 0) == lib.cc_pcg_workspace_bytes(1000, 3, 0)
+    assert lib.cc_pcg_workspace_bytes(1000, 3, 1) == 0  # k must be 1, 2, 4, 8 or 16
+
+
+def test_bad_arguments_are_rejected_without_a_gpu():
+    from cutcell import hip
+
+    lib = hip.load_library()
+    assert lib.cc_spmv_csr_f64(-1, None, None, None, None, None, None) == -1
+    assert lib.cc_pcg_jacobi_f64(0, 1, None, None, None, None, None, 1e-8, 10, 1, 16, None, 0, None, None, None) == -1
+
+
+def test_gpu_backend_without_gpu_reports_unavailable():
+    torch = pytest.importorskip("torch")
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    import numpy as np
+    import scipy.sparse as sp
+
+    from fast_solver_backends import solve_gpu_cg, solve_spd
+
+    A = sp.diags([4.0, 5.0, 6.0], format="csr")
+    x, rep = solve_gpu_cg(A, np.ones(3))
+    assert x is None and not rep.success and rep.message.startswith("HIP backend unavailable")
+    x, rep = solve_spd(A, np.ones(3), backend="auto", tol=1e-10, maxiter=20)
+    assert rep.backend == "cpu" and rep.success and rep.message.startswith("GPU skipped/fell back")

@@ -1,0 +1,215 @@
+"""Pins of the CPU oracle (no GPU).
+
+The oracle restates the reference GPU loop (code/fast_solver_backends.py:233-274)
+and SciPy CG as called by solve_cpu_cg (:90-132). Each check here is chosen so a
+plausible restatement mistake (a dropped term, a wrong stopping inequality, a
+wrong zero-diagonal rule) fails it.
+"""
+
+import csv
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import oracle
+from cutcell.oblique2d import sliver_system
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def _scipy_count(A, b, tol=1e-8, maxiter=5000):
+    d = A.diagonal()
+    inv = np.zeros_like(d)
+    m = np.abs(d) > 0
+    inv[m] = 1 / d[m]
+    M = spla.LinearOperator(A.shape, matvec=lambda x: inv * x)
+    cnt = [0]
+    x, info = spla.cg(A, b, rtol=tol, atol=0.0, maxiter=maxiter, M=M, callback=lambda _: cnt.__setitem__(0, cnt[0] + 1))
+    return x, cnt[0], info
+
+
+def test_diagonal_jacobi_converges_in_one_step():
+    # D^-1 A = I: the first search direction is the exact solution.
+    A = sp.diags([2.0, 3.0, 4.0, 5.0], format="csr")
+    b = np.array([1.0, -2.0, 0.5, 4.0])
+    x, it, st, rel = oracle.pcg_gpu_loop(A, b, tol=1e-12, maxiter=20)
+    assert (it, st) == (1, oracle.CONVERGED)
+    np.testing.assert_allclose(x, b / A.diagonal(), rtol=1e-15)
+    x2, it2, st2 = oracle.pcg_scipy(A, b, tol=1e-12, maxiter=20)
+    assert (it2, st2) == (1, oracle.CONVERGED)
+
+
+def test_hand_computed_first_iterate():
+    # A = [[2,1],[1,3]], b = [1,2], no preconditioner: r0 = p0 = b,
+    # Ap0 = [4,7], alpha = 5/18, x1 = 5/18 * [1,2].
+    A = sp.csr_matrix(np.array([[2.0, 1.0], [1.0, 3.0]]))
+    b = np.array([1.0, 2.0])
+    x, it, st, rel = oracle.pcg_gpu_loop(A, b, tol=1e-300, maxiter=1, preconditioner="none")
+    np.testing.assert_allclose(x, [5 / 18, 10 / 18], rtol=1e-15)
+    r1 = b - (5 / 18) * np.array([4.0, 7.0])
+    assert it == 1 and st == oracle.MAXITER
+    assert rel == pytest.approx(np.linalg.norm(r1) / np.linalg.norm(b), rel=1e-14)
+    # Two steps of CG solve a 2x2 SPD system exactly.
+    x, it, st, rel = oracle.pcg_gpu_loop(A, b, tol=1e-12, maxiter=5, preconditioner="none")
+    np.testing.assert_allclose(x, np.linalg.solve(A.toarray(), b), rtol=1e-13)
+    assert it == 2
+
+
+def test_breakdown_and_zero_rhs_and_maxiter():
+    n = 5
+    A = -sp.eye(n, format="csr")
+    _, it, st, _ = oracle.pcg_gpu_loop(A, np.ones(n))
+    assert (it, st) == (1, oracle.BREAKDOWN)
+    # b = 0 in the reference GPU loop: p = 0, p.Ap = 0 -> breakdown at k = 1.
+    _, it, st, _ = oracle.pcg_gpu_loop(sp.eye(n, format="csr"), np.zeros(n))
+    assert (it, st) == (1, oracle.BREAKDOWN)
+    # SciPy returns at once for b = 0.
+    _, it, st = oracle.pcg_scipy(sp.eye(n, format="csr"), np.zeros(n))
+    assert (it, st) == (0, oracle.CONVERGED)
+    s = sliver_system(8, 4, 1e-3)
+    _, it, st, _ = oracle.pcg_gpu_loop(s.A, s.b, maxiter=3)
+    assert (it, st) == (3, oracle.MAXITER)
+
+
+def test_zero_diagonal_rules_differ_between_loops():
+    # GPU loop: zero diagonal -> identity. By hand: z0 = [1, .5], rho = 1.5,
+    # Ap = [0, 1], alpha = 3, r1 = [1, -2], z1 = [1, -1], beta = 2,
+    # p1 = [3, 0], p1.Ap1 = 0 -> breakdown at k = 2.
+    A = sp.csr_matrix(np.array([[0.0, 0.0], [0.0, 2.0]]))
+    b = np.array([1.0, 1.0])
+    x, it, st, _ = oracle.pcg_gpu_loop(A, b, maxiter=5)
+    assert (it, st) == (2, oracle.BREAKDOWN)
+    np.testing.assert_array_equal(x, [3.0, 1.5])
+    # SciPy Jacobi: zero diagonal -> 0, so z1 = 0 and the second step is 0/0.
+    x2, it2, st2 = oracle.pcg_scipy(A, b, maxiter=5)
+    with np.errstate(all="ignore"):
+        xs, cnt, info = _scipy_count(A, b, maxiter=5)
+    np.testing.assert_array_equal(x2, xs)
+    assert it2 == cnt
+
+
+def test_numpy_and_c_restatements_agree():
+    s = sliver_system(12, 4, 1e-2)
+    x1, it1, st1, rel1 = oracle.pcg_gpu_loop(s.A, s.b)
+    x2, it2, st2, rel2 = oracle.pcg_gpu_loop_py(s.A, s.b)
+    # Summation order differs (NumPy/BLAS vs left to right), so the stopping
+    # iteration may move by one when ||r_k|| sits on the threshold.
+    assert st1 == st2 == oracle.CONVERGED and abs(it1 - it2) <= 1
+    assert rel1 <= 1e-8 and rel2 <= 1e-8
+    np.testing.assert_allclose(x1, x2, rtol=1e-6, atol=1e-9)
+
+
+STRESS_ROWS = list(csv.DictReader((GOLDEN / "reference_fast_solver_stress.csv").open()))
+
+
+@pytest.mark.parametrize("row", STRESS_ROWS, ids=[f"N{r['n_elements']}-eps{r['epsilon']}" for r in STRESS_ROWS])
+def test_stress_systems_match_reference_csv(row):
+    """Rebuilt sliver systems against the reference's committed sweep.
+
+    Active count, reduced size and nnz are integers and must match exactly.
+    The stopping iteration of SciPy on the rebuilt system may move by up to
+    two: the sparse assembly sums the same quadrature in a different order
+    than the reference's dense one, and on these kappa ~ 5e3 systems the
+    residual hovers near 1e-8 for a few iterations.
+    """
+    N, eps = int(row["n_elements"]), float(row["epsilon"])
+    s = sliver_system(N, 4, eps)
+    assert len(s.active) == int(row["n_active"])
+    assert s.A.shape[0] == int(row["n_reduced"])
+    assert s.A.nnz == int(row["nnz"])
+    x_sp, it_scipy, info = _scipy_count(s.A, s.b)
+    assert info == 0 and abs(it_scipy - int(row["iterations"])) <= 2
+    _, it, st = oracle.pcg_scipy(s.A, s.b)
+    assert st == oracle.CONVERGED and abs(it - it_scipy) <= 3
+    x, it_gpu, st_gpu, rel = oracle.pcg_gpu_loop(s.A, s.b)
+    assert st_gpu == oracle.CONVERGED and abs(it_gpu - it_scipy) <= 3
+    for xx in (x, x_sp):
+        assert np.linalg.norm(s.A @ xx - s.b) / np.linalg.norm(s.b) < 1e-8 * 10
+
+
+def _well_conditioned(n=60, seed=3):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Q @ np.diag(np.linspace(1.0, 30.0, n)) @ Q.T
+    return sp.csr_matrix(0.5 * (A + A.T)), rng.standard_normal(n)
+
+
+def test_scipy_semantics_exact_on_well_conditioned_system():
+    # Fast, clean convergence leaves no room for rounding to move the stop.
+    A, b = _well_conditioned()
+    xs, cnt, info = _scipy_count(A, b, tol=1e-10)
+    x, it, st = oracle.pcg_scipy(A, b, tol=1e-10)
+    assert (it, st) == (cnt, oracle.CONVERGED) and info == 0
+    np.testing.assert_allclose(x, xs, rtol=1e-9)
+    # GPU-loop semantics: same recurrence, non-strict test on ||r_k||.
+    xg, itg, stg, rel = oracle.pcg_gpu_loop(A, b, tol=1e-10)
+    assert stg == oracle.CONVERGED and itg == cnt
+    np.testing.assert_allclose(xg, xs, rtol=1e-9)
+
+
+def test_solution_against_superlu():
+    s = sliver_system(16, 4, 1e-4)
+    x, it, st, rel = oracle.pcg_gpu_loop(s.A, s.b, tol=1e-12)
+    xs = spla.splu(s.A.tocsc()).solve(s.b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+
+
+def test_mixed_refinement_reaches_fp64_tolerance():
+    s = sliver_system(16, 4, 1e-3)
+    x, total, sweeps, st, rel = oracle.mixed_refinement_py(s.A, s.b, tol=1e-8, inner_tol=1e-4)
+    assert st == oracle.CONVERGED and rel <= 1e-8
+    assert 2 <= sweeps <= 4
+    xs = spla.splu(s.A.tocsc()).solve(s.b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-6
+
+
+def test_summation_order_moves_the_stop_by_rounding_only():
+    """A row/column permutation changes only the summation order; the oracle's
+    stopping iteration may move, but the trajectories satisfy the parity
+    criterion used for the GPU."""
+    from parity import check_trajectory
+
+    for N, eps in [(8, 0.1), (16, 1e-2), (32, 0.1)]:
+        s = sliver_system(N, 4, eps)
+        _, it, st, _, h = oracle.pcg_gpu_loop(s.A, s.b, history=True)
+        for seed in range(3):
+            perm = np.random.default_rng(seed).permutation(s.A.shape[0])
+            _, it2, st2, _, h2 = oracle.pcg_gpu_loop(s.A[perm][:, perm], s.b[perm], history=True)
+            assert st == st2 == oracle.CONVERGED
+            check_trajectory(h, it, h2, it2, 1e-8)
+
+
+def test_parity_criterion_rejects_a_wrong_recurrence():
+    """A planted mistake (beta from the unpreconditioned r.r) must fail the check."""
+    from parity import check_trajectory
+
+    s = sliver_system(16, 4, 1e-2)
+    _, it, _, _, h = oracle.pcg_gpu_loop(s.A, s.b, history=True)
+    A = s.A.tocsr()
+    d = A.diagonal()
+    x = np.zeros_like(s.b)
+    r = s.b.copy()
+    z = r / d
+    p = z.copy()
+    rho = r @ z
+    bn = np.linalg.norm(s.b)
+    hb = [np.linalg.norm(r) / bn]
+    itb = 0
+    for k in range(1, 5001):
+        q = A @ p
+        a = rho / (p @ q)
+        x += a * p
+        r -= a * q
+        hb.append(np.linalg.norm(r) / bn)
+        itb = k
+        if hb[-1] <= 1e-8:
+            break
+        z = r / d
+        rho_new = r @ r  # wrong: should be r.z
+        p = z + (rho_new / (rho if k > 1 else r @ r)) * p
+        rho = rho_new
+    with pytest.raises(AssertionError):
+        check_trajectory(h, it, np.array(hb), itb, 1e-8)
