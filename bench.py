@@ -80,12 +80,24 @@ def spmm_bytes(n: int, nnz: int, k: int, value_bytes: int, vec_bytes: int) -> in
     return int(nnz * (value_bytes + 4) + 4 * (n + 1) + 2 * k * n * vec_bytes)
 
 
-def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int):
+def sig(v, digits: int = 4):
+    """Round to significant digits (keeps tiny residuals readable)."""
+    if isinstance(v, float) and v != 0.0 and np.isfinite(v):
+        return float(f"{v:.{digits}g}")
+    return v
+
+
+def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int, tune: bool = True):
     kw = dict(tol=TOL, maxiter=20000, precision=method)
     if method == "mixed":
         kw.update(inner_tol=1e-4, max_outer=12)
     ws = hip.Workspace()
     X = torch.empty_like(Bd)
+    tuned = None
+    if tune:
+        from cutcell.autotune import spmm_config
+
+        tuned = spmm_config(Ad, Bd.shape[1], method)
     for _ in range(warmup):
         hip.pcg(Ad, Bd, X=X, workspace=ws, **kw)
     torch.cuda.synchronize()
@@ -114,6 +126,7 @@ def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int):
         spmm_launches=spmm_cnt,
         update_avg_us=1e3 * upd_ms / max(upd_cnt, 1),
         direction_avg_us=1e3 * dir_ms / max(dir_cnt, 1),
+        spmm_config=tuned,
         X=X,
     )
 
@@ -131,6 +144,52 @@ def cpu_baseline(system, b: np.ndarray, budget_s: float = 30.0):
                 iterations=it, converged=st == oracle.CONVERGED)
 
 
+def batched_section(hip, torch, steps: int, warmup: int, seed: int = 0):
+    """Batched uneven 2D systems in one persistent launch (reported beside the headline)."""
+    from cutcell.batched import BatchedSystems
+    from cutcell.workloads import c3_batch
+
+    t0 = time.perf_counter()
+    systems = c3_batch(seed=seed)
+    build_s = time.perf_counter() - t0
+    bs = BatchedSystems([s.A for s in systems])
+    B = bs.concat([s.b for s in systems])
+    X = torch.empty_like(B)
+    out = {"systems": bs.nsys, "total_unknowns": bs.total_n, "total_nnz": bs.total_nnz, "n_max": bs.n_max,
+           "build_s": sig(build_s)}
+    for method in ("fp64", "mixed"):
+        cfg = bs.solve_device(B, X, tol=TOL, maxiter=20000, precision=method)  # tunes on first use
+        for _ in range(warmup):
+            bs.solve_device(B, X, tol=TOL, maxiter=20000, precision=method)
+        torch.cuda.synchronize()
+        hip.profile_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            bs.solve_device(B, X, tol=TOL, maxiter=20000, precision=method)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        k_ms, k_cnt = hip.profile_read("batched")
+        hip.profile_enable(False)
+        res = bs.read_results()
+        iters = np.array([r.iterations for r in res])
+        ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= TOL * (1.0 if method == "mixed" else 10.0) for r in res)
+        vb = 8 if method == "fp64" else 4
+        # Matrix bytes streamed over the launch: every system's CSR once per iteration.
+        sys_bytes = bs.nnz * (vb + 4) + 4 * (bs.n + 1)
+        launch_bytes = float((sys_bytes * iters).sum())
+        avg_ms = k_ms / max(k_cnt, 1)
+        out[method] = {
+            "solves_per_s": sig(bs.nsys * steps / dt),
+            "ms_per_launch": sig(avg_ms),
+            "converged": ok,
+            "iterations_range": [int(iters.min()), int(iters.max())],
+            "matrix_GBps_over_launch": sig(launch_bytes / (avg_ms * 1e-3) / 1e9),
+            "launch": cfg,
+        }
+        log(f"batched {method}: {out[method]}")
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,6 +200,8 @@ def main(argv=None) -> int:
     ap.add_argument("--methods", default="fp64,mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-systems section")
+    ap.add_argument("--no-tune", action="store_true", help="use the default SpMM launch shape")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -173,7 +234,7 @@ def main(argv=None) -> int:
     runs = {}
     for m in args.methods.split(","):
         log(f"timing {m}: warmup={args.warmup} steps={args.steps}")
-        runs[m] = time_method(hip, torch, Ad, Bd, m, args.steps, args.warmup)
+        runs[m] = time_method(hip, torch, Ad, Bd, m, args.steps, args.warmup, tune=not args.no_tune)
         r = runs[m]
         log(f"{m}: {r['solves_per_s']:.1f} solves/s, {r['ms_per_step']:.1f} ms/step, converged={r['converged']}, "
             f"iters {min(r['iterations'])}-{max(r['iterations'])}, spmm {r['spmm_avg_us']:.1f} us")
@@ -197,6 +258,11 @@ def main(argv=None) -> int:
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
                     kernel=f"k_spmm<{'double' if value_bytes == 8 else 'float'},k=16>",
                     bytes_per_launch=traffic_alg, avg_us=round(head["spmm_avg_us"], 2))
+
+    batched = None
+    if not args.no_batched:
+        log("batched section: 241 uneven 2D systems in one persistent launch ...")
+        batched = batched_section(hip, torch, max(1, args.steps), args.warmup)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -226,7 +292,7 @@ def main(argv=None) -> int:
             "nrhs": NRHS,
         },
         "methods": {
-            m: {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items() if k not in ("X", "iterations", "outer")}
+            m: {k: sig(v) for k, v in r.items() if k not in ("X", "iterations", "outer")}
             | {"iterations_range": [min(r["iterations"]), max(r["iterations"])],
                "outer_range": [min(r["outer"]), max(r["outer"])],
                "error_vs_tight_fp64": errors.get(m)}
@@ -234,6 +300,7 @@ def main(argv=None) -> int:
         },
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched_uneven_2d": batched,
     }
     print(json.dumps(out), flush=True)
     return 0

@@ -18,13 +18,14 @@ constexpr int kProfDirection = 2;
 constexpr int kProfBatched = 3;
 constexpr int kProfClasses = 8;
 
-// Per right-hand-side state of one CG recurrence (device memory).
-// rho is double-buffered by iteration parity so that the block that publishes
-// rho_{k+1} never races the blocks still reading rho_k.
+// Per right-hand-side state of one CG recurrence (device memory). Only the
+// one-block finalize kernels write it; the streaming kernels read it.
 struct ColState {
-  double rho[2];
+  double rho;
   double bnorm;
   double rel;
+  double alpha;
+  double beta;
   int32_t status;
   int32_t iterations;
 };
@@ -40,6 +41,10 @@ struct OuterState {
   int32_t inner_total;
   int32_t pad;
 };
+
+// Event-timing hooks shared by the solver sources (defined in pcg.hip).
+void prof_begin_batched(hipStream_t s);
+void prof_end_batched(hipStream_t s);
 
 template <typename T>
 struct Accum {

@@ -2,20 +2,30 @@
 //
 // One HIP stream, no host synchronisation inside an iteration: every scalar of
 // the CG recurrence (alpha, beta, ||r||, the stopping and breakdown decisions)
-// lives on the device. Each iteration is three launches:
-//   S  q = A p                      (+ per-block partials of p.q)
-//   U  x += a p, r -= a q, z = r/d  (+ per-block partials of r.r, r.z)
-//   P  p = z + b p                  (stopping test, beta)
-// Cross-block sums never use atomics: the consumer kernel re-reduces the
-// producer's per-block partials in a fixed order, so results are bitwise
-// reproducible run to run, and every block takes the same decisions from the
-// same sums. The host polls the per-column status every `check_every`
-// iterations; kernels launched after every column has stopped return at once.
+// lives on the device. One iteration is five launches:
+//   S   q = A p                       (+ per-block partials of p.q)
+//   F1  alpha = rho / p.q, breakdown  (one block)
+//   U   x += a p, r -= a q, z = r/d   (+ per-block partials of r.r, r.z)
+//   F2  rel, stop test, beta, rho     (one block)
+//   P   p = z + b p
+// Cross-block sums never use atomics: the one-block finalize kernels reduce
+// the producers' per-block partials in a fixed order, so results are bitwise
+// reproducible run to run, and the streaming kernels read only the k
+// per-column scalars. The host polls the per-column status every
+// `check_every` iterations; kernels launched after every column has stopped
+// return at once.
 //
 // The recurrence and the stopping / breakdown rules follow the reference GPU
 // loop, code/fast_solver_backends.py:233-274 (x0 = 0, r = b, z = r/d with
 // zero diagonal -> identity, rel = ||r||/||b|| <= tol non-strict, breakdown
 // when p.Ap <= 0 or non-finite, iterations = k at exit).
+//
+// SpMM layout: a row team of T = K*NZ lanes; lane (z, c) takes nonzeros z,
+// z+NZ, ... of the row for right-hand side c (X and Y row-major [n][K], so the
+// K values of a gathered row are one contiguous segment). Each block owns a
+// contiguous chunk of rows, and chunks are dealt to blocks so that the blocks
+// the hardware places on one XCD (block id mod 8) cover adjacent chunks: the
+// x rows a chunk gathers then mostly hit that XCD's L2.
 
 #include <hip/hip_runtime.h>
 
@@ -28,6 +38,70 @@
 #include "common.h"
 
 namespace cc {
+
+constexpr int kBlock = 256;
+constexpr int kXcds = 8;
+
+// ---------------------------------------------------------------------------
+// Optional per-kernel-class event timing (process-wide, not thread-safe).
+// ---------------------------------------------------------------------------
+
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, size_t>> pending;
+  double ms[kProfClasses] = {0};
+  long long count[kProfClasses] = {0};
+};
+static Profiler g_prof;
+
+static void prof_mark(hipStream_t s) {
+  if (g_prof.used >= g_prof.pool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    g_prof.pool.push_back(e);
+  }
+  hipEventRecord(g_prof.pool[g_prof.used], s);
+  ++g_prof.used;
+}
+
+void prof_begin(int cls, hipStream_t s) {
+  if (!g_prof.on) return;
+  g_prof.pending.push_back({cls, g_prof.used});
+  prof_mark(s);
+}
+
+void prof_end(hipStream_t s) {
+  if (!g_prof.on) return;
+  prof_mark(s);
+}
+
+// Call only after the stream has been synchronised.
+void prof_collect() {
+  if (!g_prof.on) return;
+  for (auto& pr : g_prof.pending) {
+    float t = 0.f;
+    if (pr.second + 1 < g_prof.used &&
+        hipEventElapsedTime(&t, g_prof.pool[pr.second], g_prof.pool[pr.second + 1]) == hipSuccess) {
+      g_prof.ms[pr.first] += t;
+      g_prof.count[pr.first] += 1;
+    }
+  }
+  g_prof.pending.clear();
+  g_prof.used = 0;
+}
+
+// ---------------------------------------------------------------------------
+// SpMM launch configuration (per k; set by the autotuner or the defaults).
+// ---------------------------------------------------------------------------
+
+struct SpmmConfig {
+  int nz;      // nonzero lanes per row team (K*nz <= 64); 0 = pick from the row length
+  int grid;    // blocks; 0 = min(kMaxParts, rows / teams per block)
+  int unroll;  // 1 or 4 independent loads in flight per lane; 0 = 4
+};
+static SpmmConfig g_spmm_cfg[17] = {};
 
 // ---------------------------------------------------------------------------
 // Reductions.
@@ -94,12 +168,9 @@ __global__ __launch_bounds__(256) void k_diag(int n, const int* __restrict__ ind
   diag[row] = d;
 }
 
-// Sparse product over row teams of T = K*NZ lanes: lane (z, c) with
-// c = lane % K takes nonzeros z, z+NZ, ... of the row for right-hand side c;
-// the NZ partial sums are combined with xor shuffles inside the team.
 // MODE 0: Y = A X, partial[block][c] = sum_rows X.Y      (p.q of CG)
 // MODE 1: Y = B - A X, partial[block][c] = sum_rows Y.Y  (true residual)
-template <typename MT, typename VT, int K, int NZ, int MODE>
+template <typename MT, typename VT, int K, int NZ, int MODE, int UNR>
 __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ indptr, const int* __restrict__ indices,
                                               const MT* __restrict__ vals, const VT* __restrict__ X,
                                               VT* __restrict__ Y, const double* __restrict__ B,
@@ -114,14 +185,38 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
   const int zs = (lane / K) % NZ;
   const int wave = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
-  const int team0 = (blockIdx.x * wpb + wave) * TPW + lane / T;
-  const int nteams = gridDim.x * wpb * TPW;
+  const int tpb = wpb * TPW;
+  const int team = wave * TPW + lane / T;
+  // XCD-aware chunk of rows for this block.
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int chunk = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
+  const int rows_per = (n + G - 1) / G;
+  const int r0 = chunk * rows_per;
+  const int r1 = min(n, r0 + rows_per);
   double dot = 0.0;
-  for (int row = team0; row < n; row += nteams) {
+  for (int row = r0 + team; row < r1; row += tpb) {
     const int beg = indptr[row];
-    const int end = indptr[row + 1];
+    const int len = indptr[row + 1] - beg;
+    const int* __restrict__ ixp = indices + beg;
+    const MT* __restrict__ vap = vals + beg;
     AT acc = 0;
-    for (int j = beg + zs; j < end; j += NZ) acc += (AT)vals[j] * (AT)X[(size_t)indices[j] * K + c];
+    int j = zs;
+    if constexpr (UNR == 4) {
+      AT a1 = 0, a2 = 0, a3 = 0;
+      for (; j + 3 * NZ < len; j += 4 * NZ) {
+        const int c0 = ixp[j], c1 = ixp[j + NZ], c2 = ixp[j + 2 * NZ], c3 = ixp[j + 3 * NZ];
+        const MT v0 = vap[j], v1 = vap[j + NZ], v2 = vap[j + 2 * NZ], v3 = vap[j + 3 * NZ];
+        const VT x0 = X[(size_t)c0 * K + c], x1 = X[(size_t)c1 * K + c];
+        const VT x2 = X[(size_t)c2 * K + c], x3 = X[(size_t)c3 * K + c];
+        acc += (AT)v0 * (AT)x0;
+        a1 += (AT)v1 * (AT)x1;
+        a2 += (AT)v2 * (AT)x2;
+        a3 += (AT)v3 * (AT)x3;
+      }
+      acc = (acc + a1) + (a2 + a3);
+    }
+    for (; j < len; j += NZ) acc += (AT)vap[j] * (AT)X[(size_t)ixp[j] * K + c];
 #pragma unroll
     for (int o = K; o < T; o <<= 1) acc += __shfl_xor(acc, o, 64);
     if (zs == 0) {
@@ -137,7 +232,7 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
       }
     }
   }
-  if (partial) block_col_sum<K>(dot, partial + (size_t)blockIdx.x * K, smem);
+  if (partial) block_col_sum<K>(dot, partial + (size_t)b * K, smem);
 }
 
 // x = 0, r = src * inv_scale[c], z = r/d, p = z; partials (r.z, r.r).
@@ -183,10 +278,11 @@ __global__ __launch_bounds__(256) void k_cg_setup_finish(const double* __restric
     double bn = sqrt(bb);
     if (bn == 0.0) bn = 1.0;
     ColState s;
-    s.rho[0] = rz;
-    s.rho[1] = rz;
+    s.rho = rz;
     s.bnorm = bn;
     s.rel = sqrt(bb) / bn;
+    s.alpha = 0.0;
+    s.beta = 0.0;
     s.status = (run_mask == nullptr || run_mask[c]) ? CC_RUNNING : CC_CONVERGED;
     s.iterations = 0;
     st[c] = s;
@@ -194,47 +290,47 @@ __global__ __launch_bounds__(256) void k_cg_setup_finish(const double* __restric
   }
 }
 
-// alpha = rho / p.q per column (breakdown when p.q <= 0 or non-finite);
-// x += alpha p, r -= alpha q, z = r/d; partials (r.r, r.z).
+// F1: alpha = rho / p.q; breakdown when p.q <= 0 or non-finite.
+template <int K>
+__global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ partial, int nparts,
+                                                  ColState* __restrict__ st, int iter) {
+  __shared__ double smem[256];
+  __shared__ double den[K];
+  if (all_stopped<K>(st)) return;
+  reduce_partials<K>(partial, nparts, den, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    if (st[c].status == CC_RUNNING) {
+      const double dn = den[c];
+      if (!(dn > 0.0 && isfinite(dn))) {
+        st[c].status = CC_BREAKDOWN;
+        st[c].iterations = iter;
+        st[c].alpha = 0.0;
+      } else {
+        st[c].alpha = st[c].rho / dn;
+      }
+    }
+  }
+}
+
+// U: x += alpha p, r -= alpha q, z = r/d; partials (r.r, r.z).
 template <typename VT, int K>
 __global__ __launch_bounds__(256) void k_cg_update(int n, VT* __restrict__ X, VT* __restrict__ R, VT* __restrict__ Z,
                                                    const VT* __restrict__ P, const VT* __restrict__ Q,
                                                    const double* __restrict__ diag, int precond,
-                                                   const double* __restrict__ partial_in, int nparts_in,
-                                                   ColState* __restrict__ st, int iter,
+                                                   const ColState* __restrict__ st,
                                                    double* __restrict__ partial_out) {
-  __shared__ double smem[256];
-  __shared__ double den[K];
-  __shared__ double alpha[K];
-  __shared__ int active[K];
+  __shared__ double smem[16 * 16];
   if (all_stopped<K>(st)) return;
-  reduce_partials<K>(partial_in, nparts_in, den, smem);
-  if (threadIdx.x < K) {
-    const int c = threadIdx.x;
-    bool act = st[c].status == CC_RUNNING;
-    const double dn = den[c];
-    if (act && !(dn > 0.0 && isfinite(dn))) {
-      act = false;
-      if (blockIdx.x == 0) {
-        st[c].status = CC_BREAKDOWN;
-        st[c].iterations = iter;
-      }
-    }
-    alpha[c] = act ? st[c].rho[iter & 1] / dn : 0.0;
-    active[c] = act;
-  }
-  __syncthreads();
-  const size_t total = (size_t)n * K;
   const int c = threadIdx.x % K;  // fixed per thread: the stride is a multiple of K
-  const bool act = active[c];
-  const double a = alpha[c];
+  const bool act = st[c].status == CC_RUNNING;
+  const double a = st[c].alpha;
+  const size_t total = (size_t)n * K;
   double rr = 0.0, rz = 0.0;
   if (act) {
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-      const VT p = P[e];
-      const VT q = Q[e];
-      const VT x = (VT)((double)X[e] + a * (double)p);
-      const VT r = (VT)((double)R[e] - a * (double)q);
+      const VT x = (VT)((double)X[e] + a * (double)P[e]);
+      const VT r = (VT)((double)R[e] - a * (double)Q[e]);
       const double d = precond ? diag[e / K] : 0.0;
       const VT z = (precond && fabs(d) > 0.0) ? (VT)((double)r / d) : r;
       X[e] = x;
@@ -248,54 +344,59 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, VT* __restrict__ X, VT
   block_col_sum<K>(rz, partial_out + ((size_t)blockIdx.x * 2 + 1) * K, smem);
 }
 
-// rel = ||r||/||b||: stop when rel <= tol (or iter == maxiter); otherwise
-// beta = (r.z)_new / rho and p = z + beta p.
-template <typename VT, int K>
-__global__ __launch_bounds__(256) void k_cg_direction(int n, VT* __restrict__ P, const VT* __restrict__ Z,
-                                                      const double* __restrict__ partial_in, int nparts_in,
-                                                      ColState* __restrict__ st, int iter, double tol, int maxiter,
-                                                      double* __restrict__ hist) {
+// F2: rel = ||r||/||b||; stop when rel <= tol (or iter == maxiter); otherwise
+// beta = (r.z)_new / rho and rho = (r.z)_new.
+template <int K>
+__global__ __launch_bounds__(256) void k_cg_beta(const double* __restrict__ partial, int nparts,
+                                                 ColState* __restrict__ st, int iter, double tol, int maxiter,
+                                                 double* __restrict__ hist) {
   __shared__ double smem[256];
   __shared__ double sums[2 * K];
-  __shared__ double beta[K];
-  __shared__ int active[K];
   if (all_stopped<K>(st)) return;
-  reduce_partials<2 * K>(partial_in, nparts_in, sums, smem);
+  reduce_partials<2 * K>(partial, nparts, sums, smem);
   if (threadIdx.x < K) {
     const int c = threadIdx.x;
-    bool act = st[c].status == CC_RUNNING;
-    double b = 0.0;
-    if (act) {
+    if (st[c].status == CC_RUNNING) {
       const double rr = sums[c];
       const double rz = sums[K + c];
       const double rel = sqrt(rr) / st[c].bnorm;
-      const double rho = st[c].rho[iter & 1];
-      if (blockIdx.x == 0) {
-        st[c].rel = rel;
-        st[c].iterations = iter;
-        if (hist) hist[(size_t)iter * K + c] = rel;
-      }
+      st[c].rel = rel;
+      st[c].iterations = iter;
+      if (hist) hist[(size_t)iter * K + c] = rel;
       if (rel <= tol) {
-        act = false;
-        if (blockIdx.x == 0) st[c].status = CC_CONVERGED;
+        st[c].status = CC_CONVERGED;
       } else if (iter >= maxiter) {
-        act = false;
-        if (blockIdx.x == 0) st[c].status = CC_MAXITER;
+        st[c].status = CC_MAXITER;
       } else {
-        b = rz / rho;
-        if (blockIdx.x == 0) st[c].rho[(iter + 1) & 1] = rz;
+        st[c].beta = rz / st[c].rho;
+        st[c].rho = rz;
       }
     }
-    beta[c] = b;
-    active[c] = act;
   }
-  __syncthreads();
+}
+
+// P: p = z + beta p for the columns still running.
+template <typename VT, int K>
+__global__ __launch_bounds__(256) void k_cg_direction(int n, VT* __restrict__ P, const VT* __restrict__ Z,
+                                                      const ColState* __restrict__ st) {
+  if (all_stopped<K>(st)) return;
   const int c = threadIdx.x % K;
-  if (!active[c]) return;
-  const double b = beta[c];
+  if (st[c].status != CC_RUNNING) return;
+  const double b = st[c].beta;
   const size_t total = (size_t)n * K;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
     P[e] = (VT)((double)Z[e] + b * (double)P[e]);
+}
+
+// Column sums of squares of a row-major [n][K] block: partial[block][c].
+template <int K>
+__global__ __launch_bounds__(256) void k_colsq(int n, const double* __restrict__ V, double* __restrict__ partial) {
+  __shared__ double smem[16 * 16];
+  const size_t total = (size_t)n * K;
+  double s = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    s += V[e] * V[e];
+  block_col_sum<K>(s, partial + (size_t)blockIdx.x * K, smem);
 }
 
 // Outer refinement bookkeeping (one block): rel = ||B - A X|| / ||B||.
@@ -329,8 +430,7 @@ __global__ __launch_bounds__(256) void k_outer_finish(const double* __restrict__
   }
 }
 
-// X64 += scale[c] * D32 for the columns of this sweep; bookkeeping of the
-// inner iteration counts.
+// X64 += scale[c] * D32 for the columns of this sweep.
 template <int K>
 __global__ __launch_bounds__(256) void k_outer_correct(int n, double* __restrict__ X, const float* __restrict__ D,
                                                        const OuterState* __restrict__ ost,
@@ -361,65 +461,15 @@ __global__ void k_zero(double* __restrict__ p, size_t count) {
 // Host-side launch helpers.
 // ---------------------------------------------------------------------------
 
-constexpr int kBlock = 256;
-
-// Optional per-kernel-class event timing (process-wide, not thread-safe).
-// Each timed launch is bracketed by two events on the launch stream; the
-// elapsed times are collected whenever the host has synchronised the stream.
-struct Profiler {
-  bool on = false;
-  std::vector<hipEvent_t> pool;
-  size_t used = 0;
-  std::vector<std::pair<int, size_t>> pending;
-  double ms[kProfClasses] = {0};
-  long long count[kProfClasses] = {0};
-};
-static Profiler g_prof;
-
-static void prof_mark(hipStream_t s) {
-  if (g_prof.used >= g_prof.pool.size()) {
-    hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
-    g_prof.pool.push_back(e);
-  }
-  hipEventRecord(g_prof.pool[g_prof.used], s);
-  ++g_prof.used;
-}
-
-static void prof_begin(int cls, hipStream_t s) {
-  if (!g_prof.on) return;
-  g_prof.pending.push_back({cls, g_prof.used});
-  prof_mark(s);
-}
-
-static void prof_end(hipStream_t s) {
-  if (!g_prof.on) return;
-  prof_mark(s);
-}
-
-// Call only after the stream has been synchronised.
-static void prof_collect() {
-  if (!g_prof.on) return;
-  for (auto& pr : g_prof.pending) {
-    float t = 0.f;
-    if (pr.second + 1 < g_prof.used && hipEventElapsedTime(&t, g_prof.pool[pr.second], g_prof.pool[pr.second + 1]) == hipSuccess) {
-      g_prof.ms[pr.first] += t;
-      g_prof.count[pr.first] += 1;
-    }
-  }
-  g_prof.pending.clear();
-  g_prof.used = 0;
-}
-
 static int stream_grid(size_t work) {
-  size_t g = (work + kBlock - 1) / kBlock;
+  size_t g = (work + kBlock * 4 - 1) / (kBlock * 4);
   if (g < 1) g = 1;
   if (g > (size_t)kMaxParts) g = kMaxParts;
   return (int)g;
 }
 
 template <int K>
-static int nz_lanes(double avg_row) {
+static int default_nz(double avg_row) {
   if (K > 1) return 64 / K;
   if (avg_row >= 96.0) return 64;
   if (avg_row >= 40.0) return 32;
@@ -428,36 +478,49 @@ static int nz_lanes(double avg_row) {
 }
 
 template <typename MT, typename VT, int K, int NZ, int MODE>
-static int spmm_launch_nz(int n, const int* indptr, const int* indices, const MT* vals, const VT* X, VT* Y,
-                          const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+static int spmm_launch_nz(int n, int unroll, int grid_req, const int* indptr, const int* indices, const MT* vals,
+                          const VT* X, VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s,
+                          int* grid_out) {
   constexpr int T = K * NZ;
   const int teams_per_block = (kBlock / 64) * (64 / T);
-  int grid = (n + teams_per_block - 1) / teams_per_block;
-  if (grid < 1) grid = 1;
+  int grid = grid_req;
+  if (grid <= 0) grid = (n + teams_per_block - 1) / teams_per_block;
   if (grid > kMaxParts) grid = kMaxParts;
-  hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals, X, Y, B,
-                     partial, st);
+  if (grid >= 2 * kXcds) grid -= grid % kXcds;
+  if (grid < 1) grid = 1;
+  if (unroll >= 4)
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals, X, Y,
+                       B, partial, st);
+  else
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals, X, Y,
+                       B, partial, st);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
 
 template <typename MT, typename VT, int K, int MODE>
-static int spmm_launch(int n, int nz, const int* indptr, const int* indices, const MT* vals, const VT* X, VT* Y,
-                       const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
-  if constexpr (K == 1) {
-    switch (nz) {
-      case 64: return spmm_launch_nz<MT, VT, 1, 64, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
-      case 32: return spmm_launch_nz<MT, VT, 1, 32, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
-      case 16: return spmm_launch_nz<MT, VT, 1, 16, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
-      default: return spmm_launch_nz<MT, VT, 1, 8, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
-    }
-  } else {
-    return spmm_launch_nz<MT, VT, K, 64 / K, MODE>(n, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
-  }
+static int spmm_launch(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
+                       VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  const SpmmConfig cfg = g_spmm_cfg[K];
+  const int nz = cfg.nz > 0 ? cfg.nz : default_nz<K>(avg_row);
+  const int unroll = cfg.unroll > 0 ? cfg.unroll : 4;
+  const int grid = cfg.grid;
+#define CC_NZ(NZV)                                                                                              \
+  if (nz == NZV && K * NZV <= 64)                                                                               \
+    return spmm_launch_nz<MT, VT, K, (K * NZV <= 64 ? NZV : 1), MODE>(n, unroll, grid, indptr, indices, vals, X, \
+                                                                       Y, B, partial, st, s, grid_out);
+  CC_NZ(64)
+  CC_NZ(32)
+  CC_NZ(16)
+  CC_NZ(8)
+  CC_NZ(4)
+  CC_NZ(2)
+  CC_NZ(1)
+#undef CC_NZ
+  return CC_E_ARG;
 }
 
 static double avg_row_len(int n, const int* indptr, hipStream_t s) {
-  // Only the two end values are needed: nnz = indptr[n] - indptr[0].
   int ends[2] = {0, 0};
   if (hipMemcpyAsync(&ends[0], indptr, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return 64.0;
   if (hipMemcpyAsync(&ends[1], indptr + n, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return 64.0;
@@ -515,11 +578,11 @@ static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external) {
   return b;
 }
 
-// The device-resident CG loop for k columns. src is the float64 right-hand
-// side block; inv_scale (device, may be null) scales it into VT. Returns a HIP
-// status; the per-column state is left in b.st.
+// The device-resident CG loop for K columns. src is the float64 right-hand
+// side block; inv_scale (device, may be null) scales it into VT. The
+// per-column state is left in b.st (and copied to host_st).
 template <typename MT, typename VT, int K>
-static int cg_run(int n, int nz, const int* indptr, const int* indices, const MT* vals, const double* src,
+static int cg_run(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const double* src,
                   const double* inv_scale, const int* run_mask, CgBuffers<VT>& b, int precond, double tol,
                   int maxiter, int check_every, hipStream_t s, std::vector<ColState>& host_st,
                   double* hist = nullptr) {
@@ -533,15 +596,16 @@ static int cg_run(int n, int nz, const int* indptr, const int* indices, const MT
   for (int it = 1; it <= maxiter; ++it) {
     int gs = 0;
     prof_begin(kProfSpmm, s);
-    CC_CHECK((spmm_launch<MT, VT, K, 0>(n, nz, indptr, indices, vals, b.P, b.Q, nullptr, b.pa, b.st, s, &gs)));
+    CC_CHECK((spmm_launch<MT, VT, K, 0>(n, avg_row, indptr, indices, vals, b.P, b.Q, nullptr, b.pa, b.st, s, &gs)));
     prof_end(s);
+    hipLaunchKernelGGL((k_cg_alpha<K>), dim3(1), dim3(kBlock), 0, s, b.pa, gs, b.st, it);
     prof_begin(kProfUpdate, s);
     hipLaunchKernelGGL((k_cg_update<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.X, b.R, b.Z, b.P, b.Q, b.diag,
-                       precond, b.pa, gs, b.st, it, b.pb);
+                       precond, b.st, b.pb);
     prof_end(s);
+    hipLaunchKernelGGL((k_cg_beta<K>), dim3(1), dim3(kBlock), 0, s, b.pb, sg, b.st, it, tol, maxiter, hist);
     prof_begin(kProfDirection, s);
-    hipLaunchKernelGGL((k_cg_direction<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.P, b.Z, b.pb, sg, b.st, it, tol,
-                       maxiter, hist);
+    hipLaunchKernelGGL((k_cg_direction<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.P, b.Z, b.st);
     prof_end(s);
     CC_CHECK(hipGetLastError());
     if (it % check_every == 0 || it == maxiter) {
@@ -559,25 +623,14 @@ static int cg_run(int n, int nz, const int* indptr, const int* indices, const MT
   return 0;
 }
 
-// Column sums of squares of a row-major [n][K] block: partial[block][c].
-template <int K>
-__global__ __launch_bounds__(256) void k_colsq(int n, const double* __restrict__ V, double* __restrict__ partial) {
-  __shared__ double smem[16 * 16];
-  const size_t total = (size_t)n * K;
-  double s = 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
-    s += V[e] * V[e];
-  block_col_sum<K>(s, partial + (size_t)blockIdx.x * K, smem);
-}
-
 // True relative residuals ||B - A X|| / ||B|| (||B|| = 0 -> 1) of the float64
 // block X, written to rel[K] (host). The per-block partial sums are combined
 // on the host in block order.
 template <int K>
-static int true_residual(int n, int nz, const int* indptr, const int* indices, const double* vals, const double* B,
-                         const double* X, double* R, double* partial, hipStream_t s, double* rel) {
+static int true_residual(int n, double avg_row, const int* indptr, const int* indices, const double* vals,
+                         const double* B, const double* X, double* R, double* partial, hipStream_t s, double* rel) {
   int gs = 0;
-  CC_CHECK((spmm_launch<double, double, K, 1>(n, nz, indptr, indices, vals, X, R, B, partial, nullptr, s, &gs)));
+  CC_CHECK((spmm_launch<double, double, K, 1>(n, avg_row, indptr, indices, vals, X, R, B, partial, nullptr, s, &gs)));
   const int sg = stream_grid((size_t)n * K);
   hipLaunchKernelGGL((k_colsq<K>), dim3(sg), dim3(kBlock), 0, s, n, B, partial + (size_t)kMaxParts * K);
   CC_CHECK(hipGetLastError());
@@ -605,16 +658,15 @@ static int pcg_f64(int n, const int* indptr, const int* indices, const double* v
   CgBuffers<double> b = carve_cg<double>(cv, n, K, X);
   if (!cv.ok()) return CC_E_WORKSPACE;
   const double avg = avg_row_len(n, indptr, s);
-  const int nz = nz_lanes<K>(avg);
   if (precond) {
     hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, indptr, indices, vals, b.diag);
     CC_CHECK(hipGetLastError());
   }
   std::vector<ColState> hs;
-  CC_CHECK((cg_run<double, double, K>(n, nz, indptr, indices, vals, B, nullptr, nullptr, b, precond, tol, maxiter,
+  CC_CHECK((cg_run<double, double, K>(n, avg, indptr, indices, vals, B, nullptr, nullptr, b, precond, tol, maxiter,
                                       check_every, s, hs, hist)));
   std::vector<double> rel(K);
-  CC_CHECK(true_residual<K>(n, nz, indptr, indices, vals, B, X, b.R, b.pa, s, rel.data()));
+  CC_CHECK(true_residual<K>(n, avg, indptr, indices, vals, B, X, b.R, b.pa, s, rel.data()));
   for (int c = 0; c < K; ++c) {
     results[c].iterations = hs[c].iterations;
     results[c].status = hs[c].status == CC_RUNNING ? CC_MAXITER : hs[c].status;
@@ -630,8 +682,8 @@ template <int K>
 static size_t mixed_bytes(size_t n) {
   size_t b = cg_bytes<float>(n, K);
   auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
-  add(n * K * sizeof(double));                    // R64
-  add((size_t)kMaxParts * K * sizeof(double));    // residual partials
+  add(n * K * sizeof(double));                  // R64
+  add((size_t)kMaxParts * K * sizeof(double));  // residual partials
   add(K * sizeof(OuterState));
   add(K * sizeof(int));
   add(K * sizeof(double));
@@ -651,14 +703,13 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
   double* inv_scale = cv.take<double>(K);
   if (!cv.ok()) return CC_E_WORKSPACE;
   const double avg = avg_row_len(n, indptr, s);
-  const int nz = nz_lanes<K>(avg);
   if (precond) {
     hipLaunchKernelGGL(k_diag, dim3((n + 255) / 256), dim3(256), 0, s, n, indptr, indices, vals, b.diag);
   }
   const int sg = stream_grid((size_t)n * K);
   hipLaunchKernelGGL(k_zero, dim3(sg), dim3(kBlock), 0, s, X, (size_t)n * K);
   int gs = 0;
-  CC_CHECK((spmm_launch<double, double, K, 1>(n, nz, indptr, indices, vals, X, R64, B, pr, nullptr, s, &gs)));
+  CC_CHECK((spmm_launch<double, double, K, 1>(n, avg, indptr, indices, vals, X, R64, B, pr, nullptr, s, &gs)));
   hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 1);
   CC_CHECK(hipGetLastError());
   std::vector<OuterState> ho(K);
@@ -670,15 +721,15 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
     bool any = false;
     for (int c = 0; c < K; ++c) any |= ho[c].status == CC_RUNNING;
     if (!any || used >= maxiter) break;
-    CC_CHECK((cg_run<float, float, K>(n, nz, indptr, indices, vals32, R64, inv_scale, run_mask, b, precond, inner_tol,
-                                      maxiter - used, check_every, s, hs)));
+    CC_CHECK((cg_run<float, float, K>(n, avg, indptr, indices, vals32, R64, inv_scale, run_mask, b, precond,
+                                      inner_tol, maxiter - used, check_every, s, hs)));
     int most = 0;
     for (int c = 0; c < K; ++c)
       if (ho[c].status == CC_RUNNING && hs[c].iterations > most) most = hs[c].iterations;
     used += most > 0 ? most : 1;
     hipLaunchKernelGGL((k_outer_correct<K>), dim3(sg), dim3(kBlock), 0, s, n, X, b.X, ost, run_mask);
     hipLaunchKernelGGL((k_outer_account<K>), dim3(1), dim3(64), 0, s, ost, b.st, run_mask);
-    CC_CHECK((spmm_launch<double, double, K, 1>(n, nz, indptr, indices, vals, X, R64, B, pr, nullptr, s, &gs)));
+    CC_CHECK((spmm_launch<double, double, K, 1>(n, avg, indptr, indices, vals, X, R64, B, pr, nullptr, s, &gs)));
     hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 0);
     CC_CHECK(hipGetLastError());
   }
@@ -694,6 +745,10 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
   }
   return 0;
 }
+
+// Hooks used by the batched solver (batched.hip).
+void prof_begin_batched(hipStream_t s) { prof_begin(kProfBatched, s); }
+void prof_end_batched(hipStream_t s) { prof_end(s); }
 
 }  // namespace cc
 
@@ -715,7 +770,12 @@ using namespace cc;
 
 extern "C" {
 
-int cc_version(void) { return 10000; }
+int cc_version(void) { return 10100; }
+
+int cc_device_count(int* count) {
+  if (!count) return CC_E_ARG;
+  return (int)hipGetDeviceCount(count);
+}
 
 int cc_profile_enable(int32_t on) {
   g_prof.on = on != 0;
@@ -739,9 +799,17 @@ int cc_profile_read(int32_t which, double* total_ms, int64_t* count) {
   return 0;
 }
 
-int cc_device_count(int* count) {
-  if (!count) return CC_E_ARG;
-  return (int)hipGetDeviceCount(count);
+int cc_profile_collect(void) {
+  prof_collect();
+  return 0;
+}
+
+int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll) {
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
+  if (nz_lanes < 0 || k * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
+  if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4)) return CC_E_ARG;
+  g_spmm_cfg[k] = SpmmConfig{nz_lanes, grid, unroll};
+  return 0;
 }
 
 int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals, const double* x,
@@ -749,8 +817,8 @@ int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, co
   if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !x || !y))) return CC_E_ARG;
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const int nz = nz_lanes<1>(avg_row_len(n, indptr, s));
-  return spmm_launch<double, double, 1, 0>(n, nz, indptr, indices, vals, x, y, nullptr, nullptr, nullptr, s, nullptr);
+  return spmm_launch<double, double, 1, 0>(n, avg_row_len(n, indptr, s), indptr, indices, vals, x, y, nullptr,
+                                           nullptr, nullptr, s, nullptr);
 }
 
 int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, const float* vals, const float* x,
@@ -758,8 +826,8 @@ int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, co
   if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !x || !y))) return CC_E_ARG;
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const int nz = nz_lanes<1>(avg_row_len(n, indptr, s));
-  return spmm_launch<float, float, 1, 0>(n, nz, indptr, indices, vals, x, y, nullptr, nullptr, nullptr, s, nullptr);
+  return spmm_launch<float, float, 1, 0>(n, avg_row_len(n, indptr, s), indptr, indices, vals, x, y, nullptr, nullptr,
+                                         nullptr, s, nullptr);
 }
 
 int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
@@ -768,8 +836,18 @@ int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* 
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
   const double avg = avg_row_len(n, indptr, s);
-  CC_DISPATCH_K(k, (spmm_launch<double, double, KK, 0>(n, nz_lanes<KK>(avg), indptr, indices, vals, X, Y, nullptr,
-                                                       nullptr, nullptr, s, nullptr)));
+  CC_DISPATCH_K(k, (spmm_launch<double, double, KK, 0>(n, avg, indptr, indices, vals, X, Y, nullptr, nullptr, nullptr,
+                                                       s, nullptr)));
+}
+
+int cc_spmm_csr_f32(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const float* vals,
+                    const float* X, float* Y, cc_stream_t stream) {
+  if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !X || !Y))) return CC_E_ARG;
+  if (n == 0) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  const double avg = avg_row_len(n, indptr, s);
+  CC_DISPATCH_K(k, (spmm_launch<float, float, KK, 0>(n, avg, indptr, indices, vals, X, Y, nullptr, nullptr, nullptr,
+                                                     s, nullptr)));
 }
 
 int cc_jacobi_diag_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals, double* diag,
@@ -792,7 +870,7 @@ int cc_residual_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* 
   double* R = cv.take<double>((size_t)n * k);
   double* part = cv.take<double>((size_t)kMaxParts * 2 * k);
   const double avg = avg_row_len(n, indptr, s);
-  CC_DISPATCH_K(k, (true_residual<KK>(n, nz_lanes<KK>(avg), indptr, indices, vals, B, X, R, part, s, rel)));
+  CC_DISPATCH_K(k, (true_residual<KK>(n, avg, indptr, indices, vals, B, X, R, part, s, rel)));
 }
 
 size_t cc_pcg_workspace_bytes(int32_t n, int32_t k, int32_t precision) {

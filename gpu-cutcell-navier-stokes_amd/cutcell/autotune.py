@@ -1,0 +1,175 @@
+"""Project autotuner for the HIP launch shapes.
+
+Two tables, cached on disk as JSON (``$CUTCELL_AUTOTUNE_CACHE`` or
+``~/.cache/cutcell/autotune.json``) and keyed by GPU architecture and a bucket
+of the problem shape:
+
+- ``spmm``: for one matrix shape class and k right-hand sides, the nonzero
+  lanes per row team, the grid and the unroll depth of the CSR SpMM that the
+  CG loop runs every iteration (set in the library with
+  ``cc_spmm_set_config``);
+- ``batched``: for a batch of small systems, the workgroup size and the lanes
+  per row team of the persistent batched solve.
+
+Candidates are timed with HIP events on the current stream; the fastest is
+kept. Timing runs only short, bounded work (a few SpMM launches, or a batched
+solve capped at a few tens of iterations).
+"""
+
+from __future__ import annotations
+
+import json
+import math
+import os
+from pathlib import Path
+
+from . import hip
+
+_MEM: dict = {}
+
+
+def cache_path() -> Path:
+    p = os.environ.get("CUTCELL_AUTOTUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "cutcell" / "autotune.json"
+
+
+def _load() -> dict:
+    if _MEM:
+        return _MEM
+    p = cache_path()
+    if p.exists():
+        try:
+            _MEM.update(json.loads(p.read_text()))
+        except (OSError, ValueError):
+            pass
+    return _MEM
+
+
+def _store(key: str, value: dict) -> None:
+    table = _load()
+    table[key] = value
+    p = cache_path()
+    try:
+        p.parent.mkdir(parents=True, exist_ok=True)
+        p.write_text(json.dumps(table, indent=1, sort_keys=True))
+    except OSError:
+        pass
+
+
+def _arch() -> str:
+    import torch
+
+    props = torch.cuda.get_device_properties(torch.cuda.current_device())
+    return getattr(props, "gcnArchName", props.name).split(":")[0]
+
+
+def _bucket(x: float) -> int:
+    return int(round(math.log2(max(x, 1.0))))
+
+
+def _time_ms(fn, reps: int) -> float:
+    import torch
+
+    s = torch.cuda.current_stream()
+    fn()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(s)
+    for _ in range(reps):
+        fn()
+    end.record(s)
+    end.synchronize()
+    return start.elapsed_time(end) / reps
+
+
+# ---------------------------------------------------------------------------
+# SpMM of the single-system CG loop.
+# ---------------------------------------------------------------------------
+
+
+def spmm_candidates(k: int, mean_row: float):
+    nzs = [nz for nz in (64, 32, 16, 8, 4, 2, 1) if k * nz <= 64 and nz * 2 <= max(mean_row, 2) * 2]
+    nzs = nzs[:3] if nzs else [max(1, 64 // k)]
+    for nz in nzs:
+        for grid in (256, 512, 1024):
+            for unroll in (1, 4):
+                yield dict(nz=nz, grid=grid, unroll=unroll)
+
+
+def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool = False) -> dict:
+    """Pick and apply the SpMM launch shape for ``A`` with ``k`` right-hand sides."""
+    import torch
+
+    lib = hip.require_gpu()
+    mean_row = A.nnz / max(A.n, 1)
+    key = f"spmm|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
+    table = _load()
+    if key in table and not force:
+        cfg = table[key]
+        lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"])
+        return cfg
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    X = torch.randn(A.n, k, dtype=dt, device=A.vals.device)
+    Y = torch.empty_like(X)
+    vals = A.vals if precision == "fp64" else A.ensure_f32()
+    fn_name = "cc_spmm_csr_f64" if precision == "fp64" else "cc_spmm_csr_f32"
+    fn = getattr(lib, fn_name)
+    best = None
+    for cand in spmm_candidates(k, mean_row):
+        if lib.cc_spmm_set_config(k, cand["nz"], cand["grid"], cand["unroll"]) != 0:
+            continue
+        call = lambda: fn(A.n, k, hip._ptr(A.indptr), hip._ptr(A.indices), hip._ptr(vals), hip._ptr(X), hip._ptr(Y),
+                          hip._stream())
+        ms = _time_ms(call, 5)
+        if best is None or ms < best[0]:
+            best = (ms, cand)
+    cfg = dict(best[1], ms=best[0])
+    lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"])
+    _store(key, cfg)
+    return cfg
+
+
+def reset_spmm_config() -> None:
+    lib = hip.load_library()
+    for k in hip.SUPPORTED_K:
+        lib.cc_spmm_set_config(k, 0, 0, 0)
+
+
+# ---------------------------------------------------------------------------
+# Batched persistent solve.
+# ---------------------------------------------------------------------------
+
+
+def batched_candidates(n_max: int):
+    for block in (256, 512, 1024):
+        rpt = -(-n_max // block)
+        if rpt > 16 or (rpt > 8 and block * 16 > 8192):
+            continue
+        for team in (8, 16, 32, 64):
+            yield dict(block_threads=block, team_lanes=team)
+
+
+def batched_config(bs, precision: str = "fp64", force: bool = False, tune_iters: int = 30) -> dict:
+    """Pick the workgroup size and row-team width for a batch of systems."""
+    import torch
+
+    key = f"batched|{_arch()}|{precision}|nmax{_bucket(bs.n_max)}|row{_bucket(bs.mean_row)}|sys{_bucket(bs.nsys)}"
+    table = _load()
+    if key in table and not force:
+        return table[key]
+    B = torch.ones(bs.total_n, dtype=torch.float64, device=bs.device)
+    X = torch.empty_like(B)
+    best = None
+    for cand in batched_candidates(bs.n_max):
+        call = lambda: bs.solve_device(B, X, tol=0.0, maxiter=tune_iters, precision=precision, **cand)
+        try:
+            ms = _time_ms(call, 2)
+        except hip.HipError:
+            continue
+        if best is None or ms < best[0]:
+            best = (ms, cand)
+    if best is None:
+        raise hip.HipError("no batched launch shape could be run")
+    cfg = dict(best[1], ms_per_iter=best[0] / tune_iters)
+    _store(key, cfg)
+    return cfg

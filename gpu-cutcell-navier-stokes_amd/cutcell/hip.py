@@ -59,6 +59,14 @@ _SIGNATURES = {
     "cc_profile_enable": ([_I32], ctypes.c_int),
     "cc_profile_reset": ([], ctypes.c_int),
     "cc_profile_read": ([_I32, _P, _P], ctypes.c_int),
+    "cc_profile_collect": ([], ctypes.c_int),
+    "cc_spmm_set_config": ([_I32, _I32, _I32, _I32], ctypes.c_int),
+    "cc_spmm_csr_f32": ([_I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_pcg_batched": (
+        [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _F64, _I32, _I32, _I32,
+         _I32, _P, _P, _P],
+        ctypes.c_int,
+    ),
     "cc_spmv_csr_f64": ([_I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_spmv_csr_f32": ([_I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_spmm_csr_f64": ([_I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
@@ -227,8 +235,13 @@ def profile_enable(on: bool = True) -> None:
 
 
 def profile_read(which: str) -> tuple[float, int]:
-    """(summed milliseconds, launch count) of a timed kernel class."""
+    """(summed milliseconds, launch count) of a timed kernel class.
+
+    Synchronises the device first so that launches made outside the solvers'
+    own polling loops are counted."""
     lib = load_library()
+    _torch().cuda.synchronize()
+    lib.cc_profile_collect()
     ms = ctypes.c_double()
     cnt = ctypes.c_int64()
     _check(lib.cc_profile_read(PROFILE_CLASSES[which], ctypes.addressof(ms), ctypes.addressof(cnt)), "profile_read")
@@ -265,6 +278,7 @@ def pcg(
     X=None,
     workspace: Workspace | None = None,
     history: bool = False,
+    tune: bool = False,
 ):
     """Device Jacobi-PCG on ``k`` right-hand sides (``B`` is ``[n]`` or ``[n, k]``).
 
@@ -283,6 +297,10 @@ def pcg(
     X2 = torch.empty_like(B2) if X is None else X.reshape(A.n, k)
     prec = {"none": 0, "jacobi": 1}[preconditioner]
     mode = {"fp64": 0, "mixed": 1}[precision]
+    if tune:
+        from .autotune import spmm_config
+
+        spmm_config(A, k, precision)
     nbytes = int(lib.cc_pcg_workspace_bytes(A.n, k, mode))
     ws = (workspace or _default_ws).get(nbytes, A.vals.device)
     res = (PcgResult * k)()

@@ -72,6 +72,16 @@ int cc_device_count(int* count);
 int cc_profile_enable(int32_t on);
 int cc_profile_reset(void);
 int cc_profile_read(int32_t which, double* total_ms, int64_t* count);
+/* Folds the events of launches made outside the solvers' own polling (the
+ * standalone SpMV/SpMM entry points and the batched solve) into the totals;
+ * call after synchronising their stream. */
+int cc_profile_collect(void);
+
+/* SpMM/SpMV launch shape for k right-hand sides, chosen by the autotuner
+ * (no reference counterpart): nz_lanes nonzero lanes per row team
+ * (k*nz_lanes <= 64, power of two; 0 = from the mean row length), grid
+ * blocks (0 = one block per 4..16 rows up to 1024), unroll 1 or 4 (0 = 4). */
+int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll);
 
 /* y = A x, float64 (replaces the cuSPARSE product A_gpu @ p). */
 int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
@@ -82,6 +92,8 @@ int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, co
 /* Y = A X for k right-hand sides, X and Y row-major [n][k]. */
 int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
                     const double* X, double* Y, cc_stream_t stream);
+int cc_spmm_csr_f32(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const float* vals,
+                    const float* X, float* Y, cc_stream_t stream);
 
 /* d[i] = A[i,i] (0 where the row stores no diagonal entry). */
 int cc_jacobi_diag_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
@@ -121,6 +133,28 @@ int cc_pcg_jacobi_mixed(int32_t n, int32_t k, const int32_t* indptr, const int32
                         const float* vals32, const double* B, double* X, double tol, int32_t maxiter,
                         double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
                         void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream);
+
+/* Batched persistent solve of nsys independent systems of uneven size in one
+ * launch (replaces a loop of solve_spd calls over independent cases, the
+ * reference's case-level parallelism in code/fast_solver_workbench.py:201-217,
+ * each case being solve_gpu_cg's loop, code/fast_solver_backends.py:233-274).
+ * The systems' CSR arrays are concatenated: system s has sys_n[s] rows, its
+ * row pointers (local, starting at 0) at indptr + sys_indptr_off[s], its
+ * column indices (local) and values at sys_nnz_off[s], and its right-hand side
+ * and solution at sys_vec_off[s]. All per-system arrays and order (the
+ * dispatch order, longest first) are device memory. n_max <= 8192 (the
+ * workgroup keeps p and Ap of its system in LDS). precision 0 = float64,
+ * 1 = float32 inner / float64 outer (vals32 required). block_threads in
+ * {256, 512, 1024} and team_lanes in {4, 8, 16, 32, 64} come from the
+ * autotuner; grid <= 0 means one workgroup per system. counter is one device
+ * int32 of scratch; results[nsys] is device memory. Asynchronous. */
+int cc_pcg_batched(int32_t nsys, int32_t n_max, const int32_t* sys_n, const int64_t* sys_indptr_off,
+                   const int64_t* sys_nnz_off, const int64_t* sys_vec_off, const int32_t* order,
+                   const int32_t* indptr, const int32_t* indices, const double* vals, const float* vals32,
+                   const double* B, double* X, double tol, int32_t maxiter, int32_t preconditioner,
+                   int32_t precision, double inner_tol, int32_t max_outer, int32_t block_threads,
+                   int32_t team_lanes, int32_t grid, int32_t* counter, cc_pcg_result* results,
+                   cc_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,20 @@
+"""CPU checks of the autotuner's candidate sets (no GPU)."""
+
+from cutcell import autotune
+
+
+def test_spmm_candidates_respect_the_wave():
+    for k in (1, 2, 4, 8, 16):
+        cands = list(autotune.spmm_candidates(k, 300.0))
+        assert cands
+        for c in cands:
+            assert k * c["nz"] <= 64 and c["grid"] in (256, 512, 1024) and c["unroll"] in (1, 4)
+
+
+def test_batched_candidates_fit_lds():
+    for n_max in (100, 2809, 4096, 8192):
+        cands = list(autotune.batched_candidates(n_max))
+        assert cands
+        for c in cands:
+            rpt = -(-n_max // c["block_threads"])
+            assert rpt <= 16 and 2 * rpt * c["block_threads"] * 8 <= 160 * 1024
