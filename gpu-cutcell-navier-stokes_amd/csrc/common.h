@@ -45,12 +45,29 @@ struct OuterState {
 // Event-timing hooks shared by the solver sources (defined in pcg.hip).
 void prof_begin_batched(hipStream_t s);
 void prof_end_batched(hipStream_t s);
+void prof_begin(int cls, hipStream_t s);
+void prof_end(hipStream_t s);
 
 template <typename T>
 struct Accum {
   using type = T;
 };
 
+}  // namespace cc
+
+#include <functional>
+
+struct cc_pcg_result;
+
+namespace cc {
+// Float64 single-right-hand-side Jacobi-PCG over an operator (pcg.hip).
+// apply(P, Q, partial, &nparts, state) launches Q = A P with per-block
+// partials of P.Q; residual(X, scratch, &rel) computes ||B - A X|| / ||B||.
+int cg_operator_f64(int n, const std::function<int(const double*, double*, double*, int*, const ColState*)>& apply,
+                    const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
+                    int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
+                    const std::function<int(const double*, double*, double*)>& residual);
+size_t cg_operator_bytes(size_t n);
 }  // namespace cc
 
 #define CC_CHECK(expr)             \

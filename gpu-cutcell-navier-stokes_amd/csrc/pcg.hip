@@ -32,6 +32,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 #include "../../include/cutcell_hip.h"
@@ -97,9 +98,10 @@ void prof_collect() {
 // ---------------------------------------------------------------------------
 
 struct SpmmConfig {
-  int nz;      // nonzero lanes per row team (K*nz <= 64); 0 = pick from the row length
+  int nz;      // nonzero slots per row team ((K/cpl)*nz <= 64); 0 = pick from the row length
   int grid;    // blocks; 0 = min(kMaxParts, rows / teams per block)
   int unroll;  // 1 or 4 independent loads in flight per lane; 0 = 4
+  int cpl;     // right-hand sides per lane (vector width); 0 = widest 16-byte load
 };
 static SpmmConfig g_spmm_cfg[17] = {};
 
@@ -168,21 +170,81 @@ __global__ __launch_bounds__(256) void k_diag(int n, const int* __restrict__ ind
   diag[row] = d;
 }
 
+// Vector access of CPL consecutive right-hand sides (one 8- or 16-byte load).
+template <typename VT, int CPL>
+struct VecT;
+template <>
+struct VecT<double, 1> {
+  using T = double;
+};
+template <>
+struct VecT<double, 2> {
+  using T = double2;
+};
+template <>
+struct VecT<float, 1> {
+  using T = float;
+};
+template <>
+struct VecT<float, 2> {
+  using T = float2;
+};
+template <>
+struct VecT<float, 4> {
+  using T = float4;
+};
+
+template <typename VT, int CPL>
+__device__ inline void vload(const VT* __restrict__ p, VT* out) {
+  if constexpr (CPL == 1) {
+    out[0] = *p;
+  } else {
+    using V = typename VecT<VT, CPL>::T;
+    const V v = *reinterpret_cast<const V*>(p);
+    out[0] = v.x;
+    out[1] = v.y;
+    if constexpr (CPL == 4) {
+      out[2] = v.z;
+      out[3] = v.w;
+    }
+  }
+}
+
+template <typename VT, int CPL>
+__device__ inline void vstore(VT* __restrict__ p, const VT* in) {
+  if constexpr (CPL == 1) {
+    *p = in[0];
+  } else {
+    using V = typename VecT<VT, CPL>::T;
+    V v;
+    v.x = in[0];
+    v.y = in[1];
+    if constexpr (CPL == 4) {
+      v.z = in[2];
+      v.w = in[3];
+    }
+    *reinterpret_cast<V*>(p) = v;
+  }
+}
+
 // MODE 0: Y = A X, partial[block][c] = sum_rows X.Y      (p.q of CG)
 // MODE 1: Y = B - A X, partial[block][c] = sum_rows Y.Y  (true residual)
-template <typename MT, typename VT, int K, int NZ, int MODE, int UNR>
+// A row team has KL = K/CPL lanes per nonzero slot (each lane owns CPL
+// consecutive right-hand sides, loaded as one vector) and NZ slots.
+template <typename MT, typename VT, int K, int NZ, int MODE, int UNR, int CPL>
 __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ indptr, const int* __restrict__ indices,
                                               const MT* __restrict__ vals, const VT* __restrict__ X,
                                               VT* __restrict__ Y, const double* __restrict__ B,
                                               double* __restrict__ partial, const ColState* __restrict__ st) {
-  constexpr int T = K * NZ;
+  constexpr int KL = K / CPL;
+  constexpr int T = KL * NZ;
   constexpr int TPW = 64 / T;
-  __shared__ double smem[16 * 16];
+  __shared__ double smem[4 * 16];
   if (all_stopped<K>(st)) return;
   using AT = typename Accum<VT>::type;
   const int lane = threadIdx.x & 63;
-  const int c = lane % K;
-  const int zs = (lane / K) % NZ;
+  const int c0 = (lane % KL) * CPL;
+  const int zs = (lane / KL) % NZ;
   const int wave = threadIdx.x >> 6;
   const int wpb = blockDim.x >> 6;
   const int tpb = wpb * TPW;
@@ -194,45 +256,92 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
   const int rows_per = (n + G - 1) / G;
   const int r0 = chunk * rows_per;
   const int r1 = min(n, r0 + rows_per);
-  double dot = 0.0;
+  double dot[CPL];
+#pragma unroll
+  for (int m = 0; m < CPL; ++m) dot[m] = 0.0;
   for (int row = r0 + team; row < r1; row += tpb) {
     const int beg = indptr[row];
     const int len = indptr[row + 1] - beg;
     const int* __restrict__ ixp = indices + beg;
     const MT* __restrict__ vap = vals + beg;
-    AT acc = 0;
+    AT acc[CPL];
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) acc[m] = 0;
     int j = zs;
     if constexpr (UNR == 4) {
-      AT a1 = 0, a2 = 0, a3 = 0;
-      for (; j + 3 * NZ < len; j += 4 * NZ) {
-        const int c0 = ixp[j], c1 = ixp[j + NZ], c2 = ixp[j + 2 * NZ], c3 = ixp[j + 3 * NZ];
-        const MT v0 = vap[j], v1 = vap[j + NZ], v2 = vap[j + 2 * NZ], v3 = vap[j + 3 * NZ];
-        const VT x0 = X[(size_t)c0 * K + c], x1 = X[(size_t)c1 * K + c];
-        const VT x2 = X[(size_t)c2 * K + c], x3 = X[(size_t)c3 * K + c];
-        acc += (AT)v0 * (AT)x0;
-        a1 += (AT)v1 * (AT)x1;
-        a2 += (AT)v2 * (AT)x2;
-        a3 += (AT)v3 * (AT)x3;
-      }
-      acc = (acc + a1) + (a2 + a3);
-    }
-    for (; j < len; j += NZ) acc += (AT)vap[j] * (AT)X[(size_t)ixp[j] * K + c];
+      AT a1[CPL], a2[CPL], a3[CPL];
 #pragma unroll
-    for (int o = K; o < T; o <<= 1) acc += __shfl_xor(acc, o, 64);
+      for (int m = 0; m < CPL; ++m) a1[m] = a2[m] = a3[m] = 0;
+      for (; j + 3 * NZ < len; j += 4 * NZ) {
+        const int q0 = ixp[j], q1 = ixp[j + NZ], q2 = ixp[j + 2 * NZ], q3 = ixp[j + 3 * NZ];
+        const AT v0 = (AT)vap[j], v1 = (AT)vap[j + NZ], v2 = (AT)vap[j + 2 * NZ], v3 = (AT)vap[j + 3 * NZ];
+        VT x0[CPL], x1[CPL], x2[CPL], x3[CPL];
+        vload<VT, CPL>(X + (size_t)q0 * K + c0, x0);
+        vload<VT, CPL>(X + (size_t)q1 * K + c0, x1);
+        vload<VT, CPL>(X + (size_t)q2 * K + c0, x2);
+        vload<VT, CPL>(X + (size_t)q3 * K + c0, x3);
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+          acc[m] += v0 * (AT)x0[m];
+          a1[m] += v1 * (AT)x1[m];
+          a2[m] += v2 * (AT)x2[m];
+          a3[m] += v3 * (AT)x3[m];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) acc[m] = (acc[m] + a1[m]) + (a2[m] + a3[m]);
+    }
+    for (; j < len; j += NZ) {
+      const AT v = (AT)vap[j];
+      VT xv[CPL];
+      vload<VT, CPL>(X + (size_t)ixp[j] * K + c0, xv);
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) acc[m] += v * (AT)xv[m];
+    }
+#pragma unroll
+    for (int o = KL; o < T; o <<= 1)
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) acc[m] += __shfl_xor(acc[m], o, 64);
     if (zs == 0) {
-      const size_t e = (size_t)row * K + c;
+      const size_t e = (size_t)row * K + c0;
+      VT yv[CPL];
       if (MODE == 0) {
-        const VT y = (VT)acc;
-        Y[e] = y;
-        if (partial) dot += (double)X[e] * (double)y;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) yv[m] = (VT)acc[m];
+        vstore<VT, CPL>(Y + e, yv);
+        if (partial) {
+          VT xv[CPL];
+          vload<VT, CPL>(X + e, xv);
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) dot[m] += (double)xv[m] * (double)yv[m];
+        }
       } else {
-        const double y = B[e] - (double)acc;
-        Y[e] = (VT)y;
-        dot += y * y;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+          const double y = B[e + m] - (double)acc[m];
+          yv[m] = (VT)y;
+          dot[m] += y * y;
+        }
+        vstore<VT, CPL>(Y + e, yv);
       }
     }
   }
-  if (partial) block_col_sum<K>(dot, partial + (size_t)b * K, smem);
+  if (partial) {
+    // Sum over the lanes that own the same columns, then over waves.
+#pragma unroll
+    for (int o = KL; o < 64; o <<= 1)
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) dot[m] += __shfl_xor(dot[m], o, 64);
+    if (lane < KL)
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) smem[wave * K + c0 + m] = dot[m];
+    __syncthreads();
+    if (threadIdx.x < K) {
+      double s = 0.0;
+      for (int w = 0; w < wpb; ++w) s += smem[w * K + threadIdx.x];
+      partial[(size_t)b * K + threadIdx.x] = s;
+    }
+  }
 }
 
 // x = 0, r = src * inv_scale[c], z = r/d, p = z; partials (r.z, r.r).
@@ -468,20 +577,26 @@ static int stream_grid(size_t work) {
   return (int)g;
 }
 
-template <int K>
+template <typename VT, int K>
+static int default_cpl() {
+  const int w = 16 / (int)sizeof(VT);  // 16-byte vector loads
+  return K >= w ? w : K;
+}
+
+template <int KL>
 static int default_nz(double avg_row) {
-  if (K > 1) return 64 / K;
+  if (KL > 1) return 64 / KL;
   if (avg_row >= 96.0) return 64;
   if (avg_row >= 40.0) return 32;
   if (avg_row >= 20.0) return 16;
   return 8;
 }
 
-template <typename MT, typename VT, int K, int NZ, int MODE>
+template <typename MT, typename VT, int K, int NZ, int MODE, int CPL>
 static int spmm_launch_nz(int n, int unroll, int grid_req, const int* indptr, const int* indices, const MT* vals,
                           const VT* X, VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s,
                           int* grid_out) {
-  constexpr int T = K * NZ;
+  constexpr int T = (K / CPL) * NZ;
   const int teams_per_block = (kBlock / 64) * (64 / T);
   int grid = grid_req;
   if (grid <= 0) grid = (n + teams_per_block - 1) / teams_per_block;
@@ -489,26 +604,27 @@ static int spmm_launch_nz(int n, int unroll, int grid_req, const int* indptr, co
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
   if (unroll >= 4)
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals, X, Y,
-                       B, partial, st);
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals,
+                       X, Y, B, partial, st);
   else
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals, X, Y,
-                       B, partial, st);
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1, CPL>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals,
+                       X, Y, B, partial, st);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
 
-template <typename MT, typename VT, int K, int MODE>
-static int spmm_launch(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
-                       VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+template <typename MT, typename VT, int K, int MODE, int CPL>
+static int spmm_launch_cpl(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
+                           VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  constexpr int KL = K / CPL;
   const SpmmConfig cfg = g_spmm_cfg[K];
-  const int nz = cfg.nz > 0 ? cfg.nz : default_nz<K>(avg_row);
+  const int nz = cfg.nz > 0 ? cfg.nz : default_nz<KL>(avg_row);
   const int unroll = cfg.unroll > 0 ? cfg.unroll : 4;
   const int grid = cfg.grid;
-#define CC_NZ(NZV)                                                                                              \
-  if (nz == NZV && K * NZV <= 64)                                                                               \
-    return spmm_launch_nz<MT, VT, K, (K * NZV <= 64 ? NZV : 1), MODE>(n, unroll, grid, indptr, indices, vals, X, \
-                                                                       Y, B, partial, st, s, grid_out);
+#define CC_NZ(NZV)                                                                                                  \
+  if (nz == NZV && KL * NZV <= 64)                                                                                  \
+    return spmm_launch_nz<MT, VT, K, (KL * NZV <= 64 ? NZV : 1), MODE, CPL>(n, unroll, grid, indptr, indices, vals, \
+                                                                          X, Y, B, partial, st, s, grid_out);
   CC_NZ(64)
   CC_NZ(32)
   CC_NZ(16)
@@ -518,6 +634,20 @@ static int spmm_launch(int n, double avg_row, const int* indptr, const int* indi
   CC_NZ(1)
 #undef CC_NZ
   return CC_E_ARG;
+}
+
+template <typename MT, typename VT, int K, int MODE>
+static int spmm_launch(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
+                       VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  int cpl = g_spmm_cfg[K].cpl > 0 ? g_spmm_cfg[K].cpl : default_cpl<VT, K>();
+  if (cpl > K || cpl * (int)sizeof(VT) > 16) cpl = default_cpl<VT, K>();
+  if (cpl >= 4 && sizeof(VT) == 4 && K >= 4)
+    return spmm_launch_cpl<MT, VT, K, MODE, (sizeof(VT) == 4 && K >= 4 ? 4 : 1)>(n, avg_row, indptr, indices, vals, X,
+                                                                                 Y, B, partial, st, s, grid_out);
+  if (cpl >= 2 && K >= 2)
+    return spmm_launch_cpl<MT, VT, K, MODE, (K >= 2 ? 2 : 1)>(n, avg_row, indptr, indices, vals, X, Y, B, partial, st,
+                                                              s, grid_out);
+  return spmm_launch_cpl<MT, VT, K, MODE, 1>(n, avg_row, indptr, indices, vals, X, Y, B, partial, st, s, grid_out);
 }
 
 static double avg_row_len(int n, const int* indptr, hipStream_t s) {
@@ -578,14 +708,15 @@ static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external) {
   return b;
 }
 
-// The device-resident CG loop for K columns. src is the float64 right-hand
-// side block; inv_scale (device, may be null) scales it into VT. The
-// per-column state is left in b.st (and copied to host_st).
-template <typename MT, typename VT, int K>
-static int cg_run(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const double* src,
-                  const double* inv_scale, const int* run_mask, CgBuffers<VT>& b, int precond, double tol,
-                  int maxiter, int check_every, hipStream_t s, std::vector<ColState>& host_st,
-                  double* hist = nullptr) {
+// The device-resident CG loop for K columns over any operator:
+// apply(P, Q, partial, &nparts) launches Q = A P and writes the per-block
+// partials of P.Q. src is the float64 right-hand side block; inv_scale
+// (device, may be null) scales it into VT. The per-column state is left in
+// b.st (and copied to host_st).
+template <typename VT, int K, typename ApplyFn>
+static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_scale, const int* run_mask,
+                   CgBuffers<VT>& b, int precond, double tol, int maxiter, int check_every, hipStream_t s,
+                   std::vector<ColState>& host_st, double* hist) {
   const int sg = stream_grid((size_t)n * K);
   hipLaunchKernelGGL((k_cg_setup<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, src, inv_scale, b.X, b.R, b.Z, b.P, b.diag,
                      precond, b.pa);
@@ -596,7 +727,7 @@ static int cg_run(int n, double avg_row, const int* indptr, const int* indices, 
   for (int it = 1; it <= maxiter; ++it) {
     int gs = 0;
     prof_begin(kProfSpmm, s);
-    CC_CHECK((spmm_launch<MT, VT, K, 0>(n, avg_row, indptr, indices, vals, b.P, b.Q, nullptr, b.pa, b.st, s, &gs)));
+    CC_CHECK(apply(b.P, b.Q, b.pa, &gs));
     prof_end(s);
     hipLaunchKernelGGL((k_cg_alpha<K>), dim3(1), dim3(kBlock), 0, s, b.pa, gs, b.st, it);
     prof_begin(kProfUpdate, s);
@@ -622,6 +753,48 @@ static int cg_run(int n, double avg_row, const int* indptr, const int* indices, 
   prof_collect();
   return 0;
 }
+
+template <typename MT, typename VT, int K>
+static int cg_run(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const double* src,
+                  const double* inv_scale, const int* run_mask, CgBuffers<VT>& b, int precond, double tol,
+                  int maxiter, int check_every, hipStream_t s, std::vector<ColState>& host_st,
+                  double* hist = nullptr) {
+  const ColState* st = b.st;
+  auto apply = [&](const VT* P, VT* Q, double* partial, int* gs) {
+    return spmm_launch<MT, VT, K, 0>(n, avg_row, indptr, indices, vals, P, Q, nullptr, partial, st, s, gs);
+  };
+  return cg_loop<VT, K>(n, apply, src, inv_scale, run_mask, b, precond, tol, maxiter, check_every, s, host_st, hist);
+}
+
+// Generic float64 single-right-hand-side CG over an operator (used by the
+// matrix-free operator in sumfact.hip). diag_src (device) is copied in as the
+// Jacobi diagonal. The float64 true residual is computed by `residual`.
+int cg_operator_f64(int n, const std::function<int(const double*, double*, double*, int*, const ColState*)>& apply,
+                    const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
+                    int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
+                    const std::function<int(const double*, double*, double*)>& residual) {
+  Carve cv{(char*)ws, 0, wsb};
+  CgBuffers<double> b = carve_cg<double>(cv, n, 1, X);
+  if (!cv.ok()) return CC_E_WORKSPACE;
+  if (precond && diag_src)
+    CC_CHECK(hipMemcpyAsync(b.diag, diag_src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const ColState* st = b.st;
+  auto ap = [&](const double* P, double* Q, double* partial, int* gs) { return apply(P, Q, partial, gs, st); };
+  std::vector<ColState> hs;
+  CC_CHECK((cg_loop<double, 1>(n, ap, B, nullptr, nullptr, b, precond && diag_src, tol, maxiter, check_every, s, hs,
+                               hist)));
+  double rel = 0.0;
+  CC_CHECK(residual(X, b.R, &rel));
+  result->iterations = hs[0].iterations;
+  result->status = hs[0].status == CC_RUNNING ? CC_MAXITER : hs[0].status;
+  result->outer = 1;
+  result->inner_total = hs[0].iterations;
+  result->recursive_rel = hs[0].rel;
+  result->true_rel = rel;
+  return 0;
+}
+
+size_t cg_operator_bytes(size_t n) { return cg_bytes<double>(n, 1); }
 
 // True relative residuals ||B - A X|| / ||B|| (||B|| = 0 -> 1) of the float64
 // block X, written to rel[K] (host). The per-block partial sums are combined
@@ -804,11 +977,14 @@ int cc_profile_collect(void) {
   return 0;
 }
 
-int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll) {
+int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll, int32_t cols_per_lane) {
   if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
-  if (nz_lanes < 0 || k * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
+  if (cols_per_lane != 0 && cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4) return CC_E_ARG;
+  if (cols_per_lane > k) return CC_E_ARG;
+  const int kl = k / (cols_per_lane > 0 ? cols_per_lane : 1);
+  if (nz_lanes < 0 || kl * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
   if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4)) return CC_E_ARG;
-  g_spmm_cfg[k] = SpmmConfig{nz_lanes, grid, unroll};
+  g_spmm_cfg[k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane};
   return 0;
 }
 

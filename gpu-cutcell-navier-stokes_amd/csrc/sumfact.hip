@@ -1,0 +1,386 @@
+// sumfact.hip - matrix-free aggregated operator y = E^T (K + c M) E x.
+//
+// Launch sequence of one apply (all on the caller's stream):
+//   G  u = E x on the background dof grid (interior copy, small-dof stencils)
+//   T  v = A_all u: the tensor-product operator of every background cell,
+//      Kz(x)My(x)Mx + Mz(x)Ky(x)Mx + Mz(x)My(x)(Kx + c Mx), by sum factorisation.
+//      A workgroup owns an 8x8x8 output tile, stages the (8+2p)^3 input halo in
+//      LDS and runs three banded 1D passes (x: a = Mx u, b = (Kx + c Mx) u;
+//      y: c = My b + Ky a, d = My a; z: v = Mz c + Kz d), each from LDS.
+//   S  per shell cell s (cut, or empty but touching an active dof), one wave:
+//      W_s = C_s u_s - A_s u_s (cut) or -A_s u_s (empty), A_s the cell's
+//      full tensor block built from its 1D element forms.
+//   C  v[g] += sum of the W entries of the shell cells containing g (CSR over
+//      touched dofs, fixed order: deterministic, no atomics).
+//   E  y = E^T v, with per-block partials of x.y for the CG loop.
+// The operator is the same as the explicit E^T A E of the reference's
+// restrict_spd (code/bell_3d_core.py:550-552) but never formed.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "../../include/cutcell_hip.h"
+#include "common.h"
+
+namespace cc {
+
+constexpr int kTile = 8;
+
+__device__ inline int wrap_or_invalid(int i, int n, int periodic) {
+  if (periodic) {
+    i %= n;
+    return i < 0 ? i + n : i;
+  }
+  return (i >= 0 && i < n) ? i : -1;
+}
+
+__global__ __launch_bounds__(256) void k_mf_gather(cc_sumfact_desc d, const double* __restrict__ x,
+                                                   double* __restrict__ u) {
+  const int n_full = d.nb[0] * d.nb[1] * d.nb[2];
+  for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_full; g += gridDim.x * blockDim.x) {
+    const int r = d.full2red[g];
+    double val = 0.0;
+    if (r >= 0) {
+      val = x[r];
+    } else {
+      const int sm = d.full2small[g];
+      if (sm >= 0)
+        for (int t = d.small_ptr[sm]; t < d.small_ptr[sm + 1]; ++t) val += d.small_w[t] * x[d.small_col[t]];
+    }
+    u[g] = val;
+  }
+}
+
+template <int P1>
+__global__ __launch_bounds__(256) void k_mf_tensor(cc_sumfact_desc d, const double* __restrict__ u,
+                                                   double* __restrict__ v) {
+  constexpr int p = P1 - 1;
+  constexpr int W = 2 * p + 1;
+  constexpr int T = kTile;
+  constexpr int R = T + 2 * p;
+  __shared__ double su[R * R * R];
+  __shared__ double sa[R * R * T];
+  __shared__ double sb[R * R * T];
+  __shared__ double sc[R * T * T];
+  __shared__ double sd[R * T * T];
+  const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T, z0 = blockIdx.z * T;
+  const int tid = threadIdx.x;
+  const double c = d.mass_coef;
+  for (int i = tid; i < R * R * R; i += blockDim.x) {
+    const int lz = i / (R * R), ly = (i / R) % R, lx = i % R;
+    const int gx = wrap_or_invalid(x0 - p + lx, nbx, d.periodic[0]);
+    const int gy = wrap_or_invalid(y0 - p + ly, nby, d.periodic[1]);
+    const int gz = wrap_or_invalid(z0 - p + lz, nbz, d.periodic[2]);
+    su[i] = (gx >= 0 && gy >= 0 && gz >= 0) ? u[((size_t)gz * nby + gy) * nbx + gx] : 0.0;
+  }
+  __syncthreads();
+  // x pass: rows (lz, ly) of the halo region, outputs tx.
+  const double* __restrict__ bmx = d.band_m[0];
+  const double* __restrict__ bkx = d.band_k[0];
+  for (int i = tid; i < R * R * T; i += blockDim.x) {
+    const int lz = i / (R * T), ly = (i / T) % R, tx = i % T;
+    const int gx = x0 + tx;
+    double a = 0.0, b = 0.0;
+    if (gx < nbx) {
+      const double* uu = su + (lz * R + ly) * R + tx;
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        const double m = bmx[gx * W + k];
+        const double kk = bkx[gx * W + k];
+        a += m * uu[k];
+        b += (kk + c * m) * uu[k];
+      }
+    }
+    sa[i] = a;
+    sb[i] = b;
+  }
+  __syncthreads();
+  // y pass: c = My b + Ky a, d = My a on (lz, ty, tx).
+  const double* __restrict__ bmy = d.band_m[1];
+  const double* __restrict__ bky = d.band_k[1];
+  for (int i = tid; i < R * T * T; i += blockDim.x) {
+    const int lz = i / (T * T), ty = (i / T) % T, tx = i % T;
+    const int gy = y0 + ty;
+    double cc_ = 0.0, dd = 0.0;
+    if (gy < nby) {
+#pragma unroll
+      for (int k = 0; k < W; ++k) {
+        const double m = bmy[gy * W + k];
+        const double kk = bky[gy * W + k];
+        const int li = (lz * R + ty + k) * T + tx;
+        cc_ += m * sb[li] + kk * sa[li];
+        dd += m * sa[li];
+      }
+    }
+    sc[i] = cc_;
+    sd[i] = dd;
+  }
+  __syncthreads();
+  // z pass: v = Mz c + Kz d.
+  const double* __restrict__ bmz = d.band_m[2];
+  const double* __restrict__ bkz = d.band_k[2];
+  for (int i = tid; i < T * T * T; i += blockDim.x) {
+    const int tz = i / (T * T), ty = (i / T) % T, tx = i % T;
+    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+    if (gx >= nbx || gy >= nby || gz >= nbz) continue;
+    double out = 0.0;
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const int li = ((tz + k) * T + ty) * T + tx;
+      out += bmz[gz * W + k] * sc[li] + bkz[gz * W + k] * sd[li];
+    }
+    v[((size_t)gz * nby + gy) * nbx + gx] = out;
+  }
+}
+
+// One wave per shell cell: W_s = C_s u_s - A_s u_s (or -A_s u_s).
+template <int P1>
+__global__ __launch_bounds__(256) void k_mf_shell(cc_sumfact_desc d, const double* __restrict__ u,
+                                                  double* __restrict__ Wout) {
+  constexpr int PP = P1 * P1 * P1;
+  constexpr int WAVES = 4;
+  __shared__ double su[WAVES][PP];
+  __shared__ double sf[WAVES][6][P1 * P1];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int s = blockIdx.x * WAVES + wv;
+  const bool live = s < d.n_corr;
+  const int ex = live ? d.corr_cell[3 * s + 0] : 0;
+  const int ey = live ? d.corr_cell[3 * s + 1] : 0;
+  const int ez = live ? d.corr_cell[3 * s + 2] : 0;
+  if (live) {
+    for (int l = lane; l < PP; l += 64) {
+      const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
+      const int gx = wrap_or_invalid(ex + lx, d.nb[0], d.periodic[0]);
+      const int gy = wrap_or_invalid(ey + ly, d.nb[1], d.periodic[1]);
+      const int gz = wrap_or_invalid(ez + lz, d.nb[2], d.periodic[2]);
+      su[wv][l] = u[((size_t)gz * d.nb[1] + gy) * d.nb[0] + gx];
+    }
+    for (int l = lane; l < P1 * P1; l += 64) {
+      sf[wv][0][l] = d.elem_m[0][(size_t)ex * P1 * P1 + l];
+      sf[wv][1][l] = d.elem_k[0][(size_t)ex * P1 * P1 + l];
+      sf[wv][2][l] = d.elem_m[1][(size_t)ey * P1 * P1 + l];
+      sf[wv][3][l] = d.elem_k[1][(size_t)ey * P1 * P1 + l];
+      sf[wv][4][l] = d.elem_m[2][(size_t)ez * P1 * P1 + l];
+      sf[wv][5][l] = d.elem_k[2][(size_t)ez * P1 * P1 + l];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const int blk = d.corr_block[s];
+  const double c = d.mass_coef;
+  for (int l = lane; l < PP; l += 64) {
+    const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
+    double acc = 0.0;
+    // Full tensor block row l: sum_j (Kz My Mx + Mz Ky Mx + Mz My (Kx + c Mx)) u_j.
+    for (int jz = 0; jz < P1; ++jz) {
+      const double mz = sf[wv][4][lz * P1 + jz], kz = sf[wv][5][lz * P1 + jz];
+      for (int jy = 0; jy < P1; ++jy) {
+        const double my = sf[wv][2][ly * P1 + jy], ky = sf[wv][3][ly * P1 + jy];
+        double sx_m = 0.0, sx_k = 0.0;
+#pragma unroll
+        for (int jx = 0; jx < P1; ++jx) {
+          const double uj = su[wv][(jz * P1 + jy) * P1 + jx];
+          sx_m += sf[wv][0][lx * P1 + jx] * uj;
+          sx_k += sf[wv][1][lx * P1 + jx] * uj;
+        }
+        acc += (kz * my + mz * ky + c * mz * my) * sx_m + mz * my * sx_k;
+      }
+    }
+    double w = -acc;
+    if (blk >= 0) {
+      const double* __restrict__ Cb = d.blocks + (size_t)blk * PP * PP;
+      double cu = 0.0;
+      for (int j = 0; j < PP; ++j) cu += Cb[(size_t)j * PP + l] * su[wv][j];  // symmetric: column l = row l
+      w += cu;
+    }
+    Wout[(size_t)s * PP + l] = w;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mf_touch(cc_sumfact_desc d, const double* __restrict__ Win,
+                                                  double* __restrict__ v) {
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.n_touch; t += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int i = d.touch_ptr[t]; i < d.touch_ptr[t + 1]; ++i) s += Win[d.touch_idx[i]];
+    v[d.touch_dof[t]] += s;
+  }
+}
+
+// y = E^T v; partial[block] = sum_j x_j y_j (when partial != null).
+__global__ __launch_bounds__(256) void k_mf_scatter(cc_sumfact_desc d, const double* __restrict__ v,
+                                                    double* __restrict__ y, const double* __restrict__ x,
+                                                    double* __restrict__ partial, const ColState* __restrict__ st) {
+  __shared__ double red[4];
+  if (st && st[0].status != CC_RUNNING) return;
+  double dot = 0.0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n_red; j += gridDim.x * blockDim.x) {
+    double s = v[d.red2full[j]];
+    for (int t = d.et_ptr[j]; t < d.et_ptr[j + 1]; ++t) s += d.et_w[t] * v[d.small_dof[d.et_small[t]]];
+    y[j] = s;
+    if (partial) dot += x[j] * s;
+  }
+  if (partial) {
+    for (int o = 32; o > 0; o >>= 1) dot += __shfl_xor(dot, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+  }
+}
+
+// Residual r = b - y and partials of (r.r, b.b) per block.
+__global__ __launch_bounds__(256) void k_mf_residual(int n, const double* __restrict__ b, const double* __restrict__ y,
+                                                     double* __restrict__ partial) {
+  __shared__ double red[2][4];
+  double rr = 0.0, bb = 0.0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const double r = b[j] - y[j];
+    rr += r * r;
+    bb += b[j] * b[j];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    rr += __shfl_xor(rr, o, 64);
+    bb += __shfl_xor(bb, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = rr;
+    red[1][threadIdx.x >> 6] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+    partial[2 * blockIdx.x + 1] = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+  }
+}
+
+struct MfBuffers {
+  double *u, *v, *W;
+};
+
+static size_t mf_bytes(const cc_sumfact_desc* d) {
+  const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
+  const int P1 = d->degree + 1;
+  const size_t pp = (size_t)P1 * P1 * P1;
+  return 2 * n_full * sizeof(double) + (size_t)d->n_corr * pp * sizeof(double) + 3 * 256;
+}
+
+static MfBuffers carve_mf(const cc_sumfact_desc* d, char* base) {
+  const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  MfBuffers b;
+  b.u = reinterpret_cast<double*>(base);
+  b.v = reinterpret_cast<double*>(base + al(n_full * sizeof(double)));
+  b.W = reinterpret_cast<double*>(base + 2 * al(n_full * sizeof(double)));
+  return b;
+}
+
+static int grid1d(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > (size_t)kMaxParts) g = kMaxParts;
+  return (int)(g < 1 ? 1 : g);
+}
+
+template <int P1>
+static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double* x, double* y, double* partial,
+                      int* nparts, const ColState* st, hipStream_t s) {
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  hipLaunchKernelGGL(k_mf_gather, dim3(grid1d(n_full)), dim3(256), 0, s, d, x, b.u);
+  dim3 tiles((d.nb[0] + kTile - 1) / kTile, (d.nb[1] + kTile - 1) / kTile, (d.nb[2] + kTile - 1) / kTile);
+  hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v);
+  if (d.n_corr > 0) {
+    hipLaunchKernelGGL((k_mf_shell<P1>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
+    hipLaunchKernelGGL(k_mf_touch, dim3(grid1d(d.n_touch)), dim3(256), 0, s, d, b.W, b.v);
+  }
+  const int gs = grid1d(d.n_red);
+  hipLaunchKernelGGL(k_mf_scatter, dim3(gs), dim3(256), 0, s, d, b.v, y, x, partial, st);
+  if (nparts) *nparts = gs;
+  return (int)hipGetLastError();
+}
+
+static int mf_apply(const cc_sumfact_desc& d, const MfBuffers& b, const double* x, double* y, double* partial,
+                    int* nparts, const ColState* st, hipStream_t s) {
+  switch (d.degree) {
+    case 2: return mf_apply_t<3>(d, b, x, y, partial, nparts, st, s);
+    case 3: return mf_apply_t<4>(d, b, x, y, partial, nparts, st, s);
+    default: return CC_E_ARG;
+  }
+}
+
+static bool desc_ok(const cc_sumfact_desc* d) {
+  if (!d || (d->degree != 2 && d->degree != 3) || d->n_red <= 0) return false;
+  for (int k = 0; k < 3; ++k) {
+    if (d->nb[k] <= 0 || d->ne[k] <= 0 || !d->band_m[k] || !d->band_k[k] || !d->elem_m[k] || !d->elem_k[k])
+      return false;
+    if (d->periodic[k] ? d->nb[k] != d->ne[k] : d->nb[k] != d->ne[k] + d->degree) return false;
+  }
+  if (!d->red2full || !d->full2red || !d->full2small || !d->small_ptr || !d->et_ptr) return false;
+  if (d->n_corr > 0 && (!d->corr_cell || !d->corr_block || !d->blocks || !d->touch_dof || !d->touch_ptr ||
+                        !d->touch_idx))
+    return false;
+  return true;
+}
+
+}  // namespace cc
+
+using namespace cc;
+
+extern "C" {
+
+size_t cc_sumfact_workspace_bytes(const cc_sumfact_desc* d, int32_t solve) {
+  if (!desc_ok(d)) return 0;
+  size_t b = mf_bytes(d) + 512;
+  if (solve) b += cg_operator_bytes((size_t)d->n_red) + (size_t)kMaxParts * 2 * sizeof(double) + 512;
+  return b;
+}
+
+int cc_sumfact_apply(const cc_sumfact_desc* d, const double* x, double* y, void* workspace, size_t workspace_bytes,
+                     cc_stream_t stream) {
+  if (!desc_ok(d) || !x || !y || !workspace) return CC_E_ARG;
+  if (workspace_bytes < cc_sumfact_workspace_bytes(d, 0)) return CC_E_WORKSPACE;
+  MfBuffers b = carve_mf(d, (char*)workspace);
+  return mf_apply(*d, b, x, y, nullptr, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* bvec, double* x, double tol, int32_t maxiter,
+                   int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                   cc_pcg_result* result, double* history, cc_stream_t stream) {
+  if (!desc_ok(d) || !bvec || !x || !workspace || !result || maxiter < 0) return CC_E_ARG;
+  if (workspace_bytes < cc_sumfact_workspace_bytes(d, 1)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  MfBuffers mb = carve_mf(d, base);
+  const size_t off = (mf_bytes(d) + 511) & ~size_t(255);
+  double* rpart = reinterpret_cast<double*>(base + off);
+  char* cgws = base + off + (((size_t)kMaxParts * 2 * sizeof(double) + 511) & ~size_t(255));
+  const size_t cgb = workspace_bytes - (size_t)(cgws - base);
+  const cc_sumfact_desc desc = *d;
+  auto apply = [&](const double* P, double* Q, double* partial, int* gs, const ColState* st) {
+    return mf_apply(desc, mb, P, Q, partial, gs, st, s);
+  };
+  auto residual = [&](const double* X, double* scratch, double* rel) {
+    CC_CHECK(mf_apply(desc, mb, X, scratch, nullptr, nullptr, nullptr, s));
+    const int g = grid1d(desc.n_red);
+    hipLaunchKernelGGL(k_mf_residual, dim3(g), dim3(256), 0, s, desc.n_red, bvec, scratch, rpart);
+    CC_CHECK(hipGetLastError());
+    std::vector<double> hp(2 * (size_t)g);
+    CC_CHECK(hipMemcpyAsync(hp.data(), rpart, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    CC_CHECK(hipStreamSynchronize(s));
+    double rr = 0.0, bb = 0.0;
+    for (int i = 0; i < g; ++i) {
+      rr += hp[2 * i];
+      bb += hp[2 * i + 1];
+    }
+    double bn = std::sqrt(bb);
+    if (bn == 0.0) bn = 1.0;
+    *rel = std::sqrt(rr) / bn;
+    return 0;
+  };
+  return cg_operator_f64(desc.n_red, apply, preconditioner ? desc.diag : nullptr, bvec, x, tol, maxiter,
+                         preconditioner, check_every, cgws, cgb, result, history, s, residual);
+}
+
+}  // extern "C"

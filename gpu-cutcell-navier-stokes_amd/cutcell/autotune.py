@@ -87,13 +87,18 @@ def _time_ms(fn, reps: int) -> float:
 # ---------------------------------------------------------------------------
 
 
-def spmm_candidates(k: int, mean_row: float):
-    nzs = [nz for nz in (64, 32, 16, 8, 4, 2, 1) if k * nz <= 64 and nz * 2 <= max(mean_row, 2) * 2]
-    nzs = nzs[:3] if nzs else [max(1, 64 // k)]
-    for nz in nzs:
-        for grid in (256, 512, 1024):
-            for unroll in (1, 4):
-                yield dict(nz=nz, grid=grid, unroll=unroll)
+def spmm_candidates(k: int, mean_row: float, value_bytes: int = 8):
+    """Launch shapes: vector width (right-hand sides per lane), nonzero slots
+    per row team, grid and unroll depth."""
+    cpls = [c for c in (1, 2, 4) if c <= k and c * value_bytes <= 16]
+    for cpl in cpls:
+        kl = k // cpl
+        nzs = [nz for nz in (64, 32, 16, 8, 4, 2, 1) if kl * nz <= 64 and nz <= max(mean_row, 1)]
+        nzs = nzs[:2] if nzs else [max(1, 64 // kl)]
+        for nz in nzs:
+            for grid in (512, 1024):
+                for unroll in (1, 4):
+                    yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll)
 
 
 def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool = False) -> dict:
@@ -106,7 +111,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     table = _load()
     if key in table and not force:
         cfg = table[key]
-        lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"])
+        lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
         return cfg
     dt = torch.float64 if precision == "fp64" else torch.float32
     X = torch.randn(A.n, k, dtype=dt, device=A.vals.device)
@@ -115,8 +120,8 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     fn_name = "cc_spmm_csr_f64" if precision == "fp64" else "cc_spmm_csr_f32"
     fn = getattr(lib, fn_name)
     best = None
-    for cand in spmm_candidates(k, mean_row):
-        if lib.cc_spmm_set_config(k, cand["nz"], cand["grid"], cand["unroll"]) != 0:
+    for cand in spmm_candidates(k, mean_row, 8 if precision == "fp64" else 4):
+        if lib.cc_spmm_set_config(k, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) != 0:
             continue
         call = lambda: fn(A.n, k, hip._ptr(A.indptr), hip._ptr(A.indices), hip._ptr(vals), hip._ptr(X), hip._ptr(Y),
                           hip._stream())
@@ -124,7 +129,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
         if best is None or ms < best[0]:
             best = (ms, cand)
     cfg = dict(best[1], ms=best[0])
-    lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"])
+    lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
     _store(key, cfg)
     return cfg
 
@@ -132,7 +137,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
 def reset_spmm_config() -> None:
     lib = hip.load_library()
     for k in hip.SUPPORTED_K:
-        lib.cc_spmm_set_config(k, 0, 0, 0)
+        lib.cc_spmm_set_config(k, 0, 0, 0, 0)
 
 
 # ---------------------------------------------------------------------------

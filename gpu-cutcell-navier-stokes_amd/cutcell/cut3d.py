@@ -518,21 +518,18 @@ def _nearest_ordered(greville, interior_mask, lines, b, need, shape):
         r *= 2
 
 
-def polynomial_preserving_extension_3d(asm: CutAssembly3D, *, support_threshold: float = 0.04,
-                                       preserve_degree: int | None = None) -> Aggregation3D:
-    bases = asm.bases
-    degree = bases[0].degree if preserve_degree is None else int(preserve_degree)
-    gamma = asm.gamma
-    active = asm.active
+def extension_core(gamma, greville, poly, shape, lines, *, support_threshold=0.04, tie_order="stable"):
+    """The 3D extension rule on given Greville points and monomial rows.
+
+    Returns ``(E, interior, small, active, polynomial_residual)``.
+    """
+    active = np.nonzero(gamma > 0.0)[0]
     interior = np.nonzero(gamma >= support_threshold)[0]
     small = active[gamma[active] < support_threshold]
     if interior.size == 0:
         raise ValueError("no interior coordinates")
     n_full = gamma.size
-    shape = (bases[2].n_basis, bases[1].n_basis, bases[0].n_basis)
-    poly = monomial_rows_3d(bases, degree)
     n_terms = poly.shape[0]
-    greville = greville_3d(bases)
     col = np.full(n_full, -1, dtype=np.int64)
     col[interior] = np.arange(interior.size)
     imask = np.zeros(n_full, dtype=bool)
@@ -541,9 +538,12 @@ def polynomial_preserving_extension_3d(asm: CutAssembly3D, *, support_threshold:
     cols_t = [np.arange(interior.size, dtype=np.int64)]
     vals_t = [np.ones(interior.size)]
     cap = min(interior.size, 6 * n_terms)
-    lines = [b_.greville for b_ in bases]
+    g_int = greville[interior]
     for b in small:
-        ordered = _nearest_ordered(greville, imask, lines, int(b), min(cap + 8, interior.size), shape)
+        if tie_order == "numpy-default":
+            ordered = interior[np.argsort(np.linalg.norm(g_int - greville[int(b)], axis=1))]
+        else:
+            ordered = _nearest_ordered(greville, imask, lines, int(b), min(cap + 8, interior.size), shape)
         k = n_terms
         chosen = ordered[: k + 8]
         C = poly[:, chosen]
@@ -562,7 +562,27 @@ def polynomial_preserving_extension_3d(asm: CutAssembly3D, *, support_threshold:
     res = 0.0
     for row in poly:
         res = max(res, float(np.max(np.abs(E @ row[interior] - row * active_mask))))
-    return Aggregation3D(E=E, interior=interior, small_support=small, active=active, gamma=gamma,
+    return E, interior, small, active, res
+
+
+def polynomial_preserving_extension_3d(asm: CutAssembly3D, *, support_threshold: float = 0.04,
+                                       preserve_degree: int | None = None, tie_order: str = "stable") -> Aggregation3D:
+    """Sparse ``E`` (``n_full x n_interior``); see the module docstring for the rule.
+
+    ``tie_order="stable"`` breaks equal Greville distances by dof index
+    (windowed nearest-neighbour search); ``"numpy-default"`` sorts the full
+    interior distance row with NumPy's default (unstable) ``argsort`` as the
+    reference does, whose tie order is that of this host's NumPy build.
+    """
+    bases = asm.bases
+    degree = bases[0].degree if preserve_degree is None else int(preserve_degree)
+    shape = (bases[2].n_basis, bases[1].n_basis, bases[0].n_basis)
+    poly = monomial_rows_3d(bases, degree)
+    greville = greville_3d(bases)
+    lines = [b_.greville for b_ in bases]
+    E, interior, small, active, res = extension_core(asm.gamma, greville, poly, shape, lines,
+                                                     support_threshold=support_threshold, tie_order=tie_order)
+    return Aggregation3D(E=E, interior=interior, small_support=small, active=active, gamma=asm.gamma,
                          polynomial_residual=res)
 
 

@@ -60,8 +60,11 @@ _SIGNATURES = {
     "cc_profile_reset": ([], ctypes.c_int),
     "cc_profile_read": ([_I32, _P, _P], ctypes.c_int),
     "cc_profile_collect": ([], ctypes.c_int),
-    "cc_spmm_set_config": ([_I32, _I32, _I32, _I32], ctypes.c_int),
+    "cc_spmm_set_config": ([_I32, _I32, _I32, _I32, _I32], ctypes.c_int),
     "cc_spmm_csr_f32": ([_I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_sumfact_workspace_bytes": ([_P, _I32], _SZ),
+    "cc_sumfact_apply": ([_P, _P, _P, _P, _SZ, _P], ctypes.c_int),
+    "cc_pcg_sumfact": ([_P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int),
     "cc_pcg_batched": (
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _F64, _I32, _I32, _I32,
          _I32, _P, _P, _P],

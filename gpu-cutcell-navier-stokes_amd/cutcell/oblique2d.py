@@ -243,11 +243,15 @@ def assemble_oblique_cut(
     *,
     with_sbp: bool = False,
     source=None,
+    flux=None,
 ) -> ObliqueCutSystem2D:
     """Assemble ``H`` and ``K`` (and optionally ``Qx, Qy, Sx, Sy`` and a load).
 
     ``source`` is an optional vectorised callable ``f(x, y)``; when given, the
     load vector ``int_Omega f phi_i`` is assembled with the same rules.
+    ``flux`` is an optional callable returning ``(g_x, g_y)`` at boundary
+    points; its normal component is added as the natural boundary term
+    ``int_{dOmega} (g . n) phi_i``.
     The default cut-cell rule order is ``max(10, p+5)``
     (``code/oblique_cut_sbp.py:293``).
     """
@@ -274,6 +278,7 @@ def assemble_oblique_cut(
     tH, tK = _Coo(n), _Coo(n)
     tQx, tQy, tSx, tSy = _Coo(n), _Coo(n), _Coo(n), _Coo(n)
     load = np.zeros(n) if source is not None else None
+    need_edges = with_sbp or flux is not None
 
     # --- uncut cells: tensor blocks, vectorised over cells --------------------
     fy, fx = np.nonzero(all_in)
@@ -325,7 +330,7 @@ def assemble_oblique_cut(
     min_frac = 1.0
     cy, cx = np.nonzero(any_in & ~all_in)
     boundary_cells = set(zip(cy.tolist(), cx.tolist()))
-    if with_sbp:
+    if need_edges:
         # Uncut cells on the square boundary carry boundary forms too.
         for iy in range(N):
             for ix in range(N):
@@ -356,7 +361,7 @@ def assemble_oblique_cut(
                 tQy.add(R, C, (phi * wq) @ dpy.T)
             if source is not None:
                 load[rows] += phi @ (source(xq, yq) * wq)
-        if with_sbp:
+        if need_edges:
             for k in range(len(clipped)):
                 a = clipped[k]
                 b = clipped[(k + 1) % len(clipped)]
@@ -366,9 +371,13 @@ def assemble_oblique_cut(
                 if np.allclose(nrm, 0.0):
                     continue
                 phe, _, _ = local_tensor(ix, iy, xe, ye)
-                bm = (phe * we) @ phe.T
-                tSx.add(R, C, nrm[0] * bm)
-                tSy.add(R, C, nrm[1] * bm)
+                if with_sbp:
+                    bm = (phe * we) @ phe.T
+                    tSx.add(R, C, nrm[0] * bm)
+                    tSy.add(R, C, nrm[1] * bm)
+                if flux is not None and load is not None:
+                    gx, gy = flux(xe, ye)
+                    load[rows] += phe @ ((gx * nrm[0] + gy * nrm[1]) * we)
 
     # Background mass diagonal: kron of the 1D full-interval masses.
     m1x = np.zeros(nx)
@@ -565,18 +574,21 @@ def oblique_system(
 ) -> CutSystem:
     """Screened Poisson ``-lap u + u = f`` on the half-plane-cut unit square.
 
-    ``rhs="manufactured"`` uses ``f = (2 pi^2 + 1) sin(pi x) sin(pi y)`` so that
-    ``u = sin(pi x) sin(pi y)`` solves the strong problem; ``rhs="gaussian"``
-    draws ``N(0, 1)`` entries from ``default_rng(seed)``.
+    ``rhs="manufactured"`` uses ``f = (2 pi^2 + 1) sin(pi x) sin(pi y)`` and the
+    natural boundary data ``grad u . n`` of ``u = sin(pi x) sin(pi y)`` on the
+    cut and on the square, so ``u`` solves the weak problem the matrix
+    discretises; ``rhs="gaussian"`` draws ``N(0, 1)`` entries from
+    ``default_rng(seed)``.
     """
     bx = make_open_uniform_basis(n_elements, degree)
     by = make_open_uniform_basis(n_elements, degree)
     cut = HalfPlaneCut.from_angle(angle, offset)
     q = quadrature_order or max(24, 2 * degree + 14)
-    f = None
+    f = g = None
     if rhs == "manufactured":
         f = lambda x, y: (2.0 * np.pi**2 + 1.0) * np.sin(np.pi * x) * np.sin(np.pi * y)
-    sysm = assemble_oblique_cut(bx, by, cut, quadrature_order=q, source=f)
+        g = lambda x, y: (np.pi * np.cos(np.pi * x) * np.sin(np.pi * y), np.pi * np.sin(np.pi * x) * np.cos(np.pi * y))
+    sysm = assemble_oblique_cut(bx, by, cut, quadrature_order=q, source=f, flux=g)
     agg = polynomial_preserving_extension(sysm, 0.05, degree, active_tol=active_tol)
     A = sym(pullback(sysm.K, agg.E) + pullback(sysm.H, agg.E))
     if rhs == "manufactured":

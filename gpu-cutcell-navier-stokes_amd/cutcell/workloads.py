@@ -24,6 +24,52 @@ import numpy as np
 from .oblique2d import oblique_system, sliver_system
 
 
+def bump_levelset(x0: float = 2.5, length: float = 4.0, height: float = 1.0):
+    """Fluid above a smooth cosine bump ``y >= h/2 (1 - cos(2 pi (x - x0)/L))``
+    on ``x0 <= x <= x0 + L`` (flat wall ``y = 0`` elsewhere)."""
+
+    def phi(x, y, z):
+        inside = (x >= x0) & (x <= x0 + length)
+        wall = np.where(inside, 0.5 * height * (1.0 - np.cos(2.0 * np.pi * (x - x0) / length)), 0.0)
+        return y - wall
+
+    return phi
+
+
+def c4_grid(n_elements: int = 64, degree: int = 3):
+    from .grid3d import Axis, Grid3D
+
+    return Grid3D(tuple(Axis(n_elements, degree) for _ in range(3)))
+
+
+def c4_operator(seed: int = 0, n_elements: int = 64, degree: int = 3):
+    """Sphere-obstacle box, matrix-free (``cutcell.matfree``)."""
+    from .cut3d import SphereObstacle
+    from .matfree import build_matrix_free
+
+    lev = SphereObstacle.random(seed)
+    mf = build_matrix_free(c4_grid(n_elements, degree), lev)
+    mf.meta.update(kind="sphere3d", seed=seed, center=lev.center.tolist(), radius=lev.radius)
+    return mf
+
+
+def c5_grid(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3, lengths=(9.0, 3.036, 4.5)):
+    """Channel box periodic in x and z, clamped in y (periodic-hill extents)."""
+    from .grid3d import Axis, Grid3D
+
+    return Grid3D((Axis(nx, degree, lengths[0], True), Axis(ny, degree, lengths[1], False),
+                   Axis(nz, degree, lengths[2], True)))
+
+
+def c5_operator(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3):
+    """Pressure-projection operator on the bump channel, matrix-free."""
+    from .matfree import build_matrix_free
+
+    mf = build_matrix_free(c5_grid(nx, ny, nz, degree), bump_levelset())
+    mf.meta.update(kind="bump_channel", shape=[nx, ny, nz])
+    return mf
+
+
 def c1_system():
     return oblique_system(32, 4, np.deg2rad(5.0), 0.85 / 32, rhs="manufactured")
 

@@ -78,10 +78,12 @@ int cc_profile_read(int32_t which, double* total_ms, int64_t* count);
 int cc_profile_collect(void);
 
 /* SpMM/SpMV launch shape for k right-hand sides, chosen by the autotuner
- * (no reference counterpart): nz_lanes nonzero lanes per row team
- * (k*nz_lanes <= 64, power of two; 0 = from the mean row length), grid
- * blocks (0 = one block per 4..16 rows up to 1024), unroll 1 or 4 (0 = 4). */
-int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll);
+ * (no reference counterpart): cols_per_lane right-hand sides per lane, loaded
+ * as one vector (1, 2 or 4 with at most 16 bytes; 0 = widest), nz_lanes
+ * nonzero slots per row team ((k/cols_per_lane)*nz_lanes <= 64, power of
+ * two; 0 = from the mean row length), grid blocks (0 = one block per few rows
+ * up to 1024), unroll 1 or 4 (0 = 4). */
+int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll, int32_t cols_per_lane);
 
 /* y = A x, float64 (replaces the cuSPARSE product A_gpu @ p). */
 int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
@@ -155,6 +157,72 @@ int cc_pcg_batched(int32_t nsys, int32_t n_max, const int32_t* sys_n, const int6
                    int32_t precision, double inner_tol, int32_t max_outer, int32_t block_threads,
                    int32_t team_lanes, int32_t grid, int32_t* counter, cc_pcg_result* results,
                    cc_stream_t stream);
+
+/* Matrix-free aggregated operator y = E^T (K + c M) E x on a tensor B-spline
+ * background grid (no reference counterpart in code: the reference only ever
+ * forms E^T A E explicitly, restrict_spd in code/bell_3d_core.py:550-552, and
+ * multiplies it with cuSPARSE, code/fast_solver_backends.py:250; this is the
+ * same operator applied without the matrix). All pointers are device memory;
+ * the struct itself is host memory. Dof order is x fastest; on element e the
+ * live functions are e + a (a = 0..degree), modulo nb on a periodic axis.
+ *  - band_m/band_k[axis]: global 1D mass/stiffness, [nb][2*degree+1], entry
+ *    (i, i + d - degree) at [i][d];
+ *  - elem_m/elem_k[axis]: per-element 1D forms, [ne][degree+1][degree+1];
+ *  - corr_cell[n_corr][3] (ex, ey, ez): the shell of cells that are not
+ *    fully fluid but touch an active dof; corr_block[s] >= 0 indexes
+ *    blocks[][P][P] (P = (degree+1)^3, local order (z, y, x)), the cut cell's
+ *    Gram block of K + c M, and -1 marks an empty cell;
+ *  - touch_dof/touch_ptr/touch_idx: CSR over the dofs that receive shell
+ *    corrections, entries index the per-cell outputs (s * P + local);
+ *  - red2full[n_red]: interior dof of each unknown; full2red/full2small:
+ *    inverse maps (-1 elsewhere); small_ptr/small_col/small_w: rows of E for
+ *    the n_small small dofs (small_dof), columns are unknowns; et_ptr/
+ *    et_small/et_w: the same entries grouped by unknown (E^T);
+ *  - diag[n_red]: diag(E^T A E) for the Jacobi preconditioner. */
+typedef struct cc_sumfact_desc {
+  int32_t nb[3];
+  int32_t ne[3];
+  int32_t degree;
+  int32_t periodic[3];
+  double mass_coef;
+  const double* band_m[3];
+  const double* band_k[3];
+  const double* elem_m[3];
+  const double* elem_k[3];
+  int32_t n_corr;
+  const int32_t* corr_cell;
+  const int32_t* corr_block;
+  const double* blocks;
+  int32_t n_touch;
+  const int32_t* touch_dof;
+  const int32_t* touch_ptr;
+  const int32_t* touch_idx;
+  int32_t n_red;
+  int32_t n_small;
+  const int32_t* red2full;
+  const int32_t* full2red;
+  const int32_t* full2small;
+  const int32_t* small_ptr;
+  const int32_t* small_col;
+  const double* small_w;
+  const int32_t* small_dof;
+  const int32_t* et_ptr;
+  const int32_t* et_small;
+  const double* et_w;
+  const double* diag;
+} cc_sumfact_desc;
+
+/* Workspace of cc_sumfact_apply (and, with solve != 0, of cc_pcg_sumfact). */
+size_t cc_sumfact_workspace_bytes(const cc_sumfact_desc* d, int32_t solve);
+/* y = E^T (K + c M) E x, float64; degree 2 or 3. Asynchronous. */
+int cc_sumfact_apply(const cc_sumfact_desc* d, const double* x, double* y, void* workspace, size_t workspace_bytes,
+                     cc_stream_t stream);
+/* Jacobi-PCG (reference GPU-loop semantics, as cc_pcg_jacobi_f64 with k = 1)
+ * on the matrix-free operator. Synchronises; result is host memory; history
+ * as in cc_pcg_jacobi_f64 (may be NULL). */
+int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* b, double* x, double tol, int32_t maxiter,
+                   int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                   cc_pcg_result* result, double* history, cc_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -4,11 +4,13 @@ from cutcell import autotune
 
 
 def test_spmm_candidates_respect_the_wave():
-    for k in (1, 2, 4, 8, 16):
-        cands = list(autotune.spmm_candidates(k, 300.0))
-        assert cands
-        for c in cands:
-            assert k * c["nz"] <= 64 and c["grid"] in (256, 512, 1024) and c["unroll"] in (1, 4)
+    for vb in (8, 4):
+        for k in (1, 2, 4, 8, 16):
+            cands = list(autotune.spmm_candidates(k, 300.0, vb))
+            assert cands
+            for c in cands:
+                assert (k // c["cpl"]) * c["nz"] <= 64 and c["cpl"] * vb <= 16
+                assert c["grid"] in (256, 512, 1024) and c["unroll"] in (1, 4)
 
 
 def test_batched_candidates_fit_lds():
