@@ -1,0 +1,336 @@
+// cnn.hip - masked 3x3x3 convolution layers for the learned preconditioner.
+//
+// Activations live on the background dof grid (nx, ny, nz; x fastest) as
+// channel-last bfloat16 tensors [voxel][16]. A layer is
+//     out[v][co] = mask[v] * act( sum_t sum_ci W[t][ci][co] in[v + off_t][ci] )
+// with t over the 27 taps (dz, dy, dx in {-1, 0, 1}), zero padding on clamped
+// axes and wrap-around on periodic ones. As a GEMM this is M = voxels,
+// N = 16 output channels, K = 27 taps x 16 input channels: every tap is one
+// 16x16x16 MFMA (bf16 inputs, fp32 accumulate) on a wave's 16 consecutive
+// x-voxels, so a wave issues 27 MFMAs per 16 voxels.
+//
+// MFMA operand layout (v_mfma_f32_16x16x16_bf16, wave64), lane l:
+//   A (16 x 16, M x K): a[i] = A[l % 16][4 (l / 16) + i]
+//   B (16 x 16, K x N): b[i] = B[4 (l / 16) + i][l % 16]
+//   D (16 x 16, M x N): d[i] = D[4 (l / 16) + i][l % 16]
+// (checked on the device by cc_mfma_probe against a host product).
+//
+// Backward: the data gradient is the same layer with the taps reversed and
+// W transposed (prepared by cc_cnn_flip_weights), the weight gradient
+//     dW[t][ci][co] = sum_v in[v + off_t][ci] g[v][co]
+// is a GEMM over voxels (M = ci, N = co, K = voxels) with per-block fp32
+// partials reduced in a fixed order by a second kernel.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/cutcell_hip.h"
+#include "common.h"
+
+namespace cc {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Round-to-nearest-even float -> bfloat16 (NaN kept quiet).
+__device__ __host__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __host__ inline float bf2f(uint16_t h) {
+  uint32_t u = (uint32_t)h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+__device__ inline int nb_index(int i, int n, int periodic) {
+  if (periodic) {
+    i = i % n;
+    return i < 0 ? i + n : i;
+  }
+  return (i >= 0 && i < n) ? i : -1;
+}
+
+struct GridArgs {
+  int nx, ny, nz;
+  int px, py, pz;  // periodic flags
+};
+
+// Forward (and data-gradient) layer. One wave = 16 consecutive x voxels of
+// one (y, z) row; ACT: 0 none, 1 relu. gate (optional, bf16 [voxel][16]):
+// multiply the result by (gate > 0), the ReLU derivative in backward mode.
+template <int ACT>
+__global__ __launch_bounds__(256) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
+                                               const uint16_t* __restrict__ W, const uint8_t* __restrict__ mask,
+                                               const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
+                                               float* __restrict__ out_f32) {
+  __shared__ float tile[4][16][17];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int row = lane % 16;  // voxel within the wave's 16 (A row / D col index)
+  const int kg = lane / 16;   // k group: channels 4 kg .. 4 kg + 3
+  const int xt = (g.nx + 15) / 16;
+  const long long wave_id = (long long)blockIdx.x * 4 + wv;
+  const long long n_waves = (long long)xt * g.ny * g.nz;
+  const bool valid = wave_id < n_waves;
+  const long long wid = valid ? wave_id : 0;
+  const int xb = (int)(wid % xt) * 16;
+  const int y = (int)((wid / xt) % g.ny);
+  const int z = (int)(wid / ((long long)xt * g.ny));
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+  if (valid) {
+    const int x = xb + row;
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      v4s b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) b[i] = (short)W[(t * 16 + 4 * kg + i) * 16 + row];
+      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const int zz = nb_index(z + dz, g.nz, g.pz);
+      const int yy = nb_index(y + dy, g.ny, g.py);
+      const int xx = (x < g.nx) ? nb_index(x + dx, g.nx, g.px) : -1;
+      v4s a = {0, 0, 0, 0};
+      if (zz >= 0 && yy >= 0 && xx >= 0) {
+        const size_t v = ((size_t)zz * g.ny + yy) * g.nx + xx;
+        const uint2 raw = *reinterpret_cast<const uint2*>(in + v * 16 + 4 * kg);
+        a[0] = (short)(raw.x & 0xffff);
+        a[1] = (short)(raw.x >> 16);
+        a[2] = (short)(raw.y & 0xffff);
+        a[3] = (short)(raw.y >> 16);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+  }
+  // acc[i] = D[voxel 4 kg + i][co = row]; transpose through LDS so that each
+  // lane stores 4 consecutive channels of one voxel. Every wave of the block
+  // reaches the barrier.
+#pragma unroll
+  for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
+  __syncthreads();
+  const int vx = xb + row;
+  if (valid && vx < g.nx) {
+    const size_t v = ((size_t)z * g.ny + y) * g.nx + vx;
+    const bool live = mask[v] != 0;
+    uint16_t h[4];
+    float f[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float val = tile[wv][row][4 * kg + i];
+      if (ACT == 1) val = val > 0.f ? val : 0.f;
+      if (gate) val = bf2f(gate[v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
+      if (!live) val = 0.f;
+      f[i] = val;
+      h[i] = f2bf(val);
+    }
+    if (out) {
+      uint2 raw;
+      raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+      raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+      *reinterpret_cast<uint2*>(out + v * 16 + 4 * kg) = raw;
+    }
+    if (out_f32) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) out_f32[v * 16 + 4 * kg + i] = f[i];
+    }
+  }
+}
+
+// Weight gradient partials: part[block][t][ci][co] = sum over the block's
+// voxel rows of in[v + off_t][ci] * gout[v][co].
+__global__ __launch_bounds__(256) void k_conv3_wgrad(GridArgs g, const uint16_t* __restrict__ in,
+                                                     const uint16_t* __restrict__ gout, float* __restrict__ part,
+                                                     int rows_per_block) {
+  __shared__ float red[27 * 256];  // 27 x 16 x 16 fp32 (27 KB), waves added in order
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int m = lane % 16;  // ci (A row) / co (B col)
+  const int kg = lane / 16;
+  const int xt = (g.nx + 15) / 16;
+  const long long n_rows = (long long)xt * g.ny * g.nz;
+  v4f acc[27];
+#pragma unroll
+  for (int t = 0; t < 27; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  const long long r0 = (long long)blockIdx.x * rows_per_block;
+  for (long long r = r0 + wv; r < r0 + rows_per_block && r < n_rows; r += 4) {
+    const int xb = (int)(r % xt) * 16;
+    const int y = (int)((r / xt) % g.ny);
+    const int z = (int)(r / ((long long)xt * g.ny));
+    // B operand: gout[voxel xb + 4 kg + i][co = m].
+    v4s bb;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = xb + 4 * kg + i;
+      bb[i] = (x < g.nx) ? (short)gout[(((size_t)z * g.ny + y) * g.nx + x) * 16 + m] : (short)0;
+    }
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
+      const int zz = nb_index(z + dz, g.nz, g.pz);
+      const int yy = nb_index(y + dy, g.ny, g.py);
+      v4s aa;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int x = xb + 4 * kg + i;
+        const int xx = (x < g.nx) ? nb_index(x + dx, g.nx, g.px) : -1;
+        aa[i] = (zz >= 0 && yy >= 0 && xx >= 0) ? (short)in[(((size_t)zz * g.ny + yy) * g.nx + xx) * 16 + m] : (short)0;
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aa, bb, acc[t], 0, 0, 0);
+    }
+  }
+  // acc[t][i] = dW[t][ci = 4 kg + i][co = m]; add the 4 waves in a fixed order.
+  for (int w = 0; w < 4; ++w) {
+    if (wv == w) {
+#pragma unroll
+      for (int t = 0; t < 27; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int e = t * 256 + (4 * kg + i) * 16 + m;
+          red[e] = (w == 0 ? 0.f : red[e]) + acc[t][i];
+        }
+    }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < 27 * 256; e += blockDim.x) part[(size_t)blockIdx.x * 27 * 256 + e] = red[e];
+}
+
+__global__ __launch_bounds__(256) void k_reduce_parts(const float* __restrict__ part, int nparts, int len,
+                                                      float* __restrict__ out, float scale, int accumulate) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < len; e += gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int b = 0; b < nparts; ++b) s += part[(size_t)b * len + e];
+    out[e] = (float)((accumulate ? (double)out[e] : 0.0) + s * scale);
+  }
+}
+
+// Data-gradient weights: Wf[t'][co][ci] = W[26 - t][ci][co] (reverse the taps,
+// swap the channel roles).
+__global__ void k_flip_weights(const uint16_t* __restrict__ W, uint16_t* __restrict__ Wf) {
+  for (int e = threadIdx.x + blockIdx.x * blockDim.x; e < 27 * 256; e += blockDim.x * gridDim.x) {
+    const int t = e / 256, ci = (e / 16) % 16, co = e % 16;
+    Wf[((26 - t) * 16 + co) * 16 + ci] = W[(t * 16 + ci) * 16 + co];
+  }
+}
+
+// Adam step on fp32 master weights; writes the bf16 copy used by the layers.
+__global__ void k_adam(int n, float* __restrict__ w, const float* __restrict__ grad, float* __restrict__ m1,
+                       float* __restrict__ m2, float lr, float b1, float b2, float eps, float c1, float c2,
+                       const uint8_t* __restrict__ frozen, uint16_t* __restrict__ wbf) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    if (frozen && frozen[i]) {
+      wbf[i] = f2bf(w[i]);
+      continue;
+    }
+    const float gi = grad[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+    wbf[i] = f2bf(w[i]);
+  }
+}
+
+// MFMA lane-layout probe: D = A B for 16x16 bf16 A (row-major) and B.
+__global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __restrict__ B, float* __restrict__ D) {
+  const int lane = threadIdx.x & 63;
+  v4s a, b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    a[i] = (short)A[(lane % 16) * 16 + 4 * (lane / 16) + i];
+    b[i] = (short)B[(4 * (lane / 16) + i) * 16 + lane % 16];
+  }
+  v4f d = {0.f, 0.f, 0.f, 0.f};
+  d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, d, 0, 0, 0);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) D[(4 * (lane / 16) + i) * 16 + lane % 16] = d[i];
+}
+
+// The preconditioner network: three masked ReLU layers and a linear one.
+int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
+                       float* out_f32, hipStream_t s) {
+  GridArgs g{net.nx, net.ny, net.nz, net.periodic[0], net.periodic[1], net.periodic[2]};
+  const long long waves = (long long)((net.nx + 15) / 16) * net.ny * net.nz;
+  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  uint16_t* acts[4] = {act0, act1, act2, act3};
+  for (int l = 0; l < 3; ++l)
+    hipLaunchKernelGGL(k_conv3<1>, dim3(blocks), dim3(256), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
+                       acts[l + 1], nullptr);
+  hipLaunchKernelGGL(k_conv3<0>, dim3(blocks), dim3(256), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
+                     out_f32);
+  return (int)hipGetLastError();
+}
+
+}  // namespace cc
+
+using namespace cc;
+
+extern "C" {
+
+int cc_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, cc_stream_t stream) {
+  if (!A || !B || !D) return CC_E_ARG;
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, (hipStream_t)stream, A, B, D);
+  return (int)hipGetLastError();
+}
+
+int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
+                         const uint16_t* weights, const uint8_t* mask, const uint16_t* gate, int32_t relu,
+                         uint16_t* out, float* out_f32, cc_stream_t stream) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !in || !weights || !mask || (!out && !out_f32)) return CC_E_ARG;
+  GridArgs g{nx, ny, nz, periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
+  const long long waves = (long long)((nx + 15) / 16) * ny * nz;
+  const long long blocks = (waves + 3) / 4;
+  if (blocks > 0x7fffffffLL) return CC_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (relu)
+    hipLaunchKernelGGL(k_conv3<1>, dim3((unsigned)blocks), dim3(256), 0, s, g, in, weights, mask, gate, out, out_f32);
+  else
+    hipLaunchKernelGGL(k_conv3<0>, dim3((unsigned)blocks), dim3(256), 0, s, g, in, weights, mask, gate, out, out_f32);
+  return (int)hipGetLastError();
+}
+
+int cc_cnn_flip_weights(const uint16_t* weights, uint16_t* flipped, cc_stream_t stream) {
+  if (!weights || !flipped) return CC_E_ARG;
+  hipLaunchKernelGGL(k_flip_weights, dim3(27), dim3(256), 0, (hipStream_t)stream, weights, flipped);
+  return (int)hipGetLastError();
+}
+
+size_t cc_cnn_wgrad_workspace_bytes(int32_t nx, int32_t ny, int32_t nz, int32_t blocks) {
+  (void)nx;
+  (void)ny;
+  (void)nz;
+  return (size_t)(blocks > 0 ? blocks : 1) * 27 * 256 * sizeof(float);
+}
+
+int cc_cnn_conv3_wgrad(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
+                       const uint16_t* gout, float* dW, float scale, int32_t accumulate, int32_t blocks,
+                       void* workspace, size_t workspace_bytes, cc_stream_t stream) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !in || !gout || !dW || !workspace || blocks <= 0) return CC_E_ARG;
+  if (workspace_bytes < cc_cnn_wgrad_workspace_bytes(nx, ny, nz, blocks)) return CC_E_WORKSPACE;
+  GridArgs g{nx, ny, nz, periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
+  const long long rows = (long long)((nx + 15) / 16) * ny * nz;
+  const int rpb = (int)((rows + blocks - 1) / blocks);
+  hipStream_t s = (hipStream_t)stream;
+  float* part = (float*)workspace;
+  hipLaunchKernelGGL(k_conv3_wgrad, dim3(blocks), dim3(256), 0, s, g, in, gout, part, rpb);
+  hipLaunchKernelGGL(k_reduce_parts, dim3(27), dim3(256), 0, s, part, blocks, 27 * 256, dW, scale, accumulate);
+  return (int)hipGetLastError();
+}
+
+int cc_cnn_adam(int32_t n, float* w, const float* grad, float* m1, float* m2, float lr, float beta1, float beta2,
+                float eps, int32_t step, const uint8_t* frozen, uint16_t* w_bf16, cc_stream_t stream) {
+  if (n <= 0 || !w || !grad || !m1 || !m2 || !w_bf16 || step < 1) return CC_E_ARG;
+  const float c1 = 1.f - powf(beta1, (float)step);
+  const float c2 = 1.f - powf(beta2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, w, grad, m1, m2, lr, beta1,
+                     beta2, eps, c1, c2, frozen, w_bf16);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

@@ -16,6 +16,7 @@ constexpr int kProfSpmm = 0;
 constexpr int kProfUpdate = 1;
 constexpr int kProfDirection = 2;
 constexpr int kProfBatched = 3;
+constexpr int kProfLearned = 4;
 constexpr int kProfClasses = 8;
 
 // Per right-hand-side state of one CG recurrence (device memory). Only the
@@ -47,6 +48,7 @@ void prof_begin_batched(hipStream_t s);
 void prof_end_batched(hipStream_t s);
 void prof_begin(int cls, hipStream_t s);
 void prof_end(hipStream_t s);
+void prof_collect();
 
 template <typename T>
 struct Accum {
@@ -68,6 +70,16 @@ int cg_operator_f64(int n, const std::function<int(const double*, double*, doubl
                     int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
                     const std::function<int(const double*, double*, double*)>& residual);
 size_t cg_operator_bytes(size_t n);
+}  // namespace cc
+
+struct cc_cnn_desc;
+namespace cc {
+// Flexible CG with the learned preconditioner over an operator (fcg.hip).
+int fcg_operator(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                 const cc_cnn_desc& net, const double* diag, const double* bvec, double* x, double tol, int maxiter,
+                 int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
+                 const std::function<int(const double*, double*, double*)>& residual);
+size_t fcg_operator_bytes(size_t n, size_t voxels);
 }  // namespace cc
 
 #define CC_CHECK(expr)             \

@@ -383,4 +383,49 @@ int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* bvec, double* x, doub
                          preconditioner, check_every, cgws, cgb, result, history, s, residual);
 }
 
+int cc_fcg_sumfact(const cc_sumfact_desc* d, const cc_cnn_desc* net, const double* bvec, double* x, double tol,
+                   int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
+                   cc_pcg_result* result, double* history, cc_stream_t stream) {
+  if (!desc_ok(d) || !net || !bvec || !x || !workspace || !result || maxiter < 0 || net->n_red != d->n_red)
+    return CC_E_ARG;
+  if (workspace_bytes < cc_fcg_sumfact_workspace_bytes(d, net)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  MfBuffers mb = carve_mf(d, base);
+  const size_t off = (mf_bytes(d) + 511) & ~size_t(255);
+  double* rpart = reinterpret_cast<double*>(base + off);
+  char* fws = base + off + (((size_t)kMaxParts * 2 * sizeof(double) + 511) & ~size_t(255));
+  const size_t fb = workspace_bytes - (size_t)(fws - base);
+  const cc_sumfact_desc desc = *d;
+  auto apply = [&](const double* P, double* Q, double* partial, int* gs) {
+    return mf_apply(desc, mb, P, Q, partial, gs, nullptr, s);
+  };
+  auto residual = [&](const double* X, double* scratch, double* rel) {
+    CC_CHECK(mf_apply(desc, mb, X, scratch, nullptr, nullptr, nullptr, s));
+    const int g = grid1d(desc.n_red);
+    hipLaunchKernelGGL(k_mf_residual, dim3(g), dim3(256), 0, s, desc.n_red, bvec, scratch, rpart);
+    CC_CHECK(hipGetLastError());
+    std::vector<double> hp(2 * (size_t)g);
+    CC_CHECK(hipMemcpyAsync(hp.data(), rpart, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+    CC_CHECK(hipStreamSynchronize(s));
+    double rr = 0.0, bb = 0.0;
+    for (int i = 0; i < g; ++i) {
+      rr += hp[2 * i];
+      bb += hp[2 * i + 1];
+    }
+    double bn = std::sqrt(bb);
+    if (bn == 0.0) bn = 1.0;
+    *rel = std::sqrt(rr) / bn;
+    return 0;
+  };
+  return fcg_operator(desc.n_red, apply, *net, desc.diag, bvec, x, tol, maxiter, check_every, fws, fb, result,
+                      history, s, residual);
+}
+
+size_t cc_fcg_sumfact_workspace_bytes(const cc_sumfact_desc* d, const cc_cnn_desc* net) {
+  if (!desc_ok(d) || !net) return 0;
+  return mf_bytes(d) + 1024 + (size_t)kMaxParts * 2 * sizeof(double) +
+         fcg_operator_bytes((size_t)d->n_red, (size_t)net->nx * net->ny * net->nz) + 512;
+}
+
 }  // extern "C"

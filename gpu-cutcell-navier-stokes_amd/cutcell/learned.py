@@ -1,0 +1,325 @@
+"""Learned preconditioner: a masked 4-layer, 16-channel 3x3x3 CNN on the
+background dof grid, trained on the GPU and applied inside flexible CG.
+
+Everything numerical runs in the HIP kernels of ``csrc/cnn.hip`` and
+``csrc/fcg.hip`` (bfloat16 implicit-GEMM convolutions on MFMA with fp32
+accumulation, fp32 weight gradients, Adam); PyTorch only owns the device
+buffers. The network is
+
+    w = D^-1 r  (Jacobi, as channel 0 of the grid at the unknowns' voxels)
+    h1 = mask * relu(conv(w)), h2 = mask * relu(conv(h1)),
+    h3 = mask * relu(conv(h2)), y = mask * conv(h3)
+    z = w + y[channel 0]
+
+with no biases, so ``z(a r) = a z(r)`` for ``a > 0``. It is trained on
+(residual, error) pairs taken from the solver's own Jacobi-PCG iterates,
+minimising ``||z(r) - e||^2 / ||e||^2`` per pair (pairs scaled to unit
+``||D^-1 r||``). Inside flexible CG the float64 true residual decides
+success; a learned solve that does not converge falls back to Jacobi-PCG.
+
+The reference has no learned preconditioner; its Jacobi rule is
+``code/fast_solver_backends.py:74-87`` (CPU) and ``:237-238`` (GPU).
+"""
+
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import hip
+
+NLAYERS = 4
+WSIZE = 27 * 16 * 16
+
+
+class CnnDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int32),
+        ("ny", ctypes.c_int32),
+        ("nz", ctypes.c_int32),
+        ("periodic", ctypes.c_int32 * 3),
+        ("weights", ctypes.c_void_p * 4),
+        ("mask", ctypes.c_void_p),
+        ("red2full", ctypes.c_void_p),
+        ("n_red", ctypes.c_int32),
+    ]
+
+
+def _bind(lib):
+    P, I, F, D, SZ = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
+    sigs = {
+        "cc_mfma_probe": [P, P, P, P],
+        "cc_cnn_conv3_forward": [I, I, I, P, P, P, P, P, I, P, P, P],
+        "cc_cnn_flip_weights": [P, P, P],
+        "cc_cnn_conv3_wgrad": [I, I, I, P, P, P, P, F, I, I, P, SZ, P],
+        "cc_cnn_adam": [I, P, P, P, P, F, F, F, F, I, P, P, P],
+        "cc_cnn_train_input": [I, P, P, P, F, P, P],
+        "cc_cnn_train_loss": [I, P, P, P, P, F, P, F, F, P, P, P, P],
+        "cc_cnn_precondition": [P, P, P, P, P, SZ, P],
+        "cc_fcg_sumfact": [P, P, P, P, D, I, I, P, SZ, P, P, P],
+    }
+    for name, args in sigs.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    for name, args in {
+        "cc_cnn_wgrad_workspace_bytes": [I, I, I, I],
+        "cc_cnn_precondition_workspace_bytes": [P],
+        "cc_fcg_sumfact_workspace_bytes": [P, P],
+    }.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = SZ
+    return lib
+
+
+def _lib():
+    return _bind(hip.require_gpu())
+
+
+def to_bf16_bits(a: np.ndarray) -> np.ndarray:
+    """Round float values to bfloat16 (nearest even); returns uint16 bits."""
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    return u.astype(np.uint16)
+
+
+def from_bf16_bits(h: np.ndarray) -> np.ndarray:
+    return (np.asarray(h, dtype=np.uint32) << 16).view(np.float32)
+
+
+@dataclass
+class TrainReport:
+    steps: int
+    batch: int
+    loss_first: float
+    loss_last: float
+    seconds: float
+    history: list
+
+
+class MaskedCNN:
+    """Weights, optimiser state and device buffers of the preconditioner net."""
+
+    def __init__(self, grid_shape, red2full: np.ndarray, periodic=(False, False, False), seed: int = 0,
+                 device: str = "cuda", init_scale: float = 1.0):
+        import torch
+
+        self.torch = torch
+        self.lib = _lib()
+        self.nx, self.ny, self.nz = (int(v) for v in grid_shape)
+        self.voxels = self.nx * self.ny * self.nz
+        self.periodic = (ctypes.c_int32 * 3)(*[int(bool(p)) for p in periodic])
+        self.device = device
+        self.n = int(red2full.size)
+        rng = np.random.default_rng(seed)
+        ws = []
+        for layer in range(NLAYERS):
+            fan_in = 27 * (1 if layer == 0 else 16)
+            w = rng.standard_normal((27, 16, 16)) * np.sqrt(2.0 / fan_in) * init_scale
+            if layer == 0:
+                w[:, 1:, :] = 0.0  # only channel 0 carries the input
+            if layer == NLAYERS - 1:
+                w *= 0.05  # start close to plain Jacobi
+                w[:, :, 1:] = 0.0
+            ws.append(w.astype(np.float32).ravel())
+        t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
+        self.w32 = t(np.stack(ws), np.float32)  # (4, WSIZE)
+        self.wbf = t(np.stack([to_bf16_bits(w) for w in ws]), np.uint16)
+        self.m1 = torch.zeros_like(self.w32)
+        self.m2 = torch.zeros_like(self.w32)
+        self.grad = torch.zeros_like(self.w32)
+        self.step_count = 0
+        mask = np.zeros(self.voxels, dtype=np.uint8)
+        mask[red2full] = 1
+        self.mask = t(mask, np.uint8)
+        self.red2full = t(red2full, np.int32)
+        # Activation buffers (bf16 [voxel][16]) and outputs.
+        z16 = lambda: torch.zeros(self.voxels * 16, dtype=torch.uint16, device=device)
+        self.acts = [z16() for _ in range(NLAYERS)]
+        self.y = torch.zeros(self.voxels * 16, dtype=torch.float32, device=device)
+        self.g = [z16() for _ in range(2)]
+        self.gout = z16()
+        self.flip = torch.zeros(WSIZE, dtype=torch.uint16, device=device)
+        self.wgrad_blocks = 512
+        nb = int(self.lib.cc_cnn_wgrad_workspace_bytes(self.nx, self.ny, self.nz, self.wgrad_blocks))
+        self.wgrad_ws = torch.empty(nb, dtype=torch.uint8, device=device)
+        self.part = torch.zeros(2 * 1024, dtype=torch.float64, device=device)
+        self._desc = None
+
+    # -- descriptor for the fused preconditioner / FCG ------------------------
+    def desc(self) -> CnnDesc:
+        d = CnnDesc()
+        d.nx, d.ny, d.nz = self.nx, self.ny, self.nz
+        for k in range(3):
+            d.periodic[k] = self.periodic[k]
+        for l in range(NLAYERS):
+            d.weights[l] = self.wbf[l].data_ptr()
+        d.mask = self.mask.data_ptr()
+        d.red2full = self.red2full.data_ptr()
+        d.n_red = self.n
+        self._desc = d
+        return d
+
+    # -- layers ---------------------------------------------------------------
+    def _conv(self, x, layer_w, relu: bool, out=None, out_f32=None, gate=None):
+        p = hip._ptr
+        hip._check(self.lib.cc_cnn_conv3_forward(self.nx, self.ny, self.nz, self.periodic, p(x), p(layer_w),
+                                                 p(self.mask), p(gate) if gate is not None else None, int(relu),
+                                                 p(out) if out is not None else None,
+                                                 p(out_f32) if out_f32 is not None else None, hip._stream()),
+                   "conv3_forward")
+
+    def forward(self):
+        """acts[0] holds the input; fills acts[1..3] and y (fp32)."""
+        for l in range(NLAYERS - 1):
+            self._conv(self.acts[l], self.wbf[l], True, out=self.acts[l + 1])
+        self._conv(self.acts[NLAYERS - 1], self.wbf[NLAYERS - 1], False, out_f32=self.y)
+
+    def backward(self, scale: float, accumulate: bool):
+        """gout holds dL/dy (channel 0); accumulates scale * dL/dW into grad."""
+        p = hip._ptr
+        g = self.gout
+        for l in range(NLAYERS - 1, -1, -1):
+            hip._check(self.lib.cc_cnn_conv3_wgrad(self.nx, self.ny, self.nz, self.periodic, p(self.acts[l]), p(g),
+                                                   p(self.grad[l]), float(scale), int(accumulate),
+                                                   self.wgrad_blocks, p(self.wgrad_ws), self.wgrad_ws.numel(),
+                                                   hip._stream()), "conv3_wgrad")
+            if l == 0:
+                break
+            hip._check(self.lib.cc_cnn_flip_weights(p(self.wbf[l]), p(self.flip), hip._stream()), "flip")
+            gnext = self.g[l % 2]
+            self._conv(g, self.flip, False, out=gnext, gate=self.acts[l])
+            g = gnext
+
+    def adam_step(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
+        self.step_count += 1
+        p = hip._ptr
+        hip._check(self.lib.cc_cnn_adam(NLAYERS * WSIZE, p(self.w32), p(self.grad), p(self.m1), p(self.m2), lr, beta1,
+                                        beta2, eps, self.step_count, None, p(self.wbf), hip._stream()), "adam")
+
+    # -- training -------------------------------------------------------------
+    def sample_loss_grad(self, r32, e32, diag, scale: float, gscale: float, accumulate: bool,
+                         e_norm2: float) -> float:
+        """Forward + backward for one (r, e) pair (``e_norm2 = ||e||^2``);
+        returns its loss (synchronises)."""
+        p = hip._ptr
+        hip._check(self.lib.cc_cnn_train_input(self.n, p(self.red2full), p(r32), p(diag), float(scale), p(self.acts[0]),
+                                               hip._stream()), "train_input")
+        self.forward()
+        t2 = scale * scale * e_norm2
+        nparts = ctypes.c_int32()
+        hip._check(self.lib.cc_cnn_train_loss(self.n, p(self.red2full), p(r32), p(e32), p(diag), float(scale), p(self.y),
+                                              float(1.0 / t2), float(gscale), p(self.gout), p(self.part),
+                                              ctypes.addressof(nparts), hip._stream()), "train_loss")
+        self.backward(1.0, accumulate)
+        parts = self.part[: 2 * nparts.value].view(-1, 2).sum(dim=0)
+        return float((parts[0] / parts[1]).item())
+
+    def train(self, R, Eerr, diag, *, steps: int = 200, batch: int = 8, lr: float = 2e-3, seed: int = 0,
+              log_every: int = 0) -> TrainReport:
+        """Adam on mini-batches of (residual, error) pairs (float32, [S, n])."""
+        torch = self.torch
+        rng = np.random.default_rng(seed)
+        S = R.shape[0]
+        scales, enorm2 = self._sample_scales(R, Eerr, diag)
+        hist = []
+        t0 = time.perf_counter()
+        for step in range(steps):
+            idx = rng.choice(S, size=min(batch, S), replace=False)
+            losses = []
+            for k, i in enumerate(idx):
+                losses.append(self.sample_loss_grad(R[i], Eerr[i], diag, float(scales[i]), 1.0 / len(idx), k > 0,
+                                                    float(enorm2[i])))
+            self.adam_step(lr)
+            hist.append(float(np.mean(losses)))
+            if log_every and step % log_every == 0:
+                print(f"[cnn] step {step} loss {hist[-1]:.4f}", flush=True)
+        torch.cuda.synchronize()
+        return TrainReport(steps=steps, batch=batch, loss_first=hist[0], loss_last=hist[-1],
+                           seconds=time.perf_counter() - t0, history=hist)
+
+    @staticmethod
+    def _sample_scales(R, Eerr, diag):
+        """Per pair: 1/||D^-1 r|| and ||e||^2 (bookkeeping of the data set)."""
+        dinv_norm = (R.double() / diag.double()).norm(dim=1).cpu().numpy()
+        enorm2 = Eerr.double().pow(2).sum(dim=1).cpu().numpy()
+        return 1.0 / np.maximum(dinv_norm, 1e-300), enorm2
+
+    def evaluate(self, R, Eerr, diag) -> float:
+        """Mean loss over the given pairs (no weight update)."""
+        scales, enorm2 = self._sample_scales(R, Eerr, diag)
+        keep = self.grad.clone()
+        out = []
+        for i in range(R.shape[0]):
+            out.append(self.sample_loss_grad(R[i], Eerr[i], diag, float(scales[i]), 0.0, True, float(enorm2[i])))
+        self.grad.copy_(keep)
+        return float(np.mean(out))
+
+    # -- application ----------------------------------------------------------
+    def precondition(self, r, diag):
+        torch = self.torch
+        d = self.desc()
+        nb = int(self.lib.cc_cnn_precondition_workspace_bytes(ctypes.addressof(d)))
+        ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        z = torch.empty_like(r)
+        hip._check(self.lib.cc_cnn_precondition(ctypes.addressof(d), hip._ptr(r), hip._ptr(diag), hip._ptr(z),
+                                                hip._ptr(ws), nb, hip._stream()), "cnn_precondition")
+        return z
+
+    def fcg(self, dmf, b, *, tol=1.0e-8, maxiter=5000, check_every=16, history=False, x=None):
+        """Flexible CG with this network on a ``DeviceMatrixFree`` operator."""
+        torch = self.torch
+        d = self.desc()
+        nb = int(self.lib.cc_fcg_sumfact_workspace_bytes(ctypes.addressof(dmf.desc), ctypes.addressof(d)))
+        if nb == 0:
+            raise ValueError("invalid descriptors")
+        ws = getattr(self, "_fcg_ws", None)
+        if ws is None or ws.numel() < nb:
+            ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            self._fcg_ws = ws
+        x = torch.empty_like(b) if x is None else x
+        res = hip.PcgResult()
+        hist = torch.full((int(maxiter) + 1,), float("nan"), dtype=torch.float64, device=b.device) if history else None
+        hip._check(self.lib.cc_fcg_sumfact(ctypes.addressof(dmf.desc), ctypes.addressof(d), hip._ptr(b), hip._ptr(x),
+                                           float(tol), int(maxiter), int(check_every), hip._ptr(ws), nb,
+                                           ctypes.addressof(res), hip._ptr(hist) if hist is not None else None,
+                                           hip._stream()), "fcg_sumfact")
+        if history:
+            return x, res, hist.cpu().numpy()
+        return x, res
+
+
+def generate_pairs(dmf, rhs, snapshots, tol_ref: float = 1.0e-12):
+    """(residual, error) pairs from the solver's own Jacobi-PCG iterates.
+
+    For every right-hand side b (rows of ``rhs``): x* from a tight solve, then
+    for each k in ``snapshots`` the k-th CG iterate x_k (a solve capped at k
+    iterations from x0 = 0 reproduces it), r_k = b - A x_k (float64 apply)
+    and e_k = x* - x_k. Returns float32 tensors ``R, E`` of shape [S, n].
+    """
+    import torch
+
+    Rs, Es = [], []
+    for b in rhs:
+        xs, res = dmf.pcg(b, tol=tol_ref, maxiter=40000)
+        for k in snapshots:
+            xk, rk = dmf.pcg(b, tol=0.0, maxiter=int(k))
+            r = b - dmf.apply(xk)
+            Rs.append(r.float())
+            Es.append((xs - xk).float())
+    return torch.stack(Rs), torch.stack(Es)
+
+
+def solve_learned_with_fallback(net: MaskedCNN, dmf, b, *, tol=1.0e-8, maxiter=5000):
+    """Flexible CG with the network; Jacobi-PCG when it does not converge."""
+    x, res = net.fcg(dmf, b, tol=tol, maxiter=maxiter)
+    ok = res.status == hip.CC_CONVERGED and res.true_rel <= max(10.0 * tol, 1e-14)
+    if ok:
+        return x, res, "learned-fcg"
+    x, res = dmf.pcg(b, tol=tol, maxiter=maxiter)
+    return x, res, "jacobi-pcg-fallback"

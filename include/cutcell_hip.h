@@ -224,6 +224,64 @@ int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* b, double* x, double 
                    int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
                    cc_pcg_result* result, double* history, cc_stream_t stream);
 
+/* ---- Learned preconditioner (no reference counterpart: the reference has
+ * only Jacobi, code/fast_solver_backends.py:74-87 and :237-238). ---------
+ * Activations are channel-last bfloat16 grids [voxel][16] on the background
+ * dof grid (nx, ny, nz; x fastest); weights are bfloat16 [27][16][16]
+ * (tap t = (dz+1)*9 + (dy+1)*3 + (dx+1), then input channel, then output
+ * channel); mask[voxel] != 0 marks the voxels that carry an unknown. */
+typedef struct cc_cnn_desc {
+  int32_t nx, ny, nz;
+  int32_t periodic[3];
+  const uint16_t* weights[4];
+  const uint8_t* mask;
+  const int32_t* red2full; /* voxel of each unknown */
+  int32_t n_red;
+} cc_cnn_desc;
+
+/* D = A B for 16x16 bfloat16 A, B (row-major), fp32 D: the MFMA lane-layout
+ * probe the convolution kernels rely on. */
+int cc_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, cc_stream_t stream);
+/* out = mask * act(conv3(in, weights)) [* (gate > 0)]; out (bfloat16) and/or
+ * out_f32 (fp32) may be given. relu: 1 = ReLU. gate may be NULL. */
+int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
+                         const uint16_t* weights, const uint8_t* mask, const uint16_t* gate, int32_t relu,
+                         uint16_t* out, float* out_f32, cc_stream_t stream);
+/* Weights of the data-gradient layer: taps reversed, channels transposed. */
+int cc_cnn_flip_weights(const uint16_t* weights, uint16_t* flipped, cc_stream_t stream);
+/* dW (fp32 [27][16][16]) = scale * sum_v in[v + off_t][ci] gout[v][co]
+ * (+ dW when accumulate != 0), reduced over `blocks` partials in a fixed
+ * order. */
+size_t cc_cnn_wgrad_workspace_bytes(int32_t nx, int32_t ny, int32_t nz, int32_t blocks);
+int cc_cnn_conv3_wgrad(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
+                       const uint16_t* gout, float* dW, float scale, int32_t accumulate, int32_t blocks,
+                       void* workspace, size_t workspace_bytes, cc_stream_t stream);
+/* One Adam step on fp32 master weights w (n values); also refreshes the
+ * bfloat16 copy. frozen (may be NULL) marks entries kept fixed. */
+int cc_cnn_adam(int32_t n, float* w, const float* grad, float* m1, float* m2, float lr, float beta1, float beta2,
+                float eps, int32_t step, const uint8_t* frozen, uint16_t* w_bf16, cc_stream_t stream);
+/* z = w + CNN(w), w = D^-1 r (zero diagonal -> identity), float64 in/out. */
+size_t cc_cnn_precondition_workspace_bytes(const cc_cnn_desc* net);
+int cc_cnn_precondition(const cc_cnn_desc* net, const double* r, const double* diag, double* z, void* workspace,
+                        size_t workspace_bytes, cc_stream_t stream);
+/* Training helpers: the network input scale * D^-1 r as channel 0 of a bf16
+ * grid (other entries untouched), and the loss ||z - scale e||^2 /
+ * ||scale e||^2 with z = scale D^-1 r + y (y: fp32 channel-0 output), its
+ * gradient (times gscale) written to channel 0 of gout, and per-block
+ * partials (||z - t||^2, ||t||^2) in partial[2 * nparts]. r, e: float32. */
+int cc_cnn_train_input(int32_t n, const int32_t* red2full, const float* r, const double* diag, float scale,
+                       uint16_t* grid, cc_stream_t stream);
+int cc_cnn_train_loss(int32_t n, const int32_t* red2full, const float* r, const float* e, const double* diag,
+                      float scale, const float* y, float inv_den, float gscale, uint16_t* gout, double* partial,
+                      int32_t* nparts, cc_stream_t stream);
+/* Flexible CG (one stored direction) with the learned preconditioner on the
+ * matrix-free operator; stopping test, iteration count, breakdown test and
+ * float64 true residual as in cc_pcg_sumfact. Synchronises. */
+size_t cc_fcg_sumfact_workspace_bytes(const cc_sumfact_desc* d, const cc_cnn_desc* net);
+int cc_fcg_sumfact(const cc_sumfact_desc* d, const cc_cnn_desc* net, const double* b, double* x, double tol,
+                   int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
+                   cc_pcg_result* result, double* history, cc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

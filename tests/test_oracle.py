@@ -213,3 +213,33 @@ def test_parity_criterion_rejects_a_wrong_recurrence():
         rho = rho_new
     with pytest.raises(AssertionError):
         check_trajectory(h, it, np.array(hb), itb, 1e-8)
+
+
+def test_cnn_oracle_gradients_match_finite_differences():
+    """The float64 CNN oracle's hand-written backward agrees with central
+    differences (so it can pin the HIP training kernels)."""
+    from oracle import cnn_ref
+
+    rng = np.random.default_rng(0)
+    nz, ny, nx = 5, 6, 7
+    mask = (rng.random((nz, ny, nx)) > 0.2).astype(float)
+    r = rng.standard_normal((nz, ny, nx))
+    e = rng.standard_normal((nz, ny, nx))
+    Ws = [0.2 * rng.standard_normal((27, 16, 16)) for _ in range(4)]
+    Ws[0][:, 1:, :] = 0.0
+    for per in ((False, False, False), (True, False, True)):
+        assert cnn_ref.fd_check(r, e, Ws, mask, n_probe=8, periodic=per) < 1e-6
+
+
+def test_cnn_oracle_conv_is_a_correlation():
+    """A single-tap identity kernel shifts the field: out[v] = in[v + off]."""
+    from oracle import cnn_ref
+
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((4, 5, 6, 16))
+    W = np.zeros((27, 16, 16))
+    t = (1 + 1) * 9 + (0 + 1) * 3 + (-1 + 1)  # dz=+1, dy=0, dx=-1
+    W[t] = np.eye(16)
+    out, _ = cnn_ref.conv3(x, W, np.ones((4, 5, 6)))
+    np.testing.assert_array_equal(out[:-1, :, 1:], x[1:, :, :-1])
+    assert np.all(out[-1] == 0) and np.all(out[:, :, 0] == 0)
