@@ -94,7 +94,8 @@ void prof_collect() {
 }
 
 // ---------------------------------------------------------------------------
-// SpMM launch configuration (per k; set by the autotuner or the defaults).
+// SpMM launch configuration (per vector width and k; set by the autotuner or
+// the defaults). Index [0] holds float64 vectors, [1] float32 ones.
 // ---------------------------------------------------------------------------
 
 struct SpmmConfig {
@@ -103,7 +104,12 @@ struct SpmmConfig {
   int unroll;  // 1 or 4 independent loads in flight per lane; 0 = 4
   int cpl;     // right-hand sides per lane (vector width); 0 = widest 16-byte load
 };
-static SpmmConfig g_spmm_cfg[17] = {};
+static SpmmConfig g_spmm_cfg[2][17] = {};
+
+template <typename VT>
+static inline const SpmmConfig& spmm_cfg(int k) {
+  return g_spmm_cfg[sizeof(VT) == 4 ? 1 : 0][k];
+}
 
 // ---------------------------------------------------------------------------
 // Reductions.
@@ -617,8 +623,9 @@ template <typename MT, typename VT, int K, int MODE, int CPL>
 static int spmm_launch_cpl(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
                            VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
   constexpr int KL = K / CPL;
-  const SpmmConfig cfg = g_spmm_cfg[K];
-  const int nz = cfg.nz > 0 ? cfg.nz : default_nz<KL>(avg_row);
+  const SpmmConfig cfg = spmm_cfg<VT>(K);
+  // A shape tuned for another vector width may not fit this one's teams.
+  const int nz = (cfg.nz > 0 && KL * cfg.nz <= 64) ? cfg.nz : default_nz<KL>(avg_row);
   const int unroll = cfg.unroll > 0 ? cfg.unroll : 4;
   const int grid = cfg.grid;
 #define CC_NZ(NZV)                                                                                                  \
@@ -639,7 +646,7 @@ static int spmm_launch_cpl(int n, double avg_row, const int* indptr, const int* 
 template <typename MT, typename VT, int K, int MODE>
 static int spmm_launch(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
                        VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
-  int cpl = g_spmm_cfg[K].cpl > 0 ? g_spmm_cfg[K].cpl : default_cpl<VT, K>();
+  int cpl = spmm_cfg<VT>(K).cpl > 0 ? spmm_cfg<VT>(K).cpl : default_cpl<VT, K>();
   if (cpl > K || cpl * (int)sizeof(VT) > 16) cpl = default_cpl<VT, K>();
   if (cpl >= 4 && sizeof(VT) == 4 && K >= 4)
     return spmm_launch_cpl<MT, VT, K, MODE, (sizeof(VT) == 4 && K >= 4 ? 4 : 1)>(n, avg_row, indptr, indices, vals, X,
@@ -977,14 +984,17 @@ int cc_profile_collect(void) {
   return 0;
 }
 
-int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll, int32_t cols_per_lane) {
+int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
+                       int32_t cols_per_lane) {
   if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
+  if (value_bytes != 4 && value_bytes != 8) return CC_E_ARG;
+  if (cols_per_lane * value_bytes > 16) return CC_E_ARG;
   if (cols_per_lane != 0 && cols_per_lane != 1 && cols_per_lane != 2 && cols_per_lane != 4) return CC_E_ARG;
   if (cols_per_lane > k) return CC_E_ARG;
   const int kl = k / (cols_per_lane > 0 ? cols_per_lane : 1);
   if (nz_lanes < 0 || kl * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
   if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4)) return CC_E_ARG;
-  g_spmm_cfg[k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane};
+  g_spmm_cfg[value_bytes == 4 ? 1 : 0][k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane};
   return 0;
 }
 

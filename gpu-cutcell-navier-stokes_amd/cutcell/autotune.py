@@ -107,11 +107,12 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
 
     lib = hip.require_gpu()
     mean_row = A.nnz / max(A.n, 1)
+    vb = 8 if precision == "fp64" else 4
     key = f"spmm|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
     table = _load()
     if key in table and not force:
         cfg = table[key]
-        lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
+        lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
         return cfg
     dt = torch.float64 if precision == "fp64" else torch.float32
     X = torch.randn(A.n, k, dtype=dt, device=A.vals.device)
@@ -120,8 +121,8 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     fn_name = "cc_spmm_csr_f64" if precision == "fp64" else "cc_spmm_csr_f32"
     fn = getattr(lib, fn_name)
     best = None
-    for cand in spmm_candidates(k, mean_row, 8 if precision == "fp64" else 4):
-        if lib.cc_spmm_set_config(k, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) != 0:
+    for cand in spmm_candidates(k, mean_row, vb):
+        if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) != 0:
             continue
         call = lambda: fn(A.n, k, hip._ptr(A.indptr), hip._ptr(A.indices), hip._ptr(vals), hip._ptr(X), hip._ptr(Y),
                           hip._stream())
@@ -129,7 +130,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
         if best is None or ms < best[0]:
             best = (ms, cand)
     cfg = dict(best[1], ms=best[0])
-    lib.cc_spmm_set_config(k, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
+    lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
     _store(key, cfg)
     return cfg
 
@@ -137,7 +138,8 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
 def reset_spmm_config() -> None:
     lib = hip.load_library()
     for k in hip.SUPPORTED_K:
-        lib.cc_spmm_set_config(k, 0, 0, 0, 0)
+        for vb in (8, 4):
+            lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0)
 
 
 # ---------------------------------------------------------------------------

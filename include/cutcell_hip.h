@@ -77,13 +77,15 @@ int cc_profile_read(int32_t which, double* total_ms, int64_t* count);
  * call after synchronising their stream. */
 int cc_profile_collect(void);
 
-/* SpMM/SpMV launch shape for k right-hand sides, chosen by the autotuner
- * (no reference counterpart): cols_per_lane right-hand sides per lane, loaded
- * as one vector (1, 2 or 4 with at most 16 bytes; 0 = widest), nz_lanes
- * nonzero slots per row team ((k/cols_per_lane)*nz_lanes <= 64, power of
- * two; 0 = from the mean row length), grid blocks (0 = one block per few rows
- * up to 1024), unroll 1 or 4 (0 = 4). */
-int cc_spmm_set_config(int32_t k, int32_t nz_lanes, int32_t grid, int32_t unroll, int32_t cols_per_lane);
+/* SpMM/SpMV launch shape for k right-hand sides of value_bytes-wide vectors
+ * (8 = float64, 4 = float32; each width keeps its own shape), chosen by the
+ * autotuner (no reference counterpart): cols_per_lane right-hand sides per
+ * lane, loaded as one vector (1, 2 or 4 with at most 16 bytes; 0 = widest),
+ * nz_lanes nonzero slots per row team ((k/cols_per_lane)*nz_lanes <= 64,
+ * power of two; 0 = from the mean row length), grid blocks (0 = one block per
+ * few rows up to 1024), unroll 1 or 4 (0 = 4). */
+int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
+                       int32_t cols_per_lane);
 
 /* y = A x, float64 (replaces the cuSPARSE product A_gpu @ p). */
 int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,

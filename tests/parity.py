@@ -7,14 +7,18 @@ residual envelopes within a factor ~1.7 (measured on the sliver systems,
 tests/test_oracle.py::test_summation_order_moves_the_stop_by_rounding_only).
 The parity check therefore compares the whole residual trajectory and asks
 that any difference in the stopping iteration be explained by the residual
-hovering at the threshold.
+hovering at the threshold: at the iteration where one run stopped, the other
+run's residual must lie within the same envelope factor of the tolerance
+(CG residual norms are not monotone, so a run one iteration short of its stop
+can sit well above the threshold; 1.8x was seen for batched solves of the
+2D oblique systems).
 """
 
 import numpy as np
 
 MAX_ITER_GAP = 3
-STOP_SLACK = 1.3
 ENVELOPE_FACTOR = 2.0
+STOP_SLACK = ENVELOPE_FACTOR
 
 
 def check_trajectory(h_a, it_a, h_b, it_b, tol):

@@ -47,6 +47,26 @@ def test_bad_arguments_are_rejected_without_a_gpu():
     assert lib.cc_pcg_jacobi_f64(0, 1, None, None, None, None, None, 1e-8, 10, 1, 16, None, 0, None, None, None) == -1
 
 
+def test_spmm_config_is_validated_per_vector_width():
+    """Launch shapes are kept per value width: four float64 right-hand sides
+    do not fit one 16-byte load, four float32 ones do."""
+    from cutcell import hip
+
+    lib = hip.load_library()
+    try:
+        assert lib.cc_spmm_set_config(16, 8, 16, 0, 0, 4) == -1
+        assert lib.cc_spmm_set_config(16, 4, 16, 0, 0, 4) == 0
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2) == 0
+        assert lib.cc_spmm_set_config(16, 8, 16, 512, 4, 2) == -1  # 8 x 16 lanes > 64
+        assert lib.cc_spmm_set_config(3, 8, 0, 0, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 2, 0, 0, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 8, 0, 0, 3, 0) == -1
+    finally:
+        for k in hip.SUPPORTED_K:
+            for vb in (8, 4):
+                assert lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0) == 0
+
+
 def test_gpu_backend_without_gpu_reports_unavailable():
     torch = pytest.importorskip("torch")
     if torch.cuda.is_available():

@@ -35,7 +35,8 @@ def test_mfma_lane_layout(lib):
     D = torch.zeros(256, dtype=torch.float32, device="cuda")
     from cutcell import hip
 
-    assert lib.cc_mfma_probe(_dev(A).data_ptr(), _dev(B).data_ptr(), D.data_ptr(), hip._stream()) == 0
+    Ad, Bd = _dev(A), _dev(B)
+    assert lib.cc_mfma_probe(Ad.data_ptr(), Bd.data_ptr(), D.data_ptr(), hip._stream()) == 0
     ref = from_bf16_bits(A).astype(np.float64) @ from_bf16_bits(B).astype(np.float64)
     np.testing.assert_allclose(D.cpu().numpy().reshape(16, 16), ref, rtol=1e-5, atol=1e-5)
 
@@ -54,8 +55,11 @@ def test_conv_forward_matches_oracle(lib, periodic, relu):
     ref, _ = cnn_ref.conv3(x, W, mask.astype(float), relu=relu, periodic=periodic)
     out = torch.zeros(nz * ny * nx * 16, dtype=torch.float32, device="cuda")
     per = (ctypes.c_int32 * 3)(*[int(v) for v in periodic])
-    st = lib.cc_cnn_conv3_forward(nx, ny, nz, per, _dev(to_bf16_bits(x)).data_ptr(), _dev(to_bf16_bits(W)).data_ptr(),
-                                  _dev(mask).data_ptr(), None, int(relu), None, out.data_ptr(), hip._stream())
+    # Device copies stay referenced until the kernel has run.
+    xd, wd, md = _dev(to_bf16_bits(x)), _dev(to_bf16_bits(W)), _dev(mask)
+    st = lib.cc_cnn_conv3_forward(nx, ny, nz, per, xd.data_ptr(), wd.data_ptr(), md.data_ptr(), None, int(relu), None,
+                                  out.data_ptr(), hip._stream())
+    torch.cuda.synchronize()
     assert st == 0
     got = out.cpu().numpy().reshape(nz, ny, nx, 16)
     assert np.abs(got - ref).max() <= 1e-5 * np.abs(ref).max()

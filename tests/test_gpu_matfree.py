@@ -40,7 +40,9 @@ def test_apply_matches_explicit_operator(case):
     x = np.random.default_rng(0).standard_normal(mf.n)
     y = dmf.apply(torch.from_numpy(x).to("cuda")).cpu().numpy()
     ref = A @ x
-    assert np.abs(y - ref).max() <= 1e-12 * np.abs(ref).max(), name
+    # Rounding bound: the reduced entries are sums with cancellation, so the
+    # scale is |A| |x| rather than |A x|.
+    assert np.abs(y - ref).max() <= 1e-11 * (abs(A) @ np.abs(x)).max(), name
 
 
 def test_pcg_matches_oracle(case):
