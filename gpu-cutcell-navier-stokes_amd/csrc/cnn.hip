@@ -252,7 +252,34 @@ __global__ void k_mfma_probe(const uint16_t* __restrict__ A, const uint16_t* __r
   for (int i = 0; i < 4; ++i) D[(4 * (lane / 16) + i) * 16 + lane % 16] = d[i];
 }
 
-// The preconditioner network: three masked ReLU layers and a linear one.
+// Tied layer weights from the two fp32 masters: (W0, W1, flip W1, flip W0),
+// rounded to bfloat16.
+__global__ void k_sym_weights(const float* __restrict__ w, uint16_t* __restrict__ layers) {
+  for (int e = threadIdx.x + blockIdx.x * blockDim.x; e < 27 * 256; e += blockDim.x * gridDim.x) {
+    const int t = e / 256, a = (e / 16) % 16, b = e % 16;
+    const int ef = ((26 - t) * 16 + b) * 16 + a;  // flip(W)[t][a][b] = W[26 - t][b][a]
+    layers[e] = f2bf(w[e]);
+    layers[256 * 27 + e] = f2bf(w[256 * 27 + e]);
+    layers[2 * 256 * 27 + e] = f2bf(w[256 * 27 + ef]);
+    layers[3 * 256 * 27 + e] = f2bf(w[ef]);
+  }
+}
+
+// Gradients of the two masters from the four layers' gradients:
+// gW0 = g0 + flip g3, gW1 = g1 + flip g2 (flip is an involution).
+__global__ void k_sym_grads(const float* __restrict__ gl, float* __restrict__ gw, int accumulate) {
+  for (int e = threadIdx.x + blockIdx.x * blockDim.x; e < 27 * 256; e += blockDim.x * gridDim.x) {
+    const int t = e / 256, a = (e / 16) % 16, b = e % 16;
+    const int ef = ((26 - t) * 16 + b) * 16 + a;
+    const float g0 = gl[e] + gl[3 * 256 * 27 + ef];
+    const float g1 = gl[256 * 27 + e] + gl[2 * 256 * 27 + ef];
+    gw[e] = accumulate ? gw[e] + g0 : g0;
+    gw[256 * 27 + e] = accumulate ? gw[256 * 27 + e] + g1 : g1;
+  }
+}
+
+// The preconditioner network: four masked linear layers (weights W0, W1,
+// flip W1, flip W0 in the descriptor), the last one written in fp32.
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
                        float* out_f32, hipStream_t s) {
   GridArgs g{net.nx, net.ny, net.nz, net.periodic[0], net.periodic[1], net.periodic[2]};
@@ -260,7 +287,7 @@ int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, u
   const unsigned blocks = (unsigned)((waves + 3) / 4);
   uint16_t* acts[4] = {act0, act1, act2, act3};
   for (int l = 0; l < 3; ++l)
-    hipLaunchKernelGGL(k_conv3<1>, dim3(blocks), dim3(256), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
+    hipLaunchKernelGGL(k_conv3<0>, dim3(blocks), dim3(256), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
                        acts[l + 1], nullptr);
   hipLaunchKernelGGL(k_conv3<0>, dim3(blocks), dim3(256), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
                      out_f32);
@@ -320,6 +347,18 @@ int cc_cnn_conv3_wgrad(int32_t nx, int32_t ny, int32_t nz, const int32_t* period
   float* part = (float*)workspace;
   hipLaunchKernelGGL(k_conv3_wgrad, dim3(blocks), dim3(256), 0, s, g, in, gout, part, rpb);
   hipLaunchKernelGGL(k_reduce_parts, dim3(27), dim3(256), 0, s, part, blocks, 27 * 256, dW, scale, accumulate);
+  return (int)hipGetLastError();
+}
+
+int cc_cnn_sym_weights(const float* w_master, uint16_t* layers, cc_stream_t stream) {
+  if (!w_master || !layers) return CC_E_ARG;
+  hipLaunchKernelGGL(k_sym_weights, dim3(27), dim3(256), 0, (hipStream_t)stream, w_master, layers);
+  return (int)hipGetLastError();
+}
+
+int cc_cnn_sym_grads(const float* g_layers, float* g_master, int32_t accumulate, cc_stream_t stream) {
+  if (!g_layers || !g_master) return CC_E_ARG;
+  hipLaunchKernelGGL(k_sym_grads, dim3(27), dim3(256), 0, (hipStream_t)stream, g_layers, g_master, accumulate);
   return (int)hipGetLastError();
 }
 

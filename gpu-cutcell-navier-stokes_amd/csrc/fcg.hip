@@ -1,10 +1,12 @@
 // fcg.hip - flexible CG with the learned (masked CNN) preconditioner.
 //
-// Preconditioner: w = D^-1 r (Jacobi), scattered as channel 0 of a bfloat16
-// grid tensor on the background dof grid; four masked 3x3x3 conv layers
-// (cnn.hip; ReLU on the three hidden layers, no biases, so the map is
-// positively homogeneous); z = w + y where y is the network's channel-0
-// output at the unknowns' voxels. The map r -> z is nonlinear, so the outer
+// Preconditioner: z = D^-1 r + S C^T C S r with S = D^-1/2 and C two masked
+// linear 3x3x3 conv layers (cnn.hip): S r is scattered as channel 0 of a
+// bfloat16 grid tensor on the background dof grid, the four layers W0, W1,
+// flip W1, flip W0 give y = C^T C S r, and z = D^-1 r + S y at the unknowns'
+// voxels. In exact arithmetic M is symmetric positive definite with
+// M >= D^-1 (Jacobi plus a positive semidefinite learned correction); the
+// bfloat16 evaluation makes the applied map slightly inexact, so the outer
 // iteration is flexible CG with one stored direction (Notay's FCG(1)):
 //   q = A p; alpha = (r.p)/(p.q); x += alpha p; r -= alpha q;
 //   z = M(r); beta = -(z.q)/(p.q); p = z + beta p.
@@ -74,19 +76,22 @@ __device__ inline void reduce_parts_n(const double* __restrict__ part, int npart
   __syncthreads();
 }
 
-// w = D^-1 r written as channel 0 of the grid at the unknowns' voxels.
+// Jacobi part D^-1 r (zero diagonal -> identity, as the Jacobi-PCG loop) and
+// the symmetric scaling S = D^-1/2 (non-positive diagonal -> 1).
+__device__ inline double jacobi_part(double r, double d) { return fabs(d) > 0.0 ? r / d : r; }
+__device__ inline double sym_scale(double d) { return d > 0.0 ? 1.0 / sqrt(d) : 1.0; }
+
+// S r written as channel 0 of the grid at the unknowns' voxels.
 __global__ __launch_bounds__(256) void k_prec_in(int n, const int* __restrict__ red2full, const double* __restrict__ r,
                                                  const double* __restrict__ diag, uint16_t* __restrict__ grid,
                                                  const FcgState* __restrict__ st) {
   if (st && st->status != CC_RUNNING) return;
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const double d = diag[j];
-    const double w = fabs(d) > 0.0 ? r[j] / d : r[j];
-    grid[(size_t)red2full[j] * 16] = f2bf_d((float)w);
-  }
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    grid[(size_t)red2full[j] * 16] = f2bf_d((float)(sym_scale(diag[j]) * r[j]));
 }
 
-// z = w + y (y = channel 0 of the last layer), partial z.q (q may be null).
+// z = D^-1 r + S y (y = channel 0 of the last layer), partial z.q (q may be
+// null).
 __global__ __launch_bounds__(256) void k_prec_out(int n, const int* __restrict__ red2full, const double* __restrict__ r,
                                                   const double* __restrict__ diag, const float* __restrict__ y,
                                                   double* __restrict__ z, const double* __restrict__ q,
@@ -96,8 +101,7 @@ __global__ __launch_bounds__(256) void k_prec_out(int n, const int* __restrict__
   double zq = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const double d = diag[j];
-    const double w = fabs(d) > 0.0 ? r[j] / d : r[j];
-    const double zz = w + (double)y[(size_t)red2full[j] * 16];
+    const double zz = jacobi_part(r[j], d) + sym_scale(d) * (double)y[(size_t)red2full[j] * 16];
     z[j] = zz;
     if (q) zq += zz * q[j];
   }
@@ -280,7 +284,7 @@ static FcgBuffers carve_fcg(char* base, size_t n, size_t voxels) {
   return b;
 }
 
-// The learned preconditioner z = w + CNN(w), w = D^-1 r; partial z.q.
+// The learned preconditioner z = D^-1 r + S C^T C S r; partial z.q.
 static int learned_prec(const cc_cnn_desc& net, const FcgBuffers& b, const double* r, const double* diag,
                         const double* q, double* part, hipStream_t s) {
   const int n = net.n_red;
@@ -345,20 +349,17 @@ int fcg_operator(int n, const std::function<int(const double*, double*, double*,
 
 size_t fcg_operator_bytes(size_t n, size_t voxels) { return fcg_bytes(n, voxels); }
 
-// Training input: grid[red2full[j]][0] = bf16(scale * D^-1 r_j).
+// Training input: grid[red2full[j]][0] = bf16(scale * S r_j).
 __global__ __launch_bounds__(256) void k_train_input(int n, const int* __restrict__ red2full,
                                                      const float* __restrict__ r, const double* __restrict__ diag,
                                                      float scale, uint16_t* __restrict__ grid) {
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
-    const double d = diag[j];
-    const double w = fabs(d) > 0.0 ? (double)r[j] / d : (double)r[j];
-    grid[(size_t)red2full[j] * 16] = f2bf_d((float)(scale * w));
-  }
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    grid[(size_t)red2full[j] * 16] = f2bf_d((float)(scale * sym_scale(diag[j]) * (double)r[j]));
 }
 
-// Loss ||z - t||^2 / ||t||^2 with z = w + y, w = scale D^-1 r, t = scale e;
-// writes dL/dy (times gscale) as channel 0 of the bf16 gradient grid and the
-// per-block partials of (||z - t||^2, ||t||^2).
+// Loss ||z - t||^2 / ||t||^2 with z = scale D^-1 r + S y, t = scale e;
+// writes dL/dy = 2 (z - t) S / ||t||^2 (times gscale) as channel 0 of the
+// bf16 gradient grid and the per-block partials of (||z - t||^2, ||t||^2).
 __global__ __launch_bounds__(256) void k_train_loss(int n, const int* __restrict__ red2full,
                                                     const float* __restrict__ r, const float* __restrict__ e,
                                                     const double* __restrict__ diag, float scale,
@@ -368,14 +369,14 @@ __global__ __launch_bounds__(256) void k_train_loss(int n, const int* __restrict
   double dd = 0.0, tt = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const double d = diag[j];
-    const double w = scale * (fabs(d) > 0.0 ? (double)r[j] / d : (double)r[j]);
+    const double sj = sym_scale(d);
     const size_t v = (size_t)red2full[j];
-    const double z = w + (double)y[v * 16];
+    const double z = scale * jacobi_part((double)r[j], d) + sj * (double)y[v * 16];
     const double t = scale * (double)e[j];
     const double diff = z - t;
     dd += diff * diff;
     tt += t * t;
-    gout[v * 16] = f2bf_d((float)(2.0 * diff * inv_den * gscale));
+    gout[v * 16] = f2bf_d((float)(2.0 * diff * sj * inv_den * gscale));
   }
   double v2[2] = {dd, tt}, out[2];
   block_sum_n<2>(v2, out, red);

@@ -169,8 +169,71 @@ class CutData:
     gamma: np.ndarray  # (n_full,)
 
 
-def levelset_cut_data(grid: Grid3D, levelset, quadrature_order: int | None = None, chunk_cells: int = 64) -> CutData:
-    """Classify cells and integrate the cut cells' local Gram blocks."""
+def _gram_chunk(args):
+    """Gram blocks of a chunk of cut cells from their clipped sub-tetrahedra."""
+    axes_spec, p, q, cells, sub, cell_of = args
+    axes = [Axis(*spec) for spec in axes_spec]
+    p1 = p + 1
+    nc = cells.shape[0]
+    ref_pts, ref_w = _tet_rule(q)
+    nq = ref_w.size
+    hvec = np.array([a.h for a in axes])
+    base = cells * hvec
+    e1 = sub[:, 1] - sub[:, 0]
+    e2 = sub[:, 2] - sub[:, 0]
+    e3 = sub[:, 3] - sub[:, 0]
+    det = np.abs(np.einsum("ti,ti->t", e1, np.cross(e2, e3)))
+    frac = np.bincount(cell_of, weights=det / 6.0, minlength=nc) / np.prod(hvec)
+    pts = (sub[:, 0, None, :] + ref_pts[None, :, 0, None] * e1[:, None, :] + ref_pts[None, :, 1, None] * e2[:, None, :]
+           + ref_pts[None, :, 2, None] * e3[:, None, :])
+    wts = det[:, None] * ref_w[None, :]
+    order = np.argsort(cell_of, kind="stable")
+    cs = cell_of[order]
+    counts = np.bincount(cs, minlength=nc)
+    maxc = int(counts.max())
+    slot = np.arange(order.size) - np.repeat(np.cumsum(counts) - counts, counts)
+    PP = np.zeros((nc, maxc, nq, 3))
+    WW = np.zeros((nc, maxc, nq))
+    PP[cs, slot] = pts[order]
+    WW[cs, slot] = wts[order]
+    PP += (np.arange(maxc)[None, :, None, None] >= counts[:, None, None, None]) * base[:, None, None, :]
+    npt = maxc * nq
+    PP = PP.reshape(nc, npt, 3)
+    WW = WW.reshape(nc, npt)
+    vals, ders = [], []
+    for k in range(3):
+        axk = axes[k]
+        e_rep = np.repeat(cells[:, k], npt)
+        xq = np.clip(PP[:, :, k].ravel(), e_rep * axk.h, (e_rep + 1) * axk.h)
+        V, D = axk.local_values(e_rep, xq)
+        vals.append(V.reshape(p1, nc, npt))
+        ders.append(D.reshape(p1, nc, npt))
+    Vx, Vy, Vz = vals
+    Dx, Dy, Dz = ders
+
+    def tensor(a, b, c):
+        return (a[:, None, None] * b[None, :, None] * c[None, None, :]).reshape(p1**3, nc, npt).transpose(1, 0, 2)
+
+    W = WW[:, None, :]
+    phi_ = tensor(Vz, Vy, Vx)
+    mass = np.matmul(phi_ * W, phi_.transpose(0, 2, 1))
+    stiff = np.zeros_like(mass)
+    for g in (tensor(Vz, Vy, Dx), tensor(Vz, Dy, Vx), tensor(Dz, Vy, Vx)):
+        stiff += np.matmul(g * W, g.transpose(0, 2, 1))
+    return mass, stiff, frac
+
+
+def levelset_cut_data(grid: Grid3D, levelset, quadrature_order: int | None = None, chunk_cells: int = 32,
+                      jobs: int | None = None) -> CutData:
+    """Classify cells and integrate the cut cells' local Gram blocks.
+
+    Cut cells are clipped all at once, sorted by their number of
+    sub-tetrahedra (so chunks carry little padding) and integrated in a
+    process pool of ``jobs`` workers (default: the CPU count, at most 16).
+    """
+    import os
+    from concurrent.futures import ProcessPoolExecutor
+
     ax, ay, az = grid.axes
     p = grid.degree
     p1 = p + 1
@@ -188,59 +251,48 @@ def levelset_cut_data(grid: Grid3D, levelset, quadrature_order: int | None = Non
     n_in = (corner >= 0.0).sum(axis=-1)
     cls = np.where(n_in == 8, 1, np.where(n_in == 0, 0, 2)).astype(np.int8)
     cz, cy, cx = np.nonzero(cls == 2)
-    ref_pts, ref_w = _tet_rule(q)
-    nq = ref_w.size
     hvec = np.array([ax.h, ay.h, az.h])
-    masses, stiffs, fracs = [], [], []
-    for s in range(0, cz.size, chunk_cells):
-        ez, ey, ex = cz[s : s + chunk_cells], cy[s : s + chunk_cells], cx[s : s + chunk_cells]
-        nc = ez.size
-        base = np.stack([ex, ey, ez], axis=1) * hvec
+    nc_all = cz.size
+    if nc_all:
+        base = np.stack([cx, cy, cz], axis=1) * hvec
         P = base[:, None, None, :] + hvec * _CORNERS[_KUHN][None, :, :, :]
-        F = corner[ez, ey, ex][:, _KUHN]
-        sub, parent = _clip_tets(P.reshape(nc * 6, 4, 3), F.reshape(nc * 6, 4))
-        cell_of = parent // 6
-        e1 = sub[:, 1] - sub[:, 0]
-        e2 = sub[:, 2] - sub[:, 0]
-        e3 = sub[:, 3] - sub[:, 0]
-        det = np.abs(np.einsum("ti,ti->t", e1, np.cross(e2, e3)))
-        fracs.append(np.bincount(cell_of, weights=det / 6.0, minlength=nc) / np.prod(hvec))
-        pts = (sub[:, 0, None, :] + ref_pts[None, :, 0, None] * e1[:, None, :] + ref_pts[None, :, 1, None] * e2[:, None, :]
-               + ref_pts[None, :, 2, None] * e3[:, None, :])
-        wts = det[:, None] * ref_w[None, :]
-        order = np.argsort(cell_of, kind="stable")
-        cs = cell_of[order]
-        counts = np.bincount(cs, minlength=nc)
-        maxc = int(counts.max())
-        slot = np.arange(order.size) - np.repeat(np.cumsum(counts) - counts, counts)
-        PP = np.zeros((nc, maxc, nq, 3))
-        WW = np.zeros((nc, maxc, nq))
-        PP[cs, slot] = pts[order]
-        WW[cs, slot] = wts[order]
-        PP += (np.arange(maxc)[None, :, None, None] >= counts[:, None, None, None]) * base[:, None, None, :]
-        npt = maxc * nq
-        PP = PP.reshape(nc, npt, 3)
-        WW = WW.reshape(nc, npt)
-        vals, ders = [], []
-        for k, (axk, el) in enumerate(((ax, ex), (ay, ey), (az, ez))):
-            e_rep = np.repeat(el, npt)
-            xq = np.clip(PP[:, :, k].ravel(), e_rep * axk.h, (e_rep + 1) * axk.h)
-            V, D = axk.local_values(e_rep, xq)
-            vals.append(V.reshape(p1, nc, npt))
-            ders.append(D.reshape(p1, nc, npt))
-        Vx, Vy, Vz = vals
-        Dx, Dy, Dz = ders
-
-        def tensor(a, b, c):
-            return (a[:, None, None] * b[None, :, None] * c[None, None, :]).reshape(p1**3, nc, npt).transpose(1, 0, 2)
-
-        W = WW[:, None, :]
-        phi_ = tensor(Vz, Vy, Vx)
-        masses.append(np.matmul(phi_ * W, phi_.transpose(0, 2, 1)))
-        Kl = np.zeros_like(masses[-1])
-        for g in (tensor(Vz, Vy, Dx), tensor(Vz, Dy, Vx), tensor(Dz, Vy, Vx)):
-            Kl += np.matmul(g * W, g.transpose(0, 2, 1))
-        stiffs.append(Kl)
+        F = corner[cz, cy, cx][:, _KUHN]
+        sub_all, parent = _clip_tets(P.reshape(nc_all * 6, 4, 3), F.reshape(nc_all * 6, 4))
+        cell_all = parent // 6
+        nsub = np.bincount(cell_all, minlength=nc_all)
+        by_count = np.argsort(nsub, kind="stable")
+        rank = np.empty(nc_all, dtype=np.int64)
+        rank[by_count] = np.arange(nc_all)
+        order = np.argsort(rank[cell_all], kind="stable")
+        sub_sorted = sub_all[order]
+        owner = rank[cell_all[order]]  # cell position in count order
+        starts = np.searchsorted(owner, np.arange(0, nc_all, chunk_cells))
+        ends = np.append(starts[1:], owner.size)
+        axes_spec = [(a.n_elements, a.degree, a.length, a.periodic) for a in grid.axes]
+        cells_sorted = np.stack([cx, cy, cz], axis=1)[by_count].astype(np.int64)
+        tasks = []
+        for k, c0 in enumerate(range(0, nc_all, chunk_cells)):
+            c1 = min(c0 + chunk_cells, nc_all)
+            tasks.append((axes_spec, p, q, cells_sorted[c0:c1], sub_sorted[starts[k] : ends[k]],
+                          owner[starts[k] : ends[k]] - c0))
+        jobs = jobs or min(16, os.cpu_count() or 1)
+        if jobs > 1 and len(tasks) > 1:
+            with ProcessPoolExecutor(max_workers=jobs) as ex:
+                parts = list(ex.map(_gram_chunk, tasks))
+        else:
+            parts = [_gram_chunk(t) for t in tasks]
+        masses = [m for m, _, _ in parts]
+        stiffs = [s_ for _, s_, _ in parts]
+        fracs = [f for _, _, f in parts]
+        # Back to the row-major order of the cut cells.
+        inv = np.empty(nc_all, dtype=np.int64)
+        inv[by_count] = np.arange(nc_all)
+        mass_sorted = np.concatenate(masses)
+        stiff_sorted = np.concatenate(stiffs)
+        frac_sorted = np.concatenate(fracs)
+        masses, stiffs, fracs = [mass_sorted[inv]], [stiff_sorted[inv]], [frac_sorted[inv]]
+    else:
+        masses, stiffs, fracs = [], [], []
     n_cut = cz.size
     cut_mass = np.concatenate(masses) if masses else np.zeros((0, p1**3, p1**3))
     cut_stiff = np.concatenate(stiffs) if stiffs else np.zeros((0, p1**3, p1**3))

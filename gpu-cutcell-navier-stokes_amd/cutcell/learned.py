@@ -4,18 +4,22 @@ background dof grid, trained on the GPU and applied inside flexible CG.
 Everything numerical runs in the HIP kernels of ``csrc/cnn.hip`` and
 ``csrc/fcg.hip`` (bfloat16 implicit-GEMM convolutions on MFMA with fp32
 accumulation, fp32 weight gradients, Adam); PyTorch only owns the device
-buffers. The network is
+buffers. The network is symmetric by construction:
 
-    w = D^-1 r  (Jacobi, as channel 0 of the grid at the unknowns' voxels)
-    h1 = mask * relu(conv(w)), h2 = mask * relu(conv(h1)),
-    h3 = mask * relu(conv(h2)), y = mask * conv(h3)
-    z = w + y[channel 0]
+    u  = S r               (S = D^-1/2, channel 0 at the unknowns' voxels)
+    h1 = mask * conv(u, W0),       h2 = mask * conv(h1, W1)
+    h3 = mask * conv(h2, flip W1), y  = mask * conv(h3, flip W0)
+    z  = D^-1 r + S y[channel 0]
 
-with no biases, so ``z(a r) = a z(r)`` for ``a > 0``. It is trained on
-(residual, error) pairs taken from the solver's own Jacobi-PCG iterates,
-minimising ``||z(r) - e||^2 / ||e||^2`` per pair (pairs scaled to unit
-``||D^-1 r||``). Inside flexible CG the float64 true residual decides
-success; a learned solve that does not converge falls back to Jacobi-PCG.
+i.e. ``z = (D^-1 + S C^T C S) r`` with ``C = L1 L0`` (the flipped layers are
+the adjoints: reversed taps, transposed channels). In exact arithmetic the
+preconditioner is symmetric positive definite and never below Jacobi
+(``M >= D^-1``), so training cannot produce an indefinite map; the
+bfloat16 evaluation makes the applied map slightly inexact, which the
+flexible CG absorbs. It is trained on (residual, error) pairs taken from the
+solver's own Jacobi-PCG iterates, minimising ``||z(r) - e||^2 / ||e||^2``
+per pair. Inside flexible CG the float64 true residual decides success; a
+learned solve that does not converge falls back to Jacobi-PCG.
 
 The reference has no learned preconditioner; its Jacobi rule is
 ``code/fast_solver_backends.py:74-87`` (CPU) and ``:237-238`` (GPU).
@@ -54,6 +58,8 @@ def _bind(lib):
         "cc_mfma_probe": [P, P, P, P],
         "cc_cnn_conv3_forward": [I, I, I, P, P, P, P, P, I, P, P, P],
         "cc_cnn_flip_weights": [P, P, P],
+        "cc_cnn_sym_weights": [P, P, P],
+        "cc_cnn_sym_grads": [P, P, I, P],
         "cc_cnn_conv3_wgrad": [I, I, I, P, P, P, P, F, I, I, P, SZ, P],
         "cc_cnn_adam": [I, P, P, P, P, F, F, F, F, I, P, P, P],
         "cc_cnn_train_input": [I, P, P, P, F, P, P],
@@ -103,10 +109,11 @@ class TrainReport:
 
 
 class MaskedCNN:
-    """Weights, optimiser state and device buffers of the preconditioner net."""
+    """Master weights, optimiser state and device buffers of the symmetric
+    preconditioner network (two fp32 masters W0, W1; four bf16 layers)."""
 
     def __init__(self, grid_shape, red2full: np.ndarray, periodic=(False, False, False), seed: int = 0,
-                 device: str = "cuda", init_scale: float = 1.0):
+                 device: str = "cuda", init_scale: float = 0.3):
         import torch
 
         self.torch = torch
@@ -117,22 +124,16 @@ class MaskedCNN:
         self.device = device
         self.n = int(red2full.size)
         rng = np.random.default_rng(seed)
-        ws = []
-        for layer in range(NLAYERS):
-            fan_in = 27 * (1 if layer == 0 else 16)
-            w = rng.standard_normal((27, 16, 16)) * np.sqrt(2.0 / fan_in) * init_scale
-            if layer == 0:
-                w[:, 1:, :] = 0.0  # only channel 0 carries the input
-            if layer == NLAYERS - 1:
-                w *= 0.05  # start close to plain Jacobi
-                w[:, :, 1:] = 0.0
-            ws.append(w.astype(np.float32).ravel())
+        w0 = rng.standard_normal((27, 16, 16)) * np.sqrt(1.0 / 27.0) * init_scale
+        w0[:, 1:, :] = 0.0  # the input is channel 0 only
+        w1 = rng.standard_normal((27, 16, 16)) * np.sqrt(1.0 / (27.0 * 16.0))
         t = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(device)
-        self.w32 = t(np.stack(ws), np.float32)  # (4, WSIZE)
-        self.wbf = t(np.stack([to_bf16_bits(w) for w in ws]), np.uint16)
+        self.w32 = t(np.stack([w0.ravel(), w1.ravel()]).astype(np.float32), np.float32)  # (2, WSIZE)
+        self.wbf = torch.zeros((NLAYERS, WSIZE), dtype=torch.uint16, device=device)
         self.m1 = torch.zeros_like(self.w32)
         self.m2 = torch.zeros_like(self.w32)
-        self.grad = torch.zeros_like(self.w32)
+        self.grad = torch.zeros_like(self.w32)  # masters' gradient
+        self.lgrad = torch.zeros((NLAYERS, WSIZE), dtype=torch.float32, device=device)  # per layer
         self.step_count = 0
         mask = np.zeros(self.voxels, dtype=np.uint8)
         mask[red2full] = 1
@@ -144,12 +145,26 @@ class MaskedCNN:
         self.y = torch.zeros(self.voxels * 16, dtype=torch.float32, device=device)
         self.g = [z16() for _ in range(2)]
         self.gout = z16()
-        self.flip = torch.zeros(WSIZE, dtype=torch.uint16, device=device)
         self.wgrad_blocks = 512
         nb = int(self.lib.cc_cnn_wgrad_workspace_bytes(self.nx, self.ny, self.nz, self.wgrad_blocks))
         self.wgrad_ws = torch.empty(nb, dtype=torch.uint8, device=device)
         self.part = torch.zeros(2 * 1024, dtype=torch.float64, device=device)
         self._desc = None
+        self.refresh_layers()
+
+    def refresh_layers(self):
+        """bf16 layers (W0, W1, flip W1, flip W0) from the fp32 masters."""
+        hip._check(self.lib.cc_cnn_sym_weights(hip._ptr(self.w32), hip._ptr(self.wbf), hip._stream()), "sym_weights")
+
+    def set_masters(self, w0: np.ndarray, w1: np.ndarray):
+        torch = self.torch
+        self.w32.copy_(torch.from_numpy(np.stack([np.ravel(w0), np.ravel(w1)]).astype(np.float32)))
+        self.refresh_layers()
+
+    def layer_weights(self) -> list:
+        """The four bf16 layers as float64 arrays [27][16][16] (host copies)."""
+        return [from_bf16_bits(self.wbf[l].cpu().numpy()).astype(np.float64).reshape(27, 16, 16)
+                for l in range(NLAYERS)]
 
     # -- descriptor for the fused preconditioner / FCG ------------------------
     def desc(self) -> CnnDesc:
@@ -166,47 +181,49 @@ class MaskedCNN:
         return d
 
     # -- layers ---------------------------------------------------------------
-    def _conv(self, x, layer_w, relu: bool, out=None, out_f32=None, gate=None):
+    def _conv(self, x, layer_w, out=None, out_f32=None):
         p = hip._ptr
         hip._check(self.lib.cc_cnn_conv3_forward(self.nx, self.ny, self.nz, self.periodic, p(x), p(layer_w),
-                                                 p(self.mask), p(gate) if gate is not None else None, int(relu),
-                                                 p(out) if out is not None else None,
+                                                 p(self.mask), None, 0, p(out) if out is not None else None,
                                                  p(out_f32) if out_f32 is not None else None, hip._stream()),
                    "conv3_forward")
 
     def forward(self):
         """acts[0] holds the input; fills acts[1..3] and y (fp32)."""
         for l in range(NLAYERS - 1):
-            self._conv(self.acts[l], self.wbf[l], True, out=self.acts[l + 1])
-        self._conv(self.acts[NLAYERS - 1], self.wbf[NLAYERS - 1], False, out_f32=self.y)
+            self._conv(self.acts[l], self.wbf[l], out=self.acts[l + 1])
+        self._conv(self.acts[NLAYERS - 1], self.wbf[NLAYERS - 1], out_f32=self.y)
 
     def backward(self, scale: float, accumulate: bool):
-        """gout holds dL/dy (channel 0); accumulates scale * dL/dW into grad."""
+        """gout holds dL/dy (channel 0); adds scale * dL/dW into the masters'
+        gradient (or sets it when ``accumulate`` is false). The data gradient
+        of layer l is the layer with flipped weights, which is layer 3 - l."""
         p = hip._ptr
         g = self.gout
         for l in range(NLAYERS - 1, -1, -1):
             hip._check(self.lib.cc_cnn_conv3_wgrad(self.nx, self.ny, self.nz, self.periodic, p(self.acts[l]), p(g),
-                                                   p(self.grad[l]), float(scale), int(accumulate),
-                                                   self.wgrad_blocks, p(self.wgrad_ws), self.wgrad_ws.numel(),
-                                                   hip._stream()), "conv3_wgrad")
+                                                   p(self.lgrad[l]), float(scale), 0, self.wgrad_blocks,
+                                                   p(self.wgrad_ws), self.wgrad_ws.numel(), hip._stream()),
+                       "conv3_wgrad")
             if l == 0:
                 break
-            hip._check(self.lib.cc_cnn_flip_weights(p(self.wbf[l]), p(self.flip), hip._stream()), "flip")
             gnext = self.g[l % 2]
-            self._conv(g, self.flip, False, out=gnext, gate=self.acts[l])
+            self._conv(g, self.wbf[NLAYERS - 1 - l], out=gnext)
             g = gnext
+        hip._check(self.lib.cc_cnn_sym_grads(p(self.lgrad), p(self.grad), int(accumulate), hip._stream()), "sym_grads")
 
     def adam_step(self, lr: float = 1e-3, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
         self.step_count += 1
         p = hip._ptr
-        hip._check(self.lib.cc_cnn_adam(NLAYERS * WSIZE, p(self.w32), p(self.grad), p(self.m1), p(self.m2), lr, beta1,
+        hip._check(self.lib.cc_cnn_adam(2 * WSIZE, p(self.w32), p(self.grad), p(self.m1), p(self.m2), lr, beta1,
                                         beta2, eps, self.step_count, None, p(self.wbf), hip._stream()), "adam")
+        self.refresh_layers()  # Adam rounded W0, W1 into layers 0, 1; rebuild all four
 
     # -- training -------------------------------------------------------------
     def sample_loss_grad(self, r32, e32, diag, scale: float, gscale: float, accumulate: bool,
                          e_norm2: float) -> float:
         """Forward + backward for one (r, e) pair (``e_norm2 = ||e||^2``);
-        returns its loss (synchronises)."""
+        returns its loss as a device scalar (no synchronisation)."""
         p = hip._ptr
         hip._check(self.lib.cc_cnn_train_input(self.n, p(self.red2full), p(r32), p(diag), float(scale), p(self.acts[0]),
                                                hip._stream()), "train_input")
@@ -218,7 +235,7 @@ class MaskedCNN:
                                               ctypes.addressof(nparts), hip._stream()), "train_loss")
         self.backward(1.0, accumulate)
         parts = self.part[: 2 * nparts.value].view(-1, 2).sum(dim=0)
-        return float((parts[0] / parts[1]).item())
+        return parts[0] / parts[1]
 
     def train(self, R, Eerr, diag, *, steps: int = 200, batch: int = 8, lr: float = 2e-3, seed: int = 0,
               log_every: int = 0) -> TrainReport:
@@ -236,7 +253,7 @@ class MaskedCNN:
                 losses.append(self.sample_loss_grad(R[i], Eerr[i], diag, float(scales[i]), 1.0 / len(idx), k > 0,
                                                     float(enorm2[i])))
             self.adam_step(lr)
-            hist.append(float(np.mean(losses)))
+            hist.append(float(torch.stack(losses).mean().item()))
             if log_every and step % log_every == 0:
                 print(f"[cnn] step {step} loss {hist[-1]:.4f}", flush=True)
         torch.cuda.synchronize()
@@ -245,10 +262,13 @@ class MaskedCNN:
 
     @staticmethod
     def _sample_scales(R, Eerr, diag):
-        """Per pair: 1/||D^-1 r|| and ||e||^2 (bookkeeping of the data set)."""
-        dinv_norm = (R.double() / diag.double()).norm(dim=1).cpu().numpy()
+        """Per pair: 1/||S r|| (the input is scaled to unit norm) and ||e||^2
+        (bookkeeping of the data set)."""
+        s = diag.double().clamp_min(0.0)
+        s = torch_where_pos_rsqrt(s)
+        in_norm = (R.double() * s).norm(dim=1).cpu().numpy()
         enorm2 = Eerr.double().pow(2).sum(dim=1).cpu().numpy()
-        return 1.0 / np.maximum(dinv_norm, 1e-300), enorm2
+        return 1.0 / np.maximum(in_norm, 1e-300), enorm2
 
     def evaluate(self, R, Eerr, diag) -> float:
         """Mean loss over the given pairs (no weight update)."""
@@ -258,7 +278,7 @@ class MaskedCNN:
         for i in range(R.shape[0]):
             out.append(self.sample_loss_grad(R[i], Eerr[i], diag, float(scales[i]), 0.0, True, float(enorm2[i])))
         self.grad.copy_(keep)
-        return float(np.mean(out))
+        return float(self.torch.stack(out).mean().item())
 
     # -- application ----------------------------------------------------------
     def precondition(self, r, diag):
@@ -292,6 +312,13 @@ class MaskedCNN:
         if history:
             return x, res, hist.cpu().numpy()
         return x, res
+
+
+def torch_where_pos_rsqrt(d):
+    """1/sqrt(d) where d > 0, else 1 (the kernels' symmetric scaling rule)."""
+    import torch
+
+    return torch.where(d > 0, torch.rsqrt(torch.where(d > 0, d, torch.ones_like(d))), torch.ones_like(d))
 
 
 def generate_pairs(dmf, rhs, snapshots, tol_ref: float = 1.0e-12):

@@ -235,7 +235,7 @@ int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* b, double* x, double 
 typedef struct cc_cnn_desc {
   int32_t nx, ny, nz;
   int32_t periodic[3];
-  const uint16_t* weights[4];
+  const uint16_t* weights[4]; /* the four layers: W0, W1, flip W1, flip W0 */
   const uint8_t* mask;
   const int32_t* red2full; /* voxel of each unknown */
   int32_t n_red;
@@ -258,18 +258,29 @@ size_t cc_cnn_wgrad_workspace_bytes(int32_t nx, int32_t ny, int32_t nz, int32_t 
 int cc_cnn_conv3_wgrad(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
                        const uint16_t* gout, float* dW, float scale, int32_t accumulate, int32_t blocks,
                        void* workspace, size_t workspace_bytes, cc_stream_t stream);
+/* The preconditioner network is symmetric: C = L1 L0 (masked linear layers,
+ * W0 lifting channel 0 to 16 channels, W1 mixing 16 -> 16) and
+ *     z = D^-1 r + S C^T C S r,  S = D^-1/2,
+ * evaluated as four layers with weights (W0, W1, flip W1, flip W0).
+ * cc_cnn_sym_weights writes those four bf16 layers from the fp32 masters
+ * [2][27][16][16]; cc_cnn_sym_grads turns the four layers' gradients
+ * [4][27][16][16] into the masters' (g0 + flip g3, g1 + flip g2), added to
+ * g_master when accumulate != 0. */
+int cc_cnn_sym_weights(const float* w_master, uint16_t* layers, cc_stream_t stream);
+int cc_cnn_sym_grads(const float* g_layers, float* g_master, int32_t accumulate, cc_stream_t stream);
 /* One Adam step on fp32 master weights w (n values); also refreshes the
  * bfloat16 copy. frozen (may be NULL) marks entries kept fixed. */
 int cc_cnn_adam(int32_t n, float* w, const float* grad, float* m1, float* m2, float lr, float beta1, float beta2,
                 float eps, int32_t step, const uint8_t* frozen, uint16_t* w_bf16, cc_stream_t stream);
-/* z = w + CNN(w), w = D^-1 r (zero diagonal -> identity), float64 in/out. */
+/* z = D^-1 r + S y, y = channel 0 of the network applied to S r (S = D^-1/2;
+ * a non-positive diagonal entry counts as 1), float64 in/out. */
 size_t cc_cnn_precondition_workspace_bytes(const cc_cnn_desc* net);
 int cc_cnn_precondition(const cc_cnn_desc* net, const double* r, const double* diag, double* z, void* workspace,
                         size_t workspace_bytes, cc_stream_t stream);
-/* Training helpers: the network input scale * D^-1 r as channel 0 of a bf16
+/* Training helpers: the network input scale * S r as channel 0 of a bf16
  * grid (other entries untouched), and the loss ||z - scale e||^2 /
- * ||scale e||^2 with z = scale D^-1 r + y (y: fp32 channel-0 output), its
- * gradient (times gscale) written to channel 0 of gout, and per-block
+ * ||scale e||^2 with z = scale D^-1 r + S y (y: fp32 channel-0 output), its
+ * gradient dL/dy (times gscale) written to channel 0 of gout, and per-block
  * partials (||z - t||^2, ||t||^2) in partial[2 * nparts]. r, e: float32. */
 int cc_cnn_train_input(int32_t n, const int32_t* red2full, const float* r, const double* diag, float scale,
                        uint16_t* grid, cc_stream_t stream);

@@ -121,3 +121,85 @@ def fd_check(r, e, Ws, mask, n_probe=6, h=1e-6, seed=0, periodic=(False, False, 
         an = grads[i][idx]
         worst = max(worst, abs(fd - an) / max(abs(fd), abs(an), 1e-12))
     return worst
+
+
+# ---------------------------------------------------------------------------
+# The symmetric (tied-weight) preconditioner network.
+#
+#   z = D^-1 r + S C^T C S r,   S = D^-1/2,   C = L1 L0 (two masked linear
+#   3x3x3 layers: W0 lifts channel 0 to 16 channels, W1 mixes 16 -> 16),
+#
+# run as four layers with weights (W0, W1, flip W1, flip W0): the adjoint of a
+# correlation is the correlation with reversed taps and transposed channels.
+# M = D^-1 + S C^T C S is symmetric positive definite with M >= D^-1.
+# ---------------------------------------------------------------------------
+
+
+def flip(W: np.ndarray) -> np.ndarray:
+    """``flip(W)[t, a, b] = W[26 - t, b, a]`` (the adjoint layer's weights)."""
+    return np.ascontiguousarray(W[::-1].transpose(0, 2, 1))
+
+
+def sym_layers(W0: np.ndarray, W1: np.ndarray):
+    return [W0, W1, flip(W1), flip(W0)]
+
+
+def _inv_sqrt(d: np.ndarray) -> np.ndarray:
+    return np.where(d > 0, 1.0 / np.sqrt(np.where(d > 0, d, 1.0)), 1.0)
+
+
+def sym_network(r, d, W0, W1, mask, periodic=(False, False, False)):
+    """``z`` of the symmetric network for grids ``r, d`` (``[z, y, x]``, zero
+    outside ``mask``); returns ``(z, acts)``."""
+    s = _inv_sqrt(d) * mask
+    x = np.zeros(r.shape + (16,))
+    x[..., 0] = s * r
+    acts = [x]
+    for W in sym_layers(W0, W1):
+        out, _ = conv3(acts[-1], W, mask, relu=False, periodic=periodic)
+        acts.append(out)
+    dinv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 1.0) * mask
+    z = dinv * r + s * acts[-1][..., 0]
+    return z, acts
+
+
+def sym_loss_and_grads(r, e, d, W0, W1, mask, periodic=(False, False, False)):
+    """``L = ||z - e||^2 / ||e||^2`` (masked voxels) and ``(dL/dW0, dL/dW1)``,
+    each the sum of its two tied layers' gradients."""
+    z, acts = sym_network(r, d, W0, W1, mask, periodic)
+    diff = (z - e) * mask
+    den = float(np.sum((e * mask) ** 2))
+    L = float(np.sum(diff**2)) / den
+    s = _inv_sqrt(d) * mask
+    g = np.zeros(acts[-1].shape)
+    g[..., 0] = 2.0 * diff * s / den
+    layers = sym_layers(W0, W1)
+    dV = [None] * 4
+    for i in range(3, -1, -1):
+        gx, dW = conv3_backward(acts[i], layers[i], mask, None, g, relu=False, periodic=periodic)
+        dV[i] = dW
+        g = gx
+    return L, (dV[0] + flip(dV[3]), dV[1] + flip(dV[2]))
+
+
+def sym_fd_check(r, e, d, W0, W1, mask, n_probe=6, h=1e-6, seed=0, periodic=(False, False, False)):
+    """Max relative deviation of the tied analytic gradients from central
+    differences."""
+    rng = np.random.default_rng(seed)
+    _, grads = sym_loss_and_grads(r, e, d, W0, W1, mask, periodic)
+    worst = 0.0
+    for _ in range(n_probe):
+        i = int(rng.integers(2))
+        idx = tuple(int(rng.integers(sz)) for sz in (27, 16, 16))
+        if i == 0:
+            idx = (idx[0], 0, idx[2])  # W0 reads input channel 0 only
+        Wp = [W0.copy(), W1.copy()]
+        Wm = [W0.copy(), W1.copy()]
+        Wp[i][idx] += h
+        Wm[i][idx] -= h
+        Lp = sym_loss_and_grads(r, e, d, *Wp, mask, periodic)[0]
+        Lm = sym_loss_and_grads(r, e, d, *Wm, mask, periodic)[0]
+        fd = (Lp - Lm) / (2 * h)
+        an = grads[i][idx]
+        worst = max(worst, abs(fd - an) / max(abs(fd), abs(an), 1e-12))
+    return worst

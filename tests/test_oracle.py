@@ -243,3 +243,43 @@ def test_cnn_oracle_conv_is_a_correlation():
     out, _ = cnn_ref.conv3(x, W, np.ones((4, 5, 6)))
     np.testing.assert_array_equal(out[:-1, :, 1:], x[1:, :, :-1])
     assert np.all(out[-1] == 0) and np.all(out[:, :, 0] == 0)
+
+
+def test_symmetric_network_oracle_is_spd_with_fd_gradients():
+    """The tied-weight network z = D^-1 r + S C^T C S r is a symmetric map
+    with M - D^-1 positive semidefinite, and its hand-written tied gradients
+    agree with central differences (clamped and periodic grids)."""
+    from oracle import cnn_ref
+
+    rng = np.random.default_rng(0)
+    nz, ny, nx = 5, 6, 7
+    mask = (rng.random((nz, ny, nx)) > 0.2).astype(float)
+    d = (0.5 + rng.random((nz, ny, nx))) * mask
+    W0 = 0.3 * rng.standard_normal((27, 16, 16))
+    W0[:, 1:, :] = 0.0
+    W1 = 0.2 * rng.standard_normal((27, 16, 16))
+    r = rng.standard_normal((nz, ny, nx)) * mask
+    e = rng.standard_normal((nz, ny, nx)) * mask
+    for per in ((False, False, False), (True, False, True)):
+        assert cnn_ref.sym_fd_check(r, e, d, W0, W1, mask, n_probe=8, periodic=per) < 1e-6
+        a = rng.standard_normal((nz, ny, nx)) * mask
+        b = rng.standard_normal((nz, ny, nx)) * mask
+        Ma, _ = cnn_ref.sym_network(a, d, W0, W1, mask, per)
+        Mb, _ = cnn_ref.sym_network(b, d, W0, W1, mask, per)
+        assert abs(np.sum(Ma * b) - np.sum(a * Mb)) <= 1e-10 * np.sum(np.abs(Ma * b))
+        dinv = np.where(d > 0, 1.0 / np.where(d > 0, d, 1.0), 0.0)
+        assert np.sum(Ma * a) >= np.sum(dinv * a * a) * (1.0 - 1e-12)
+
+
+def test_flip_is_the_adjoint_layer():
+    """<conv(x, W), y> = <x, conv(y, flip W)> on a clamped grid (no mask)."""
+    from oracle import cnn_ref
+
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((4, 5, 6, 16))
+    y = rng.standard_normal((4, 5, 6, 16))
+    W = rng.standard_normal((27, 16, 16))
+    one = np.ones((4, 5, 6))
+    a, _ = cnn_ref.conv3(x, W, one)
+    b, _ = cnn_ref.conv3(y, cnn_ref.flip(W), one)
+    assert abs(np.sum(a * y) - np.sum(x * b)) <= 1e-10 * np.sum(np.abs(a * y))
