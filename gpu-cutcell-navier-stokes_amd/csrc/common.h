@@ -65,10 +65,13 @@ namespace cc {
 // Float64 single-right-hand-side Jacobi-PCG over an operator (pcg.hip).
 // apply(P, Q, partial, &nparts, state) launches Q = A P with per-block
 // partials of P.Q; residual(X, scratch, &rel) computes ||B - A X|| / ||B||.
+// snapshots[j] (device, [n_snap][n]) receives the iterate after
+// snap_iters[j] (host) iterations.
 int cg_operator_f64(int n, const std::function<int(const double*, double*, double*, int*, const ColState*)>& apply,
                     const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
                     int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
-                    const std::function<int(const double*, double*, double*)>& residual);
+                    const std::function<int(const double*, double*, double*)>& residual,
+                    const int* snap_iters = nullptr, int n_snap = 0, double* snapshots = nullptr);
 size_t cg_operator_bytes(size_t n);
 }  // namespace cc
 

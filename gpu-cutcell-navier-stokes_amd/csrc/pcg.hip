@@ -720,10 +720,13 @@ static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external) {
 // partials of P.Q. src is the float64 right-hand side block; inv_scale
 // (device, may be null) scales it into VT. The per-column state is left in
 // b.st (and copied to host_st).
+// after_iter (may be null) is called with the iteration number once that
+// iteration's launches are enqueued (it may enqueue copies of b.X).
 template <typename VT, int K, typename ApplyFn>
 static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_scale, const int* run_mask,
                    CgBuffers<VT>& b, int precond, double tol, int maxiter, int check_every, hipStream_t s,
-                   std::vector<ColState>& host_st, double* hist) {
+                   std::vector<ColState>& host_st, double* hist,
+                   const std::function<int(int)>* after_iter = nullptr) {
   const int sg = stream_grid((size_t)n * K);
   hipLaunchKernelGGL((k_cg_setup<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, src, inv_scale, b.X, b.R, b.Z, b.P, b.diag,
                      precond, b.pa);
@@ -746,6 +749,7 @@ static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_
     hipLaunchKernelGGL((k_cg_direction<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.P, b.Z, b.st);
     prof_end(s);
     CC_CHECK(hipGetLastError());
+    if (after_iter) CC_CHECK((*after_iter)(it));
     if (it % check_every == 0 || it == maxiter) {
       CC_CHECK(hipMemcpyAsync(host_st.data(), b.st, K * sizeof(ColState), hipMemcpyDeviceToHost, s));
       CC_CHECK(hipStreamSynchronize(s));
@@ -779,7 +783,8 @@ static int cg_run(int n, double avg_row, const int* indptr, const int* indices, 
 int cg_operator_f64(int n, const std::function<int(const double*, double*, double*, int*, const ColState*)>& apply,
                     const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
                     int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
-                    const std::function<int(const double*, double*, double*)>& residual) {
+                    const std::function<int(const double*, double*, double*)>& residual, const int* snap_iters,
+                    int n_snap, double* snapshots) {
   Carve cv{(char*)ws, 0, wsb};
   CgBuffers<double> b = carve_cg<double>(cv, n, 1, X);
   if (!cv.ok()) return CC_E_WORKSPACE;
@@ -788,8 +793,21 @@ int cg_operator_f64(int n, const std::function<int(const double*, double*, doubl
   const ColState* st = b.st;
   auto ap = [&](const double* P, double* Q, double* partial, int* gs) { return apply(P, Q, partial, gs, st); };
   std::vector<ColState> hs;
+  // Snapshot j is the iterate after snap_iters[j] iterations (or the final
+  // iterate when the solve stopped earlier).
+  std::function<int(int)> snap = [&](int it) {
+    for (int j = 0; j < n_snap; ++j)
+      if (snap_iters[j] == it)
+        CC_CHECK(hipMemcpyAsync(snapshots + (size_t)j * n, b.X, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice,
+                                s));
+    return 0;
+  };
   CC_CHECK((cg_loop<double, 1>(n, ap, B, nullptr, nullptr, b, precond && diag_src, tol, maxiter, check_every, s, hs,
-                               hist)));
+                               hist, n_snap > 0 ? &snap : nullptr)));
+  for (int j = 0; j < n_snap; ++j)
+    if (snap_iters[j] > hs[0].iterations)
+      CC_CHECK(hipMemcpyAsync(snapshots + (size_t)j * n, b.X, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  CC_CHECK(hipStreamSynchronize(s));
   double rel = 0.0;
   CC_CHECK(residual(X, b.R, &rel));
   result->iterations = hs[0].iterations;

@@ -347,8 +347,10 @@ int cc_sumfact_apply(const cc_sumfact_desc* d, const double* x, double* y, void*
 
 int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* bvec, double* x, double tol, int32_t maxiter,
                    int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
-                   cc_pcg_result* result, double* history, cc_stream_t stream) {
+                   cc_pcg_result* result, double* history, const int32_t* snap_iters, int32_t n_snap,
+                   double* snapshots, cc_stream_t stream) {
   if (!desc_ok(d) || !bvec || !x || !workspace || !result || maxiter < 0) return CC_E_ARG;
+  if (n_snap < 0 || (n_snap > 0 && (!snap_iters || !snapshots))) return CC_E_ARG;
   if (workspace_bytes < cc_sumfact_workspace_bytes(d, 1)) return CC_E_WORKSPACE;
   hipStream_t s = (hipStream_t)stream;
   char* base = (char*)workspace;
@@ -380,7 +382,8 @@ int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* bvec, double* x, doub
     return 0;
   };
   return cg_operator_f64(desc.n_red, apply, preconditioner ? desc.diag : nullptr, bvec, x, tol, maxiter,
-                         preconditioner, check_every, cgws, cgb, result, history, s, residual);
+                         preconditioner, check_every, cgws, cgb, result, history, s, residual, snap_iters, n_snap,
+                         snapshots);
 }
 
 int cc_fcg_sumfact(const cc_sumfact_desc* d, const cc_cnn_desc* net, const double* bvec, double* x, double tol,

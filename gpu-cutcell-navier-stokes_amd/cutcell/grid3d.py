@@ -171,7 +171,13 @@ class CutData:
 
 def _gram_chunk(args):
     """Gram blocks of a chunk of cut cells from their clipped sub-tetrahedra."""
-    axes_spec, p, q, cells, sub, cell_of = args
+    from threadpoolctl import threadpool_limits
+
+    with threadpool_limits(limits=1):  # one BLAS thread per pool worker
+        return _gram_chunk_body(*args)
+
+
+def _gram_chunk_body(axes_spec, p, q, cells, sub, cell_of):
     axes = [Axis(*spec) for spec in axes_spec]
     p1 = p + 1
     nc = cells.shape[0]

@@ -64,7 +64,7 @@ _SIGNATURES = {
     "cc_spmm_csr_f32": ([_I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_sumfact_workspace_bytes": ([_P, _I32], _SZ),
     "cc_sumfact_apply": ([_P, _P, _P, _P, _SZ, _P], ctypes.c_int),
-    "cc_pcg_sumfact": ([_P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int),
+    "cc_pcg_sumfact": ([_P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P, _I32, _P, _P], ctypes.c_int),
     "cc_pcg_batched": (
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _F64, _I32, _I32, _I32,
          _I32, _P, _P, _P],
@@ -228,7 +228,7 @@ def residual(A: DeviceCSR, B, X) -> np.ndarray:
     return rel
 
 
-PROFILE_CLASSES = {"spmm": 0, "update": 1, "direction": 2, "batched": 3}
+PROFILE_CLASSES = {"spmm": 0, "update": 1, "direction": 2, "batched": 3, "learned": 4}
 
 
 def profile_enable(on: bool = True) -> None:

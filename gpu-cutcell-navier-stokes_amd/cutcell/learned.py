@@ -321,23 +321,26 @@ def torch_where_pos_rsqrt(d):
     return torch.where(d > 0, torch.rsqrt(torch.where(d > 0, d, torch.ones_like(d))), torch.ones_like(d))
 
 
-def generate_pairs(dmf, rhs, snapshots, tol_ref: float = 1.0e-12):
+def generate_pairs(dmf, rhs, snapshots, tol_ref: float = 1.0e-11):
     """(residual, error) pairs from the solver's own Jacobi-PCG iterates.
 
-    For every right-hand side b (rows of ``rhs``): x* from a tight solve, then
-    for each k in ``snapshots`` the k-th CG iterate x_k (a solve capped at k
-    iterations from x0 = 0 reproduces it), r_k = b - A x_k (float64 apply)
-    and e_k = x* - x_k. Returns float32 tensors ``R, E`` of shape [S, n].
+    For every right-hand side b (rows of ``rhs``) one tight Jacobi-PCG solve
+    gives x* and, on the way, the iterates x_k after each k in ``snapshots``
+    (copied out by the device loop); r_k = b - A x_k (float64 apply) and
+    e_k = x* - x_k. Snapshots at or past the solve's own stop are dropped.
+    Returns float32 tensors ``R, E`` of shape [S, n].
     """
     import torch
 
+    ks = sorted(int(k) for k in snapshots)
     Rs, Es = [], []
     for b in rhs:
-        xs, res = dmf.pcg(b, tol=tol_ref, maxiter=40000)
-        for k in snapshots:
-            xk, rk = dmf.pcg(b, tol=0.0, maxiter=int(k))
-            r = b - dmf.apply(xk)
-            Rs.append(r.float())
+        xs, res, snaps = dmf.pcg(b, tol=tol_ref, maxiter=40000, snapshots=ks)
+        for j, k in enumerate(ks):
+            if k >= res.iterations:
+                continue
+            xk = snaps[j]
+            Rs.append((b - dmf.apply(xk)).float())
             Es.append((xs - xk).float())
     return torch.stack(Rs), torch.stack(Es)
 

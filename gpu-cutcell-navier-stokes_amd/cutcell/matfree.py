@@ -400,8 +400,12 @@ class DeviceMatrixFree:
                                         hip._stream()), "sumfact_apply")
         return y
 
-    def pcg(self, b, *, tol=1.0e-8, maxiter=5000, preconditioner="jacobi", check_every=16, history=False, x=None):
-        """Device Jacobi-PCG on the matrix-free operator. Returns ``(x, result[, history])``."""
+    def pcg(self, b, *, tol=1.0e-8, maxiter=5000, preconditioner="jacobi", check_every=16, history=False, x=None,
+            snapshots=None):
+        """Device Jacobi-PCG on the matrix-free operator. Returns ``(x, result[, history][, snaps])``;
+        ``snapshots`` (iteration numbers) adds a ``[len, n]`` device tensor of
+        the iterates after those iterations (the final one where the solve
+        stopped earlier)."""
         import torch
 
         from . import hip
@@ -410,10 +414,17 @@ class DeviceMatrixFree:
         x = torch.empty_like(b) if x is None else x
         res = hip.PcgResult()
         hist = torch.full((int(maxiter) + 1,), float("nan"), dtype=torch.float64, device=b.device) if history else None
+        n_snap = 0 if snapshots is None else len(snapshots)
+        snap_iters = (ctypes.c_int32 * max(n_snap, 1))(*([int(k) for k in snapshots] if n_snap else [0]))
+        snaps = torch.empty((n_snap, self.n), dtype=torch.float64, device=b.device) if n_snap else None
         hip._check(lib.cc_pcg_sumfact(ctypes.addressof(self.desc), hip._ptr(b), hip._ptr(x), float(tol), int(maxiter),
                                       {"none": 0, "jacobi": 1}[preconditioner], int(check_every), hip._ptr(ws), nbytes,
                                       ctypes.addressof(res), hip._ptr(hist) if hist is not None else None,
+                                      snap_iters, n_snap, hip._ptr(snaps) if snaps is not None else None,
                                       hip._stream()), "pcg_sumfact")
+        out = (x, res)
         if history:
-            return x, res, hist.cpu().numpy()
-        return x, res
+            out += (hist.cpu().numpy(),)
+        if n_snap:
+            out += (snaps,)
+        return out

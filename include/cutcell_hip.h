@@ -67,8 +67,9 @@ int cc_device_count(int* count);
  * serves the benchmark's live roofline figure). When enabled, every launch of
  * a timed kernel class inside the solvers is bracketed by two hipEvents on its
  * stream; cc_profile_read returns the summed milliseconds and launch count of
- * a class: 0 = SpMV/SpMM of the CG loop, 1 = CG update, 2 = CG direction,
- * 3 = batched persistent solve. Process-wide and not thread-safe. */
+ * a class: 0 = SpMV/SpMM (or matrix-free apply) of the CG loops, 1 = CG
+ * update, 2 = CG direction, 3 = batched persistent solve, 4 = learned
+ * preconditioner apply. Process-wide and not thread-safe. */
 int cc_profile_enable(int32_t on);
 int cc_profile_reset(void);
 int cc_profile_read(int32_t which, double* total_ms, int64_t* count);
@@ -221,10 +222,15 @@ int cc_sumfact_apply(const cc_sumfact_desc* d, const double* x, double* y, void*
                      cc_stream_t stream);
 /* Jacobi-PCG (reference GPU-loop semantics, as cc_pcg_jacobi_f64 with k = 1)
  * on the matrix-free operator. Synchronises; result is host memory; history
- * as in cc_pcg_jacobi_f64 (may be NULL). */
+ * as in cc_pcg_jacobi_f64 (may be NULL). snapshots (device, [n_snap][n_red],
+ * may be NULL when n_snap = 0) receives the iterate x_j after snap_iters[j]
+ * (host array) iterations, or the final iterate where the solve stopped
+ * earlier: the (residual, error) pairs of the learned preconditioner's
+ * training set come from these iterates. */
 int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* b, double* x, double tol, int32_t maxiter,
                    int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
-                   cc_pcg_result* result, double* history, cc_stream_t stream);
+                   cc_pcg_result* result, double* history, const int32_t* snap_iters, int32_t n_snap,
+                   double* snapshots, cc_stream_t stream);
 
 /* ---- Learned preconditioner (no reference counterpart: the reference has
  * only Jacobi, code/fast_solver_backends.py:74-87 and :237-238). ---------

@@ -56,3 +56,17 @@ def test_pcg_matches_oracle(case):
     rel = np.linalg.norm(A @ x - b) / np.linalg.norm(b)
     assert res.true_rel == pytest.approx(rel, rel=1e-6)
     assert np.linalg.norm(x - x_or) / np.linalg.norm(x_or) < 1e-6
+
+
+def test_snapshots_are_the_capped_iterates(case):
+    """The iterates copied out during one solve equal the results of solves
+    capped at those iteration counts (the loop is deterministic), and a
+    snapshot past the stop holds the final iterate."""
+    name, mf, A, dmf = case
+    b = torch.from_numpy(np.random.default_rng(3).standard_normal(mf.n)).to("cuda")
+    x, res, snaps = dmf.pcg(b, tol=1e-8, maxiter=5000, snapshots=[2, 7, 20, 100000])
+    for j, k in enumerate([2, 7, 20]):
+        xk, rk = dmf.pcg(b, tol=0.0, maxiter=k)
+        assert rk.iterations == k
+        assert torch.equal(snaps[j], xk), (name, k)
+    assert torch.equal(snaps[3], x), name
