@@ -294,6 +294,9 @@ def main(argv=None) -> int:
     ap.add_argument("--learned-rhs", type=int, default=200, help="training right-hand sides (10 pairs each)")
     ap.add_argument("--learned-steps", type=int, default=400, help="Adam steps (8 pairs each)")
     ap.add_argument("--learned-eval", type=int, default=8, help="right-hand sides timed per method")
+    ap.add_argument("--copy-probe-mb", type=float, default=0.0,
+                    help="before timing, copy a buffer of this many MiB once (a known-size kernel that calibrates "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -322,6 +325,15 @@ def main(argv=None) -> int:
     Ad = hip.DeviceCSR.from_scipy(A, with_f32=True)
     Bd = torch.from_numpy(B).to("cuda")
     torch.cuda.synchronize()
+    if args.copy_probe_mb > 0:
+        words = int(args.copy_probe_mb * 2**20) // 8
+        src = torch.ones(words, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {words * 8} bytes read and written once")
+        del src, dst
 
     runs = {}
     for m in args.methods.split(","):

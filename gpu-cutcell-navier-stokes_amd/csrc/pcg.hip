@@ -103,6 +103,7 @@ struct SpmmConfig {
   int grid;    // blocks; 0 = min(kMaxParts, rows / teams per block)
   int unroll;  // 1 or 4 independent loads in flight per lane; 0 = 4
   int cpl;     // right-hand sides per lane (vector width); 0 = widest 16-byte load
+  int stream;  // 1: nontemporal matrix loads (unroll 4 only)
 };
 static SpmmConfig g_spmm_cfg[2][17] = {};
 
@@ -135,15 +136,25 @@ __device__ inline void block_col_sum(double v, double* out, double* smem) {
 }
 
 // Reduce partial[nparts][PAIRS] over the parts, in a fixed order, into
-// out[PAIRS] (shared memory). PAIRS must divide blockDim.x.
+// out[PAIRS] (shared memory). PAIRS must divide blockDim.x. Each thread keeps
+// four independent running sums (parts sub, sub + nsub, sub + 2 nsub,
+// sub + 3 nsub, then the next four ...) so four loads are in flight, and the
+// sums are combined in a fixed order: the result is bitwise reproducible.
 template <int PAIRS>
 __device__ inline void reduce_partials(const double* __restrict__ partial, int nparts, double* out, double* smem) {
   const int pair = threadIdx.x % PAIRS;
   const int sub = threadIdx.x / PAIRS;
   const int nsub = blockDim.x / PAIRS;
-  double s = 0.0;
-  for (int b = sub; b < nparts; b += nsub) s += partial[(size_t)b * PAIRS + pair];
-  smem[threadIdx.x] = s;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int b = sub;
+  for (; b + 3 * nsub < nparts; b += 4 * nsub) {
+    s0 += partial[(size_t)b * PAIRS + pair];
+    s1 += partial[(size_t)(b + nsub) * PAIRS + pair];
+    s2 += partial[(size_t)(b + 2 * nsub) * PAIRS + pair];
+    s3 += partial[(size_t)(b + 3 * nsub) * PAIRS + pair];
+  }
+  for (; b < nparts; b += nsub) s0 += partial[(size_t)b * PAIRS + pair];
+  smem[threadIdx.x] = (s0 + s1) + (s2 + s3);
   __syncthreads();
   if (threadIdx.x < PAIRS) {
     double t = 0.0;
@@ -151,6 +162,90 @@ __device__ inline void reduce_partials(const double* __restrict__ partial, int n
     out[threadIdx.x] = t;
   }
   __syncthreads();
+}
+
+// Reduce partial[nparts][PAIRS] (nparts <= kMaxParts) into out[PAIRS] with a
+// block of NT threads: every thread issues all of its kMaxParts * PAIRS / NT
+// loads at once (one round of memory latency), sums them pairwise, and the
+// block combines the per-thread sums by a tree in LDS. The order is fixed, so
+// the result is bitwise reproducible. smem holds NT doubles.
+template <int PAIRS, int NT>
+__device__ inline void reduce_partials_wide(const double* __restrict__ partial, int nparts, double* out,
+                                            double* smem) {
+  static_assert(NT % PAIRS == 0 && (kMaxParts * PAIRS) % NT == 0, "shape");
+  constexpr int PER = kMaxParts * PAIRS / NT;
+  const int total = nparts * PAIRS;
+  double v[PER];
+#pragma unroll
+  for (int j = 0; j < PER; ++j) {
+    const int e = threadIdx.x + j * NT;
+    v[j] = e < total ? partial[e] : 0.0;
+  }
+#pragma unroll
+  for (int w = 1; w < PER; w <<= 1)
+#pragma unroll
+    for (int j = 0; j + w < PER; j += 2 * w) v[j] += v[j + w];
+  // Element e = t + j NT belongs to pair e % PAIRS = t % PAIRS.
+  smem[threadIdx.x] = v[0];
+  __syncthreads();
+  for (int w = NT / 2; w >= PAIRS; w >>= 1) {
+    if (threadIdx.x < w) smem[threadIdx.x] += smem[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < PAIRS) out[threadIdx.x] = smem[threadIdx.x];
+  __syncthreads();
+}
+
+constexpr int kFinThreads = 1024;
+
+// alpha = rho / p.q; breakdown when p.q <= 0 or non-finite.
+template <int K>
+__device__ inline void finalize_alpha(const double* __restrict__ partial, int nparts, ColState* st, int iter) {
+  __shared__ double smem[kFinThreads];
+  __shared__ double den[K];
+  reduce_partials_wide<K, kFinThreads>(partial, nparts, den, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    if (st[c].status == CC_RUNNING) {
+      const double dn = den[c];
+      if (!(dn > 0.0 && isfinite(dn))) {
+        st[c].status = CC_BREAKDOWN;
+        st[c].iterations = iter;
+        st[c].alpha = 0.0;
+      } else {
+        st[c].alpha = st[c].rho / dn;
+      }
+    }
+  }
+}
+
+// rel = ||r||/||b||; stop when rel <= tol (or iter == maxiter); otherwise
+// beta = (r.z)_new / rho and rho = (r.z)_new.
+template <int K>
+__device__ inline void finalize_beta(const double* __restrict__ partial, int nparts, ColState* st, int iter,
+                                     double tol, int maxiter, double* hist) {
+  __shared__ double smem[kFinThreads];
+  __shared__ double sums[2 * K];
+  reduce_partials_wide<2 * K, kFinThreads>(partial, nparts, sums, smem);
+  if (threadIdx.x < K) {
+    const int c = threadIdx.x;
+    if (st[c].status == CC_RUNNING) {
+      const double rr = sums[c];
+      const double rz = sums[K + c];
+      const double rel = sqrt(rr) / st[c].bnorm;
+      st[c].rel = rel;
+      st[c].iterations = iter;
+      if (hist) hist[(size_t)iter * K + c] = rel;
+      if (rel <= tol) {
+        st[c].status = CC_CONVERGED;
+      } else if (iter >= maxiter) {
+        st[c].status = CC_MAXITER;
+      } else {
+        st[c].beta = rz / st[c].rho;
+        st[c].rho = rz;
+      }
+    }
+  }
 }
 
 template <int K>
@@ -237,7 +332,17 @@ __device__ inline void vstore(VT* __restrict__ p, const VT* in) {
 // MODE 1: Y = B - A X, partial[block][c] = sum_rows Y.Y  (true residual)
 // A row team has KL = K/CPL lanes per nonzero slot (each lane owns CPL
 // consecutive right-hand sides, loaded as one vector) and NZ slots.
-template <typename MT, typename VT, int K, int NZ, int MODE, int UNR, int CPL>
+template <bool NTL, typename T>
+__device__ inline T stream_ld(const T* p) {
+  if constexpr (NTL)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+
+// NTL: load the matrix (read once per product) with nontemporal loads so it
+// does not evict the gathered X rows from L2; chosen by the autotuner.
+template <typename MT, typename VT, int K, int NZ, int MODE, int UNR, int CPL, bool NTL>
 __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ indptr, const int* __restrict__ indices,
                                               const MT* __restrict__ vals, const VT* __restrict__ X,
                                               VT* __restrict__ Y, const double* __restrict__ B,
@@ -279,8 +384,10 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
 #pragma unroll
       for (int m = 0; m < CPL; ++m) a1[m] = a2[m] = a3[m] = 0;
       for (; j + 3 * NZ < len; j += 4 * NZ) {
-        const int q0 = ixp[j], q1 = ixp[j + NZ], q2 = ixp[j + 2 * NZ], q3 = ixp[j + 3 * NZ];
-        const AT v0 = (AT)vap[j], v1 = (AT)vap[j + NZ], v2 = (AT)vap[j + 2 * NZ], v3 = (AT)vap[j + 3 * NZ];
+        const int q0 = stream_ld<NTL>(ixp + j), q1 = stream_ld<NTL>(ixp + j + NZ);
+        const int q2 = stream_ld<NTL>(ixp + j + 2 * NZ), q3 = stream_ld<NTL>(ixp + j + 3 * NZ);
+        const AT v0 = (AT)stream_ld<NTL>(vap + j), v1 = (AT)stream_ld<NTL>(vap + j + NZ);
+        const AT v2 = (AT)stream_ld<NTL>(vap + j + 2 * NZ), v3 = (AT)stream_ld<NTL>(vap + j + 3 * NZ);
         VT x0[CPL], x1[CPL], x2[CPL], x3[CPL];
         vload<VT, CPL>(X + (size_t)q0 * K + c0, x0);
         vload<VT, CPL>(X + (size_t)q1 * K + c0, x1);
@@ -298,9 +405,9 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
       for (int m = 0; m < CPL; ++m) acc[m] = (acc[m] + a1[m]) + (a2[m] + a3[m]);
     }
     for (; j < len; j += NZ) {
-      const AT v = (AT)vap[j];
+      const AT v = (AT)stream_ld<NTL>(vap + j);
       VT xv[CPL];
-      vload<VT, CPL>(X + (size_t)ixp[j] * K + c0, xv);
+      vload<VT, CPL>(X + (size_t)stream_ld<NTL>(ixp + j) * K + c0, xv);
 #pragma unroll
       for (int m = 0; m < CPL; ++m) acc[m] += v * (AT)xv[m];
     }
@@ -405,27 +512,12 @@ __global__ __launch_bounds__(256) void k_cg_setup_finish(const double* __restric
   }
 }
 
-// F1: alpha = rho / p.q; breakdown when p.q <= 0 or non-finite.
+// F1 (one block of kFinThreads).
 template <int K>
-__global__ __launch_bounds__(256) void k_cg_alpha(const double* __restrict__ partial, int nparts,
-                                                  ColState* __restrict__ st, int iter) {
-  __shared__ double smem[256];
-  __shared__ double den[K];
+__global__ __launch_bounds__(kFinThreads) void k_cg_alpha(const double* __restrict__ partial, int nparts,
+                                                          ColState* __restrict__ st, int iter) {
   if (all_stopped<K>(st)) return;
-  reduce_partials<K>(partial, nparts, den, smem);
-  if (threadIdx.x < K) {
-    const int c = threadIdx.x;
-    if (st[c].status == CC_RUNNING) {
-      const double dn = den[c];
-      if (!(dn > 0.0 && isfinite(dn))) {
-        st[c].status = CC_BREAKDOWN;
-        st[c].iterations = iter;
-        st[c].alpha = 0.0;
-      } else {
-        st[c].alpha = st[c].rho / dn;
-      }
-    }
-  }
+  finalize_alpha<K>(partial, nparts, st, iter);
 }
 
 // U: x += alpha p, r -= alpha q, z = r/d; partials (r.r, r.z).
@@ -459,35 +551,13 @@ __global__ __launch_bounds__(256) void k_cg_update(int n, VT* __restrict__ X, VT
   block_col_sum<K>(rz, partial_out + ((size_t)blockIdx.x * 2 + 1) * K, smem);
 }
 
-// F2: rel = ||r||/||b||; stop when rel <= tol (or iter == maxiter); otherwise
-// beta = (r.z)_new / rho and rho = (r.z)_new.
+// F2 (one block of kFinThreads).
 template <int K>
-__global__ __launch_bounds__(256) void k_cg_beta(const double* __restrict__ partial, int nparts,
-                                                 ColState* __restrict__ st, int iter, double tol, int maxiter,
-                                                 double* __restrict__ hist) {
-  __shared__ double smem[256];
-  __shared__ double sums[2 * K];
+__global__ __launch_bounds__(kFinThreads) void k_cg_beta(const double* __restrict__ partial, int nparts,
+                                                         ColState* __restrict__ st, int iter, double tol, int maxiter,
+                                                         double* __restrict__ hist) {
   if (all_stopped<K>(st)) return;
-  reduce_partials<2 * K>(partial, nparts, sums, smem);
-  if (threadIdx.x < K) {
-    const int c = threadIdx.x;
-    if (st[c].status == CC_RUNNING) {
-      const double rr = sums[c];
-      const double rz = sums[K + c];
-      const double rel = sqrt(rr) / st[c].bnorm;
-      st[c].rel = rel;
-      st[c].iterations = iter;
-      if (hist) hist[(size_t)iter * K + c] = rel;
-      if (rel <= tol) {
-        st[c].status = CC_CONVERGED;
-      } else if (iter >= maxiter) {
-        st[c].status = CC_MAXITER;
-      } else {
-        st[c].beta = rz / st[c].rho;
-        st[c].rho = rz;
-      }
-    }
-  }
+  finalize_beta<K>(partial, nparts, st, iter, tol, maxiter, hist);
 }
 
 // P: p = z + beta p for the columns still running.
@@ -599,9 +669,9 @@ static int default_nz(double avg_row) {
 }
 
 template <typename MT, typename VT, int K, int NZ, int MODE, int CPL>
-static int spmm_launch_nz(int n, int unroll, int grid_req, const int* indptr, const int* indices, const MT* vals,
-                          const VT* X, VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s,
-                          int* grid_out) {
+static int spmm_launch_nz(int n, int unroll, int grid_req, int stream, const int* indptr, const int* indices,
+                          const MT* vals, const VT* X, VT* Y, const double* B, double* partial, const ColState* st,
+                          hipStream_t s, int* grid_out) {
   constexpr int T = (K / CPL) * NZ;
   const int teams_per_block = (kBlock / 64) * (64 / T);
   int grid = grid_req;
@@ -609,12 +679,15 @@ static int spmm_launch_nz(int n, int unroll, int grid_req, const int* indptr, co
   if (grid > kMaxParts) grid = kMaxParts;
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
-  if (unroll >= 4)
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals,
-                       X, Y, B, partial, st);
+  if (unroll >= 4 && stream)
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
+                       vals, X, Y, B, partial, st);
+  else if (unroll >= 4)
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
+                       indices, vals, X, Y, B, partial, st);
   else
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1, CPL>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices, vals,
-                       X, Y, B, partial, st);
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
+                       indices, vals, X, Y, B, partial, st);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
@@ -630,8 +703,9 @@ static int spmm_launch_cpl(int n, double avg_row, const int* indptr, const int* 
   const int grid = cfg.grid;
 #define CC_NZ(NZV)                                                                                                  \
   if (nz == NZV && KL * NZV <= 64)                                                                                  \
-    return spmm_launch_nz<MT, VT, K, (KL * NZV <= 64 ? NZV : 1), MODE, CPL>(n, unroll, grid, indptr, indices, vals, \
-                                                                          X, Y, B, partial, st, s, grid_out);
+    return spmm_launch_nz<MT, VT, K, (KL * NZV <= 64 ? NZV : 1), MODE, CPL>(n, unroll, grid, cfg.stream, indptr,    \
+                                                                          indices, vals, X, Y, B, partial, st, s, \
+                                                                          grid_out);
   CC_NZ(64)
   CC_NZ(32)
   CC_NZ(16)
@@ -717,9 +791,9 @@ static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external) {
 
 // The device-resident CG loop for K columns over any operator:
 // apply(P, Q, partial, &nparts) launches Q = A P and writes the per-block
-// partials of P.Q. src is the float64 right-hand side block; inv_scale
-// (device, may be null) scales it into VT. The per-column state is left in
-// b.st (and copied to host_st).
+// partials of P.Q. src is the float64 right-hand side block; inv_scale (device, may be
+// null) scales it into VT. The per-column state is left in b.st (and copied
+// to host_st).
 // after_iter (may be null) is called with the iteration number once that
 // iteration's launches are enqueued (it may enqueue copies of b.X).
 template <typename VT, int K, typename ApplyFn>
@@ -739,12 +813,12 @@ static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_
     prof_begin(kProfSpmm, s);
     CC_CHECK(apply(b.P, b.Q, b.pa, &gs));
     prof_end(s);
-    hipLaunchKernelGGL((k_cg_alpha<K>), dim3(1), dim3(kBlock), 0, s, b.pa, gs, b.st, it);
+    hipLaunchKernelGGL((k_cg_alpha<K>), dim3(1), dim3(kFinThreads), 0, s, b.pa, gs, b.st, it);
     prof_begin(kProfUpdate, s);
     hipLaunchKernelGGL((k_cg_update<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.X, b.R, b.Z, b.P, b.Q, b.diag,
                        precond, b.st, b.pb);
     prof_end(s);
-    hipLaunchKernelGGL((k_cg_beta<K>), dim3(1), dim3(kBlock), 0, s, b.pb, sg, b.st, it, tol, maxiter, hist);
+    hipLaunchKernelGGL((k_cg_beta<K>), dim3(1), dim3(kFinThreads), 0, s, b.pb, sg, b.st, it, tol, maxiter, hist);
     prof_begin(kProfDirection, s);
     hipLaunchKernelGGL((k_cg_direction<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.P, b.Z, b.st);
     prof_end(s);
@@ -1003,7 +1077,7 @@ int cc_profile_collect(void) {
 }
 
 int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
-                       int32_t cols_per_lane) {
+                       int32_t cols_per_lane, int32_t stream_loads) {
   if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
   if (value_bytes != 4 && value_bytes != 8) return CC_E_ARG;
   if (cols_per_lane * value_bytes > 16) return CC_E_ARG;
@@ -1012,7 +1086,8 @@ int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t
   const int kl = k / (cols_per_lane > 0 ? cols_per_lane : 1);
   if (nz_lanes < 0 || kl * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
   if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4)) return CC_E_ARG;
-  g_spmm_cfg[value_bytes == 4 ? 1 : 0][k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane};
+  if (stream_loads != 0 && (stream_loads != 1 || unroll == 1)) return CC_E_ARG;
+  g_spmm_cfg[value_bytes == 4 ? 1 : 0][k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane, stream_loads};
   return 0;
 }
 

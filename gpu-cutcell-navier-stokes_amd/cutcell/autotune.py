@@ -89,7 +89,7 @@ def _time_ms(fn, reps: int) -> float:
 
 def spmm_candidates(k: int, mean_row: float, value_bytes: int = 8):
     """Launch shapes: vector width (right-hand sides per lane), nonzero slots
-    per row team, grid and unroll depth."""
+    per row team, grid, unroll depth and nontemporal matrix loads."""
     cpls = [c for c in (1, 2, 4) if c <= k and c * value_bytes <= 16]
     for cpl in cpls:
         kl = k // cpl
@@ -97,8 +97,8 @@ def spmm_candidates(k: int, mean_row: float, value_bytes: int = 8):
         nzs = nzs[:2] if nzs else [max(1, 64 // kl)]
         for nz in nzs:
             for grid in (512, 1024):
-                for unroll in (1, 4):
-                    yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll)
+                for unroll, stream in ((1, 0), (4, 0), (4, 1)):
+                    yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll, stream=stream)
 
 
 def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool = False) -> dict:
@@ -108,11 +108,11 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     lib = hip.require_gpu()
     mean_row = A.nnz / max(A.n, 1)
     vb = 8 if precision == "fp64" else 4
-    key = f"spmm|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
+    key = f"spmm2|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
     table = _load()
     if key in table and not force:
         cfg = table[key]
-        lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
+        lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"], cfg.get("stream", 0))
         return cfg
     dt = torch.float64 if precision == "fp64" else torch.float32
     X = torch.randn(A.n, k, dtype=dt, device=A.vals.device)
@@ -122,7 +122,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     fn = getattr(lib, fn_name)
     best = None
     for cand in spmm_candidates(k, mean_row, vb):
-        if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) != 0:
+        if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"], cand["stream"]) != 0:
             continue
         call = lambda: fn(A.n, k, hip._ptr(A.indptr), hip._ptr(A.indices), hip._ptr(vals), hip._ptr(X), hip._ptr(Y),
                           hip._stream())
@@ -130,7 +130,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
         if best is None or ms < best[0]:
             best = (ms, cand)
     cfg = dict(best[1], ms=best[0])
-    lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
+    lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"], cfg["stream"])
     _store(key, cfg)
     return cfg
 
@@ -139,7 +139,7 @@ def reset_spmm_config() -> None:
     lib = hip.load_library()
     for k in hip.SUPPORTED_K:
         for vb in (8, 4):
-            lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0)
+            lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0)
 
 
 # ---------------------------------------------------------------------------

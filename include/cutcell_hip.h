@@ -84,9 +84,10 @@ int cc_profile_collect(void);
  * lane, loaded as one vector (1, 2 or 4 with at most 16 bytes; 0 = widest),
  * nz_lanes nonzero slots per row team ((k/cols_per_lane)*nz_lanes <= 64,
  * power of two; 0 = from the mean row length), grid blocks (0 = one block per
- * few rows up to 1024), unroll 1 or 4 (0 = 4). */
+ * few rows up to 1024), unroll 1 or 4 (0 = 4), stream_loads 1 = nontemporal
+ * loads of the matrix arrays (not with unroll 1), 0 = ordinary loads. */
 int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
-                       int32_t cols_per_lane);
+                       int32_t cols_per_lane, int32_t stream_loads);
 
 /* y = A x, float64 (replaces the cuSPARSE product A_gpu @ p). */
 int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
