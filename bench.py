@@ -210,12 +210,98 @@ def build_sphere_operator(seed: int, n_elements: int, degree: int = 3):
 SNAPSHOTS = (2, 5, 10, 20, 40, 70, 100, 150, 200, 300)
 
 
+def build_channel_operator():
+    """The bump-channel pressure operator (cached per machine)."""
+    from cutcell.workloads import c5_operator
+
+    path = _cache_dir() / "channel3d_c5_v1.pkl"
+    if path.exists():
+        with path.open("rb") as f:
+            return pickle.load(f)
+    t0 = time.perf_counter()
+    mf = c5_operator(mass_coef=0.0)
+    mf.meta["build_s"] = time.perf_counter() - t0
+    with path.open("wb") as f:
+        pickle.dump(mf, f, protocol=pickle.HIGHEST_PROTOCOL)
+    return mf
+
+
+def time_solver(hip, torch, solve, B):
+    """Solve each right-hand side of B in turn; solves/s and per-solve figures."""
+    solve(B[0])  # warm-up
+    torch.cuda.synchronize()
+    hip.profile_enable(True)
+    t0 = time.perf_counter()
+    res = [solve(b)[1] for b in B]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    apply_ms, apply_cnt = hip.profile_read("spmm")
+    prec_ms, prec_cnt = hip.profile_read("learned")
+    hip.profile_enable(False)
+    iters = [int(r.iterations) for r in res]
+    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
+    return {"solves_per_s": sig(len(B) / dt), "ms_per_solve": sig(1e3 * dt / len(B)), "converged": ok,
+            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
+            "true_rel_max": sig(max(float(r.true_rel) for r in res)),
+            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1)),
+            "learned_prec_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None}
+
+
+def pressure_section(hip, torch, args):
+    """Pressure projection on the bump channel: the singular E^T K E
+    (periodic in x and z), a stream of zero-sum right-hand sides (the
+    compatibility condition a discrete divergence satisfies), matrix-free
+    Jacobi-PCG against flexible CG with the multilevel learned preconditioner
+    trained on the solver's own iterates of this operator."""
+    from cutcell.learned import MultilevelCNN, generate_pairs
+    from cutcell.matfree import DeviceMatrixFree
+
+    t0 = time.perf_counter()
+    mf = build_channel_operator()
+    build_s = time.perf_counter() - t0
+    dmf = DeviceMatrixFree(mf)
+    n = mf.n
+    log(f"pressure: bump channel {mf.meta['shape']}, n={n}, {mf.meta['n_cut_cells']} cut cells ({build_s:.1f} s)")
+    rng = np.random.default_rng(3000 + args.seed)
+
+    def rhs():
+        b = rng.standard_normal(n)
+        return torch.from_numpy(b - b.mean()).to("cuda")
+
+    out = {"workload": "channel 128x64x64 elements, p=3, periodic in x and z (9 x 3.036 x 4.5), lower wall cut by "
+                       "a cosine bump (height 1, length 4); pressure operator E^T K E, matrix-free; zero-sum "
+                       "Gaussian right-hand sides",
+           "n": n, "n_cut_cells": mf.meta["n_cut_cells"], "min_cut_fraction": sig(mf.meta["min_cut_fraction"]),
+           "build_s": sig(build_s), "stream_timed": args.c5_eval}
+    B = [rhs() for _ in range(args.c5_eval)]
+    out["jacobi"] = time_solver(hip, torch, lambda b: dmf.pcg(b, tol=TOL, maxiter=20000), B)
+    log(f"pressure, jacobi: {out['jacobi']}")
+    t0 = time.perf_counter()
+    R, E = generate_pairs(dmf, [rhs() for _ in range(args.c5_pair_rhs)], (5, 20, 60, 150, 400))
+    torch.cuda.synchronize()
+    pairs_s = time.perf_counter() - t0
+    diag = torch.from_numpy(mf.diag).to("cuda")
+    periodic = tuple(a.periodic for a in mf.grid.axes)
+    mlnet = MultilevelCNN(mf.grid.nb, mf.interior, periodic=periodic, levels=args.ml_levels, seed=args.seed)
+    rep = mlnet.train(R, E, diag, steps=args.learned_steps, batch=8, lr=args.learned_lr)
+    out["training_multilevel"] = {"pairs": int(R.shape[0]), "pairs_s": sig(pairs_s),
+                                  "grids": [list(d) for d in mlnet.dims], "net_levels": mlnet.net_levels,
+                                  "steps": rep.steps, "seconds": sig(rep.seconds), "loss_first": sig(rep.loss_first),
+                                  "loss_last20": sig(float(np.mean(rep.history[-20:])))}
+    del R, E
+    torch.cuda.empty_cache()
+    out["learned_multilevel"] = time_solver(hip, torch, lambda b: mlnet.fcg(dmf, b, tol=TOL, maxiter=20000), B)
+    log(f"pressure, learned multilevel: {out['learned_multilevel']}")
+    out["speedup_solves_per_s"] = sig(out["learned_multilevel"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
+    return out
+
+
 def learned_section(hip, torch, args):
     """Learned preconditioner on the sphere-obstacle box: pairs from the
     solver's own iterates, GPU training, then flexible CG with the network
     against Jacobi-PCG on the same matrix-free operator, one right-hand side
     per solve, to relative residual 1e-8 (float64 true residual judged)."""
-    from cutcell.learned import MaskedCNN, generate_pairs, solve_learned_with_fallback
+    from cutcell.learned import MultilevelCNN, generate_pairs, solve_learned_with_fallback
     from cutcell.matfree import DeviceMatrixFree
 
     t0 = time.perf_counter()
@@ -234,10 +320,12 @@ def learned_section(hip, torch, args):
     del rhs
     log(f"learned: {R.shape[0]} (residual, error) pairs in {pairs_s:.1f} s")
     diag = torch.from_numpy(mf.diag).to("cuda")
-    net = MaskedCNN(mf.grid.nb, mf.interior, periodic=tuple(a.periodic for a in mf.grid.axes), seed=args.seed)
-    rep = net.train(R, E, diag, steps=args.learned_steps, batch=8, lr=1e-3)
-    log(f"learned: trained {rep.steps} Adam steps x {rep.batch} pairs in {rep.seconds:.1f} s, "
-        f"loss {rep.loss_first:.3f} -> {np.mean(rep.history[-20:]):.3f}")
+    periodic = tuple(a.periodic for a in mf.grid.axes)
+    net = MultilevelCNN(mf.grid.nb, mf.interior, periodic=periodic, levels=args.ml_levels, seed=args.seed)
+    rep = net.train(R, E, diag, steps=args.learned_steps, batch=8, lr=args.learned_lr)
+    log(f"learned: network on {net.levels} grids {net.dims} (nets on {net.net_levels}) trained {rep.steps} Adam "
+        f"steps x {rep.batch} pairs in {rep.seconds:.1f} s, loss {rep.loss_first:.3f} -> "
+        f"{np.mean(rep.history[-20:]):.3f}")
     del R, E
     torch.cuda.empty_cache()
     B = [torch.from_numpy(rng.standard_normal(n)).to("cuda") for _ in range(args.learned_eval)]
@@ -247,28 +335,15 @@ def learned_section(hip, torch, args):
            "n": n, "n_cut_cells": mf.meta["n_cut_cells"], "min_cut_fraction": sig(mf.meta["min_cut_fraction"]),
            "build_s": sig(build_s), "pairs": {"count": int(pairs_n), "rhs": args.learned_rhs,
                                               "seconds": sig(pairs_s), "snapshots": list(SNAPSHOTS)},
-           "training": {"steps": rep.steps, "batch": rep.batch, "seconds": sig(rep.seconds),
+           "training": {"grids": [list(d) for d in net.dims], "net_levels": net.net_levels, "steps": rep.steps,
+                        "batch": rep.batch, "lr": args.learned_lr, "seconds": sig(rep.seconds),
                         "loss_first": sig(rep.loss_first), "loss_last20": sig(float(np.mean(rep.history[-20:])))}}
-    for method in ("jacobi", "learned"):
-        solve = (lambda b: dmf.pcg(b, tol=TOL, maxiter=20000)) if method == "jacobi" else \
-                (lambda b: net.fcg(dmf, b, tol=TOL, maxiter=20000))
-        solve(B[0])  # warm-up
-        torch.cuda.synchronize()
-        hip.profile_enable(True)
-        t0 = time.perf_counter()
-        res = [solve(b)[1] for b in B]
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        apply_ms, apply_cnt = hip.profile_read("spmm")
-        prec_ms, prec_cnt = hip.profile_read("learned")
-        hip.profile_enable(False)
-        iters = [int(r.iterations) for r in res]
-        ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
-        out[method] = {"solves_per_s": sig(len(B) / dt), "ms_per_solve": sig(1e3 * dt / len(B)), "converged": ok,
-                       "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
-                       "true_rel_max": sig(max(float(r.true_rel) for r in res)),
-                       "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1)),
-                       "learned_prec_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None}
+    solvers = {
+        "jacobi": lambda b: dmf.pcg(b, tol=TOL, maxiter=20000),
+        "learned": lambda b: net.fcg(dmf, b, tol=TOL, maxiter=20000),
+    }
+    for method, solve in solvers.items():
+        out[method] = time_solver(hip, torch, solve, B)
         log(f"learned section, {method}: {out[method]}")
     _, res, how = solve_learned_with_fallback(net, dmf, B[0], tol=TOL)
     out["fallback_check"] = {"path": how, "converged": res.status == hip.CC_CONVERGED}
@@ -292,8 +367,13 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner section")
     ap.add_argument("--learned-n", type=int, default=64, help="background elements per axis of the sphere box")
     ap.add_argument("--learned-rhs", type=int, default=200, help="training right-hand sides (10 pairs each)")
-    ap.add_argument("--learned-steps", type=int, default=400, help="Adam steps (8 pairs each)")
+    ap.add_argument("--learned-steps", type=int, default=1000, help="Adam steps (8 pairs each)")
+    ap.add_argument("--learned-lr", type=float, default=2e-3, help="Adam learning rate")
     ap.add_argument("--learned-eval", type=int, default=8, help="right-hand sides timed per method")
+    ap.add_argument("--ml-levels", type=int, default=5, help="grids of the multilevel learned preconditioner")
+    ap.add_argument("--no-pressure", action="store_true", help="skip the bump-channel pressure-projection section")
+    ap.add_argument("--c5-eval", type=int, default=8, help="right-hand sides of the pressure stream timed per method")
+    ap.add_argument("--c5-pair-rhs", type=int, default=30, help="pressure right-hand sides that give training pairs")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="before timing, copy a buffer of this many MiB once (a known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
@@ -373,6 +453,11 @@ def main(argv=None) -> int:
         log("learned-preconditioner section: sphere-obstacle box, matrix-free operator ...")
         learned = learned_section(hip, torch, args)
 
+    pressure = None
+    if not args.no_pressure:
+        log("pressure-projection section: bump channel, matrix-free E^T K E ...")
+        pressure = pressure_section(hip, torch, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle C port, 1 core, 1 RHS) ...")
@@ -411,6 +496,7 @@ def main(argv=None) -> int:
         "cpu_baseline": cpu,
         "batched_uneven_2d": batched,
         "learned_sphere_3d": learned,
+        "pressure_channel_3d": pressure,
     }
     print(json.dumps(out), flush=True)
     return 0

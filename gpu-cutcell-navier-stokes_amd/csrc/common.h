@@ -55,6 +55,15 @@ struct Accum {
   using type = T;
 };
 
+// One axis of a grid coarsening (mg.hip): fine and coarse point counts and
+// whether the axis is periodic.
+struct MgAxis {
+  int nf, nc, per;
+};
+struct MgDims {
+  MgAxis a[3];
+};
+
 }  // namespace cc
 
 #include <functional>
@@ -83,6 +92,17 @@ int fcg_operator(int n, const std::function<int(const double*, double*, double*,
                  int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
                  const std::function<int(const double*, double*, double*)>& residual);
 size_t fcg_operator_bytes(size_t n, size_t voxels);
+}  // namespace cc
+
+struct cc_mlcnn_desc;
+namespace cc {
+// Flexible CG with the multilevel learned preconditioner (fcg.hip).
+int fcg_operator_ml(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                    const cc_mlcnn_desc& ml, const double* diag, const double* bvec, double* x, double tol,
+                    int maxiter, int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist,
+                    hipStream_t s, const std::function<int(const double*, double*, double*)>& residual);
+size_t fcg_operator_ml_bytes(size_t n, const cc_mlcnn_desc& ml);
+bool ml_desc_ok(const cc_mlcnn_desc& ml);
 }  // namespace cc
 
 #define CC_CHECK(expr)             \

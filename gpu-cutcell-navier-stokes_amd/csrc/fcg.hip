@@ -14,6 +14,14 @@
 // Jacobi-PCG loop (rel = ||r||/||b|| <= tol; p.q <= 0 or non-finite stops),
 // the float64 true residual is recomputed at exit, and the caller falls back
 // to Jacobi-PCG when the learned solve does not converge.
+//
+// Multilevel form (cc_mlcnn_desc): the background grid and its coarsenings
+// (every other grid point per level, mg.hip) each may carry such a network,
+// and the correction is summed over the levels through the restriction R and
+// the prolongation P = R^T:
+//   z = D^-1 r + S sum_l Q_l C_l^T C_l Q_l^T S r,  Q_l = P_1 ... P_l,
+// still symmetric and >= D^-1. The coarse levels reach the smooth error
+// that a 3x3x3 stack on the fine grid cannot.
 
 #include <hip/hip_runtime.h>
 
@@ -28,6 +36,11 @@ namespace cc {
 
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
                        float* out_f32, hipStream_t s);
+
+int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s);
+int mg_prolong(const MgDims& g, const float* coarse, int cs, const float* add, int as, float* fine, hipStream_t s);
+int mg_to_ch0(long long nvox, const float* src, int ss, uint16_t* grid, hipStream_t s);
+bool mg_level_dims(const cc_cnn_desc& fine, const cc_cnn_desc& coarse, MgDims* g);
 
 struct FcgState {
   double bnorm, rel, alpha, beta, pq, pr, zq;
@@ -90,18 +103,27 @@ __global__ __launch_bounds__(256) void k_prec_in(int n, const int* __restrict__ 
     grid[(size_t)red2full[j] * 16] = f2bf_d((float)(sym_scale(diag[j]) * r[j]));
 }
 
-// z = D^-1 r + S y (y = channel 0 of the last layer), partial z.q (q may be
-// null).
+// S r written as a single-channel float32 grid at the unknowns' voxels.
+__global__ __launch_bounds__(256) void k_prec_in_f32(int n, const int* __restrict__ red2full,
+                                                     const double* __restrict__ r, const double* __restrict__ diag,
+                                                     float* __restrict__ u0, const FcgState* __restrict__ st) {
+  if (st && st->status != CC_RUNNING) return;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    u0[red2full[j]] = (float)(sym_scale(diag[j]) * r[j]);
+}
+
+// z = D^-1 r + S y (y[voxel * ys]: channel 0 of the last layer, ys = 16, or
+// a single-channel grid, ys = 1), partial z.q (q may be null).
 __global__ __launch_bounds__(256) void k_prec_out(int n, const int* __restrict__ red2full, const double* __restrict__ r,
                                                   const double* __restrict__ diag, const float* __restrict__ y,
-                                                  double* __restrict__ z, const double* __restrict__ q,
+                                                  int ys, double* __restrict__ z, const double* __restrict__ q,
                                                   double* __restrict__ part, const FcgState* __restrict__ st) {
   __shared__ double red[4];
   if (st && st->status != CC_RUNNING) return;
   double zq = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
     const double d = diag[j];
-    const double zz = jacobi_part(r[j], d) + sym_scale(d) * (double)y[(size_t)red2full[j] * 16];
+    const double zz = jacobi_part(r[j], d) + sym_scale(d) * (double)y[(size_t)red2full[j] * ys];
     z[j] = zz;
     if (q) zq += zz * q[j];
   }
@@ -290,24 +312,23 @@ static int learned_prec(const cc_cnn_desc& net, const FcgBuffers& b, const doubl
   const int n = net.n_red;
   hipLaunchKernelGGL(k_prec_in, dim3(g1(n)), dim3(256), 0, s, n, net.red2full, r, diag, b.a0, b.st);
   CC_CHECK(cnn_forward_layers(net, b.a0, b.a1, b.a2, b.a3, b.y, s));
-  hipLaunchKernelGGL(k_prec_out, dim3(g1(n)), dim3(256), 0, s, n, net.red2full, r, diag, b.y, b.z, q, part, b.st);
+  hipLaunchKernelGGL(k_prec_out, dim3(g1(n)), dim3(256), 0, s, n, net.red2full, r, diag, b.y, 16, b.z, q, part,
+                     b.st);
   return (int)hipGetLastError();
 }
 
-int fcg_operator(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
-                 const cc_cnn_desc& net, const double* diag, const double* bvec, double* x, double tol, int maxiter,
-                 int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
-                 const std::function<int(const double*, double*, double*)>& residual) {
-  const size_t voxels = (size_t)net.nx * net.ny * net.nz;
-  if (wsb < fcg_bytes(n, voxels)) return CC_E_WORKSPACE;
-  FcgBuffers b = carve_fcg((char*)ws, n, voxels);
-  // Grid tensors start at zero; only channel 0 at the unknowns' voxels is
-  // ever rewritten afterwards.
-  CC_CHECK(hipMemsetAsync(b.a0, 0, voxels * 16 * sizeof(uint16_t), s));
+// The preconditioner of the loop: writes b.z = M(r) and, when q is given,
+// the per-block partials of z.q into part.
+using PrecFn = std::function<int(const double* r, const double* q, double* part)>;
+
+static int fcg_loop(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                    const PrecFn& prec, const FcgBuffers& b, const double* bvec, double* x, double tol, int maxiter,
+                    int check_every, cc_pcg_result* result, double* hist, hipStream_t s,
+                    const std::function<int(const double*, double*, double*)>& residual) {
   const int gs = g1(n);
   hipLaunchKernelGGL(k_fcg_setup, dim3(gs), dim3(256), 0, s, n, bvec, x, b.r, b.pa);
   hipLaunchKernelGGL(k_fcg_init, dim3(1), dim3(256), 0, s, b.pa, gs, b.st, hist);
-  CC_CHECK(learned_prec(net, b, b.r, diag, nullptr, nullptr, s));
+  CC_CHECK(prec(b.r, nullptr, nullptr));
   hipLaunchKernelGGL(k_fcg_dir, dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 1);
   CC_CHECK(hipGetLastError());
   FcgState hs{};
@@ -321,7 +342,7 @@ int fcg_operator(int n, const std::function<int(const double*, double*, double*,
     hipLaunchKernelGGL(k_fcg_update, dim3(gs), dim3(256), 0, s, n, x, b.r, b.p, b.q, b.pc, b.st);
     hipLaunchKernelGGL(k_fcg_rel, dim3(1), dim3(256), 0, s, b.pc, gs, b.st, it, tol, maxiter, hist);
     prof_begin(kProfLearned, s);
-    CC_CHECK(learned_prec(net, b, b.r, diag, b.q, b.pa, s));
+    CC_CHECK(prec(b.r, b.q, b.pa));
     prof_end(s);
     hipLaunchKernelGGL(k_fcg_beta, dim3(1), dim3(256), 0, s, b.pa, gs, b.st);
     hipLaunchKernelGGL(k_fcg_dir, dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 0);
@@ -345,6 +366,145 @@ int fcg_operator(int n, const std::function<int(const double*, double*, double*,
   result->recursive_rel = hs.rel;
   result->true_rel = rel;
   return 0;
+}
+
+int fcg_operator(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                 const cc_cnn_desc& net, const double* diag, const double* bvec, double* x, double tol, int maxiter,
+                 int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist, hipStream_t s,
+                 const std::function<int(const double*, double*, double*)>& residual) {
+  const size_t voxels = (size_t)net.nx * net.ny * net.nz;
+  if (wsb < fcg_bytes(n, voxels)) return CC_E_WORKSPACE;
+  FcgBuffers b = carve_fcg((char*)ws, n, voxels);
+  // Grid tensors start at zero; only channel 0 at the unknowns' voxels is
+  // ever rewritten afterwards.
+  CC_CHECK(hipMemsetAsync(b.a0, 0, voxels * 16 * sizeof(uint16_t), s));
+  PrecFn prec = [&](const double* r, const double* q, double* part) {
+    return learned_prec(net, b, r, diag, q, part, s);
+  };
+  return fcg_loop(n, apply, prec, b, bvec, x, tol, maxiter, check_every, result, hist, s, residual);
+}
+
+// ---------------------------------------------------------------------------
+// Multilevel learned preconditioner.
+// ---------------------------------------------------------------------------
+
+struct MlBuffers {
+  float* u[CC_ML_MAX_LEVELS];  // S r restricted to level l (single channel)
+  float* t[CC_ML_MAX_LEVELS];  // the correction prolongated down to level l
+  float* y[CC_ML_MAX_LEVELS];  // network output [voxel][16] (levels with a net)
+  uint16_t* act[CC_ML_MAX_LEVELS][4];
+};
+
+static size_t ml_vox(const cc_cnn_desc& d) { return (size_t)d.nx * d.ny * d.nz; }
+
+bool ml_desc_ok(const cc_mlcnn_desc& ml) {
+  if (ml.levels < 1 || ml.levels > CC_ML_MAX_LEVELS) return false;
+  if (ml.net[0].n_red <= 0 || !ml.net[0].red2full) return false;
+  if (!ml.has_net[ml.levels - 1]) return false;  // the coarsest level carries a network
+  for (int l = 0; l < ml.levels; ++l) {
+    const cc_cnn_desc& d = ml.net[l];
+    if (d.nx <= 0 || d.ny <= 0 || d.nz <= 0) return false;
+    if (ml.has_net[l] && (!d.mask || !d.weights[0] || !d.weights[1] || !d.weights[2] || !d.weights[3])) return false;
+    if (l > 0) {
+      MgDims g;
+      if (!mg_level_dims(ml.net[l - 1], d, &g)) return false;
+      for (int k = 0; k < 3; ++k)
+        if (d.periodic[k] != ml.net[l - 1].periodic[k]) return false;
+    }
+  }
+  return true;
+}
+
+size_t ml_bytes(const cc_mlcnn_desc& ml) {
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
+  for (int l = 0; l < ml.levels; ++l) {
+    const size_t v = ml_vox(ml.net[l]);
+    add(v * sizeof(float));
+    add(v * sizeof(float));
+    if (ml.has_net[l]) {
+      add(v * 16 * sizeof(float));
+      for (int k = 0; k < 4; ++k) add(v * 16 * sizeof(uint16_t));
+    }
+  }
+  return b + 256;
+}
+
+static MlBuffers carve_ml(const cc_mlcnn_desc& ml, char* base) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    char* p = base + off;
+    off += bytes;
+    return p;
+  };
+  MlBuffers b{};
+  for (int l = 0; l < ml.levels; ++l) {
+    const size_t v = ml_vox(ml.net[l]);
+    b.u[l] = (float*)take(v * sizeof(float));
+    b.t[l] = (float*)take(v * sizeof(float));
+    if (ml.has_net[l]) {
+      b.y[l] = (float*)take(v * 16 * sizeof(float));
+      for (int k = 0; k < 4; ++k) b.act[l][k] = (uint16_t*)take(v * 16 * sizeof(uint16_t));
+    }
+  }
+  return b;
+}
+
+// Zero what is written only in part: u[0] off the unknowns' voxels and
+// channels 1..15 of each network's input.
+static int ml_setup(const cc_mlcnn_desc& ml, const MlBuffers& b, hipStream_t s) {
+  CC_CHECK(hipMemsetAsync(b.u[0], 0, ml_vox(ml.net[0]) * sizeof(float), s));
+  for (int l = 0; l < ml.levels; ++l)
+    if (ml.has_net[l]) CC_CHECK(hipMemsetAsync(b.act[l][0], 0, ml_vox(ml.net[l]) * 16 * sizeof(uint16_t), s));
+  return 0;
+}
+
+// z = D^-1 r + S sum_l Q_l C_l^T C_l Q_l^T S r; partial z.q when q is given.
+static int ml_prec(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double* r, const double* diag,
+                   const double* q, double* z, double* part, const FcgState* st, hipStream_t s) {
+  const cc_cnn_desc& f = ml.net[0];
+  const int n = f.n_red;
+  const int L = ml.levels;
+  hipLaunchKernelGGL(k_prec_in_f32, dim3(g1(n)), dim3(256), 0, s, n, f.red2full, r, diag, mb.u[0], st);
+  MgDims g[CC_ML_MAX_LEVELS];
+  for (int l = 1; l < L; ++l) {
+    mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
+    CC_CHECK(mg_restrict(g[l], mb.u[l - 1], 1, mb.u[l], 1, s));
+  }
+  for (int l = 0; l < L; ++l) {
+    if (!ml.has_net[l]) continue;
+    CC_CHECK(mg_to_ch0((long long)ml_vox(ml.net[l]), mb.u[l], 1, mb.act[l][0], s));
+    CC_CHECK(cnn_forward_layers(ml.net[l], mb.act[l][0], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s));
+  }
+  const float* src = mb.y[L - 1];
+  int ss = 16;
+  for (int l = L - 1; l >= 1; --l) {
+    CC_CHECK(mg_prolong(g[l], src, ss, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 16, mb.t[l - 1], s));
+    src = mb.t[l - 1];
+    ss = 1;
+  }
+  hipLaunchKernelGGL(k_prec_out, dim3(g1(n)), dim3(256), 0, s, n, f.red2full, r, diag, src, ss, z, q, part, st);
+  return (int)hipGetLastError();
+}
+
+int fcg_operator_ml(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                    const cc_mlcnn_desc& ml, const double* diag, const double* bvec, double* x, double tol,
+                    int maxiter, int check_every, void* ws, size_t wsb, cc_pcg_result* result, double* hist,
+                    hipStream_t s, const std::function<int(const double*, double*, double*)>& residual) {
+  const size_t core = (fcg_bytes(n, 0) + 255) & ~size_t(255);
+  if (wsb < core + ml_bytes(ml)) return CC_E_WORKSPACE;
+  FcgBuffers b = carve_fcg((char*)ws, n, 0);
+  MlBuffers mb = carve_ml(ml, (char*)ws + core);
+  CC_CHECK(ml_setup(ml, mb, s));
+  PrecFn prec = [&](const double* r, const double* q, double* part) {
+    return ml_prec(ml, mb, r, diag, q, b.z, part, b.st, s);
+  };
+  return fcg_loop(n, apply, prec, b, bvec, x, tol, maxiter, check_every, result, hist, s, residual);
+}
+
+size_t fcg_operator_ml_bytes(size_t n, const cc_mlcnn_desc& ml) {
+  return ((fcg_bytes(n, 0) + 255) & ~size_t(255)) + ml_bytes(ml);
 }
 
 size_t fcg_operator_bytes(size_t n, size_t voxels) { return fcg_bytes(n, voxels); }
@@ -383,9 +543,83 @@ __global__ __launch_bounds__(256) void k_train_loss(int n, const int* __restrict
   if (threadIdx.x < 2) part[2 * blockIdx.x + threadIdx.x] = out[threadIdx.x];
 }
 
+// Multilevel training input: u0[red2full[j]] = scale * S r_j (float32).
+__global__ __launch_bounds__(256) void k_mg_train_input(int n, const int* __restrict__ red2full,
+                                                        const float* __restrict__ r, const double* __restrict__ diag,
+                                                        float scale, float* __restrict__ u0) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+    u0[red2full[j]] = (float)(scale * sym_scale(diag[j]) * (double)r[j]);
+}
+
+// Multilevel loss: z = scale D^-1 r + S t0 (t0 the summed correction on the
+// background grid, single channel), L = ||z - scale e||^2 / ||scale e||^2;
+// g0 = dL/dt0 (times gscale) at the unknowns' voxels; partials as k_train_loss.
+__global__ __launch_bounds__(256) void k_mg_train_loss(int n, const int* __restrict__ red2full,
+                                                       const float* __restrict__ r, const float* __restrict__ e,
+                                                       const double* __restrict__ diag, float scale,
+                                                       const float* __restrict__ t0, float inv_den, float gscale,
+                                                       float* __restrict__ g0, double* __restrict__ part) {
+  __shared__ double red[8];
+  double dd = 0.0, tt = 0.0;
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+    const double d = diag[j];
+    const double sj = sym_scale(d);
+    const size_t v = (size_t)red2full[j];
+    const double z = scale * jacobi_part((double)r[j], d) + sj * (double)t0[v];
+    const double t = scale * (double)e[j];
+    const double diff = z - t;
+    dd += diff * diff;
+    tt += t * t;
+    g0[v] = (float)(2.0 * diff * sj * inv_den * gscale);
+  }
+  double v2[2] = {dd, tt}, out[2];
+  block_sum_n<2>(v2, out, red);
+  if (threadIdx.x < 2) part[2 * blockIdx.x + threadIdx.x] = out[threadIdx.x];
+}
+
 }  // namespace cc
 
 using namespace cc;
+
+extern "C" int cc_mg_train_input(int32_t n, const int32_t* red2full, const float* r, const double* diag, float scale,
+                                 float* u0, cc_stream_t stream) {
+  if (n <= 0 || !red2full || !r || !diag || !u0) return CC_E_ARG;
+  hipLaunchKernelGGL(k_mg_train_input, dim3(g1(n)), dim3(256), 0, (hipStream_t)stream, n, red2full, r, diag, scale,
+                     u0);
+  return (int)hipGetLastError();
+}
+
+extern "C" int cc_mg_train_loss(int32_t n, const int32_t* red2full, const float* r, const float* e,
+                                const double* diag, float scale, const float* t0, float inv_den, float gscale,
+                                float* g0, double* partial, int32_t* nparts, cc_stream_t stream) {
+  if (n <= 0 || !red2full || !r || !e || !diag || !t0 || !g0 || !partial || !nparts) return CC_E_ARG;
+  const int g = g1(n);
+  hipLaunchKernelGGL(k_mg_train_loss, dim3(g), dim3(256), 0, (hipStream_t)stream, n, red2full, r, e, diag, scale, t0,
+                     inv_den, gscale, g0, partial);
+  *nparts = g;
+  return (int)hipGetLastError();
+}
+
+extern "C" size_t cc_mlcnn_precondition_workspace_bytes(const cc_mlcnn_desc* ml) {
+  if (!ml || !ml_desc_ok(*ml)) return 0;
+  return ((fcg_bytes(ml->net[0].n_red, 0) + 255) & ~size_t(255)) + ml_bytes(*ml);
+}
+
+extern "C" int cc_mlcnn_precondition(const cc_mlcnn_desc* ml, const double* r, const double* diag, double* z,
+                                     void* workspace, size_t workspace_bytes, cc_stream_t stream) {
+  if (!ml || !ml_desc_ok(*ml) || !r || !diag || !z || !workspace) return CC_E_ARG;
+  if (workspace_bytes < cc_mlcnn_precondition_workspace_bytes(ml)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  const int n = ml->net[0].n_red;
+  const size_t core = (fcg_bytes(n, 0) + 255) & ~size_t(255);
+  FcgBuffers b = carve_fcg((char*)workspace, n, 0);
+  MlBuffers mb = carve_ml(*ml, (char*)workspace + core);
+  CC_CHECK(ml_setup(*ml, mb, s));
+  FcgState init{};
+  init.status = CC_RUNNING;
+  CC_CHECK(hipMemcpyAsync(b.st, &init, sizeof(FcgState), hipMemcpyHostToDevice, s));
+  return ml_prec(*ml, mb, r, diag, nullptr, z, nullptr, b.st, s);
+}
 
 extern "C" int cc_cnn_train_input(int32_t n, const int32_t* red2full, const float* r, const double* diag, float scale,
                                   uint16_t* grid, cc_stream_t stream) {

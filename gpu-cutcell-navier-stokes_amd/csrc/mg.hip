@@ -1,0 +1,203 @@
+// mg.hip - background-grid hierarchy of the multilevel learned preconditioner.
+//
+// Level l + 1 of a dof grid keeps every other grid point of level l: on a
+// clamped axis n_c = (n_f + 1) / 2 and coarse point c sits on fine point 2c;
+// on a periodic axis n_f is even and n_c = n_f / 2 (indices wrap). The
+// prolongation P is tensor-product linear interpolation:
+//   fine 2c      <- coarse c            (weight 1)
+//   fine 2c + 1  <- coarse c, c + 1     (weights 1/2, 1/2; c + 1 wraps on a
+//                                        periodic axis; past the end of a
+//                                        clamped one, coarse c alone with
+//                                        weight 1, so P keeps constants)
+// and the restriction is its exact transpose R = P^T (coarse c gathers fine
+// 2c - 1, 2c, 2c + 1 with weights 1/2, 1, 1/2 where the fine point exists,
+// and 1 for the trailing fine point of an even-length clamped axis),
+// so that  Q_l C^T C Q_l^T  is symmetric for any level operator C.
+// Arrays are single-channel float32 grids (x fastest) read and written with
+// an element stride, so a network's channel-0 output ([voxel][16]) can be
+// prolongated in place.
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/cutcell_hip.h"
+#include "common.h"
+
+namespace cc {
+
+
+// Restriction tap d in {-1, 0, 1} of coarse point c: the fine index and
+// weight, or false when that fine point does not exist.
+__device__ inline bool rest_tap(int c, int d, const MgAxis& ax, int* f, float* w) {
+  int i = 2 * c + d;
+  if (i < 0) {
+    if (!ax.per) return false;
+    i += ax.nf;
+  } else if (i >= ax.nf) {
+    if (!ax.per) return false;
+    i -= ax.nf;
+  }
+  *f = i;
+  // The last coarse point is the only source of a trailing odd fine point.
+  *w = (d == 0 || (d == 1 && !ax.per && c + 1 == ax.nc)) ? 1.0f : 0.5f;
+  return true;
+}
+
+// Prolongation sources of fine point f: count 1 or 2, coarse indices and weights.
+__device__ inline int pro_src(int f, const MgAxis& ax, int* c, float* w) {
+  if ((f & 1) == 0) {
+    c[0] = f >> 1;
+    w[0] = 1.0f;
+    return 1;
+  }
+  c[0] = (f - 1) >> 1;
+  w[0] = 0.5f;
+  const int c1 = c[0] + 1;
+  if (c1 < ax.nc) {
+    c[1] = c1;
+    w[1] = 0.5f;
+    return 2;
+  }
+  if (ax.per) {
+    c[1] = 0;
+    w[1] = 0.5f;
+    return 2;
+  }
+  w[0] = 1.0f;
+  return 1;
+}
+
+__global__ __launch_bounds__(256) void k_mg_restrict(MgDims g, const float* __restrict__ fine, int fs,
+                                                     float* __restrict__ coarse, int cs) {
+  const int ncx = g.a[0].nc, ncy = g.a[1].nc, ncz = g.a[2].nc;
+  const long long total = (long long)ncx * ncy * ncz;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+       v += (long long)gridDim.x * blockDim.x) {
+    const int cx = (int)(v % ncx), cy = (int)((v / ncx) % ncy), cz = (int)(v / ((long long)ncx * ncy));
+    float acc = 0.0f;
+    for (int dz = -1; dz <= 1; ++dz) {
+      int fz;
+      float wz;
+      if (!rest_tap(cz, dz, g.a[2], &fz, &wz)) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        int fy;
+        float wy;
+        if (!rest_tap(cy, dy, g.a[1], &fy, &wy)) continue;
+        const size_t rowoff = ((size_t)fz * g.a[1].nf + fy) * g.a[0].nf;
+#pragma unroll
+        for (int dx = -1; dx <= 1; ++dx) {
+          int fx;
+          float wx;
+          if (!rest_tap(cx, dx, g.a[0], &fx, &wx)) continue;
+          acc += (wz * wy * wx) * fine[(rowoff + fx) * fs];
+        }
+      }
+    }
+    coarse[(size_t)v * cs] = acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mg_prolong(MgDims g, const float* __restrict__ coarse, int cs,
+                                                    const float* __restrict__ add, int as, float* __restrict__ fine) {
+  const int nfx = g.a[0].nf, nfy = g.a[1].nf, nfz = g.a[2].nf;
+  const long long total = (long long)nfx * nfy * nfz;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+       v += (long long)gridDim.x * blockDim.x) {
+    const int fx = (int)(v % nfx), fy = (int)((v / nfx) % nfy), fz = (int)(v / ((long long)nfx * nfy));
+    int cx[2], cy[2], cz[2];
+    float wx[2], wy[2], wz[2];
+    const int nx = pro_src(fx, g.a[0], cx, wx);
+    const int ny = pro_src(fy, g.a[1], cy, wy);
+    const int nz = pro_src(fz, g.a[2], cz, wz);
+    float acc = add ? add[(size_t)v * as] : 0.0f;
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+          acc += (wz[k] * wy[j] * wx[i]) *
+                 coarse[(((size_t)cz[k] * g.a[1].nc + cy[j]) * g.a[0].nc + cx[i]) * cs];
+    fine[v] = acc;
+  }
+}
+
+__device__ inline uint16_t mg_f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__global__ __launch_bounds__(256) void k_mg_to_ch0(long long nvox, const float* __restrict__ src, int ss,
+                                                   uint16_t* __restrict__ grid) {
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+       v += (long long)gridDim.x * blockDim.x)
+    grid[(size_t)v * 16] = mg_f2bf(src[(size_t)v * ss]);
+}
+
+static int mg_grid(long long work) {
+  long long g = (work + 255) / 256;
+  if (g > 4096) g = 4096;
+  return (int)(g < 1 ? 1 : g);
+}
+
+static bool mg_dims(const int32_t* nf, const int32_t* nc, const int32_t* per, MgDims* g) {
+  if (!nf || !nc || !per) return false;
+  for (int k = 0; k < 3; ++k) {
+    if (nf[k] <= 0 || nc[k] <= 0) return false;
+    if (per[k] ? (nf[k] % 2 != 0 || nc[k] != nf[k] / 2) : nc[k] != (nf[k] + 1) / 2) return false;
+    g->a[k] = MgAxis{nf[k], nc[k], per[k] != 0};
+  }
+  return true;
+}
+
+int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s) {
+  const long long nc = (long long)g.a[0].nc * g.a[1].nc * g.a[2].nc;
+  hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(nc)), dim3(256), 0, s, g, fine, fs, coarse, cs);
+  return (int)hipGetLastError();
+}
+
+int mg_prolong(const MgDims& g, const float* coarse, int cs, const float* add, int as, float* fine, hipStream_t s) {
+  const long long nf = (long long)g.a[0].nf * g.a[1].nf * g.a[2].nf;
+  hipLaunchKernelGGL(k_mg_prolong, dim3(mg_grid(nf)), dim3(256), 0, s, g, coarse, cs, add, as, fine);
+  return (int)hipGetLastError();
+}
+
+int mg_to_ch0(long long nvox, const float* src, int ss, uint16_t* grid, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_to_ch0, dim3(mg_grid(nvox)), dim3(256), 0, s, nvox, src, ss, grid);
+  return (int)hipGetLastError();
+}
+
+bool mg_level_dims(const cc_cnn_desc& fine, const cc_cnn_desc& coarse, MgDims* g) {
+  const int32_t nf[3] = {fine.nx, fine.ny, fine.nz};
+  const int32_t nc[3] = {coarse.nx, coarse.ny, coarse.nz};
+  return mg_dims(nf, nc, fine.periodic, g);
+}
+
+}  // namespace cc
+
+using namespace cc;
+
+extern "C" {
+
+int cc_mg_restrict(const int32_t* nf, const int32_t* nc, const int32_t* periodic, const float* fine,
+                   int32_t fine_stride, float* coarse, int32_t coarse_stride, cc_stream_t stream) {
+  MgDims g;
+  if (!mg_dims(nf, nc, periodic, &g) || !fine || !coarse || fine_stride < 1 || coarse_stride < 1) return CC_E_ARG;
+  return mg_restrict(g, fine, fine_stride, coarse, coarse_stride, (hipStream_t)stream);
+}
+
+int cc_mg_prolong(const int32_t* nf, const int32_t* nc, const int32_t* periodic, const float* coarse,
+                  int32_t coarse_stride, const float* add, int32_t add_stride, float* fine, cc_stream_t stream) {
+  MgDims g;
+  if (!mg_dims(nf, nc, periodic, &g) || !coarse || !fine || coarse_stride < 1 || (add && add_stride < 1))
+    return CC_E_ARG;
+  return mg_prolong(g, coarse, coarse_stride, add, add_stride, fine, (hipStream_t)stream);
+}
+
+int cc_mg_to_ch0(int64_t nvox, const float* src, int32_t src_stride, uint16_t* grid, cc_stream_t stream) {
+  if (nvox <= 0 || !src || !grid || src_stride < 1) return CC_E_ARG;
+  return mg_to_ch0(nvox, src, src_stride, grid, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -101,7 +101,7 @@ void prof_collect() {
 struct SpmmConfig {
   int nz;      // nonzero slots per row team ((K/cpl)*nz <= 64); 0 = pick from the row length
   int grid;    // blocks; 0 = min(kMaxParts, rows / teams per block)
-  int unroll;  // 1 or 4 independent loads in flight per lane; 0 = 4
+  int unroll;  // 1, 4 or 8 (software-pipelined) loads in flight per lane; 0 = 4
   int cpl;     // right-hand sides per lane (vector width); 0 = widest 16-byte load
   int stream;  // 1: nontemporal matrix loads (unroll 4 only)
 };
@@ -332,6 +332,29 @@ __device__ inline void vstore(VT* __restrict__ p, const VT* in) {
 // MODE 1: Y = B - A X, partial[block][c] = sum_rows Y.Y  (true residual)
 // A row team has KL = K/CPL lanes per nonzero slot (each lane owns CPL
 // consecutive right-hand sides, loaded as one vector) and NZ slots.
+// First row r in [0, n] with indptr[r] >= target (indptr is nondecreasing and
+// indptr[n] >= target), found by one wave with a 64-ary search: every round
+// probes 64 evenly spaced row pointers and keeps the bracket of the first
+// one at or past the target. Three rounds cover 262k rows. Every lane of the
+// wave returns the same row.
+__device__ inline int row_lower_bound(const int* __restrict__ indptr, int n, long long target) {
+  const int lane = threadIdx.x & 63;
+  int lo = 0, hi = n;
+  while (hi - lo > 64) {
+    const int step = (hi - lo + 63) / 64;
+    const int idx = min(lo + lane * step, hi);
+    const unsigned long long m = __ballot(indptr[idx] >= target);
+    const int f = m ? __ffsll((long long)m) - 1 : 64;
+    const int nlo = f == 0 ? lo : lo + (f - 1) * step;
+    const int nhi = f == 64 ? hi : min(lo + f * step, hi);
+    lo = nlo;
+    hi = nhi;
+  }
+  const int idx = lo + lane;
+  const unsigned long long m = __ballot(idx <= hi && indptr[min(idx, hi)] >= target);
+  return m ? lo + __ffsll((long long)m) - 1 : hi;
+}
+
 template <bool NTL, typename T>
 __device__ inline T stream_ld(const T* p) {
   if constexpr (NTL)
@@ -360,13 +383,26 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
   const int wpb = blockDim.x >> 6;
   const int tpb = wpb * TPW;
   const int team = wave * TPW + lane / T;
-  // XCD-aware chunk of rows for this block.
+  // XCD-aware chunk of rows for this block: chunks hold equal shares of the
+  // nonzeros (rows along a cut are up to three times longer than the rest),
+  // and the blocks one XCD runs (block id mod 8) take adjacent chunks.
   const int G = gridDim.x;
   const int b = blockIdx.x;
   const int chunk = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
-  const int rows_per = (n + G - 1) / G;
-  const int r0 = chunk * rows_per;
-  const int r1 = min(n, r0 + rows_per);
+  __shared__ int s_rows[2];
+  if (wave == 0) {
+    const long long base = indptr[0];
+    const long long total = (long long)indptr[n] - base;
+    const int a0 = chunk == 0 ? 0 : row_lower_bound(indptr, n, base + total * chunk / G);
+    const int a1 = chunk == G - 1 ? n : row_lower_bound(indptr, n, base + total * (chunk + 1) / G);
+    if (lane == 0) {
+      s_rows[0] = a0;
+      s_rows[1] = a1;
+    }
+  }
+  __syncthreads();
+  const int r0 = s_rows[0];
+  const int r1 = s_rows[1];
   double dot[CPL];
 #pragma unroll
   for (int m = 0; m < CPL; ++m) dot[m] = 0.0;
@@ -403,6 +439,57 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
       }
 #pragma unroll
       for (int m = 0; m < CPL; ++m) acc[m] = (acc[m] + a1[m]) + (a2[m] + a3[m]);
+    }
+    if constexpr (UNR == 8) {
+      // Software-pipelined: the indices and values of the next 8 entries of
+      // this lane are in flight while the X rows of the current 8 are
+      // gathered, so each step waits on one memory latency (the gather),
+      // not on the index load and then the gather. Entries past the row end
+      // read the row's own X row (always in range) and contribute zero.
+      constexpr int U = 8;
+      int qa[U], qb[U];
+      AT va[U], vb[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int jj = zs + u * NZ;
+        qa[u] = jj < len ? stream_ld<NTL>(ixp + jj) : row;
+        va[u] = jj < len ? (AT)stream_ld<NTL>(vap + jj) : (AT)0;
+      }
+      AT a1[CPL];
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) a1[m] = 0;
+      for (int j0 = zs; j0 < len; j0 += U * NZ) {
+        const int jn = j0 + U * NZ;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int jj = jn + u * NZ;
+          qb[u] = jj < len ? stream_ld<NTL>(ixp + jj) : row;
+          vb[u] = jj < len ? (AT)stream_ld<NTL>(vap + jj) : (AT)0;
+        }
+        VT xv[U][CPL];
+#pragma unroll
+        for (int u = 0; u < U; ++u) vload<VT, CPL>(X + (size_t)qa[u] * K + c0, xv[u]);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const bool live = j0 + u * NZ < len;
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) {
+            const AT t = live ? va[u] * (AT)xv[u][m] : (AT)0;
+            if (u & 1)
+              a1[m] += t;
+            else
+              acc[m] += t;
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          qa[u] = qb[u];
+          va[u] = vb[u];
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) acc[m] += a1[m];
+      j = len;
     }
     for (; j < len; j += NZ) {
       const AT v = (AT)stream_ld<NTL>(vap + j);
@@ -679,7 +766,13 @@ static int spmm_launch_nz(int n, int unroll, int grid_req, int stream, const int
   if (grid > kMaxParts) grid = kMaxParts;
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
-  if (unroll >= 4 && stream)
+  if (unroll == 8 && stream)
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 8, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
+                       vals, X, Y, B, partial, st);
+  else if (unroll == 8)
+    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 8, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
+                       indices, vals, X, Y, B, partial, st);
+  else if (unroll >= 4 && stream)
     hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
                        vals, X, Y, B, partial, st);
   else if (unroll >= 4)
@@ -1085,7 +1178,7 @@ int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t
   if (cols_per_lane > k) return CC_E_ARG;
   const int kl = k / (cols_per_lane > 0 ? cols_per_lane : 1);
   if (nz_lanes < 0 || kl * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
-  if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4)) return CC_E_ARG;
+  if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4 && unroll != 8)) return CC_E_ARG;
   if (stream_loads != 0 && (stream_loads != 1 || unroll == 1)) return CC_E_ARG;
   g_spmm_cfg[value_bytes == 4 ? 1 : 0][k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane, stream_loads};
   return 0;

@@ -37,21 +37,29 @@ __device__ inline int wrap_or_invalid(int i, int n, int periodic) {
   return (i >= 0 && i < n) ? i : -1;
 }
 
+// u = E x, interior dofs: u[g] = x[full2red[g]] (0 off the unknowns; the
+// small dofs are written by k_mf_gather_small afterwards).
 __global__ __launch_bounds__(256) void k_mf_gather(cc_sumfact_desc d, const double* __restrict__ x,
                                                    double* __restrict__ u) {
   const int n_full = d.nb[0] * d.nb[1] * d.nb[2];
   for (int g = blockIdx.x * blockDim.x + threadIdx.x; g < n_full; g += gridDim.x * blockDim.x) {
     const int r = d.full2red[g];
-    double val = 0.0;
-    if (r >= 0) {
-      val = x[r];
-    } else {
-      const int sm = d.full2small[g];
-      if (sm >= 0)
-        for (int t = d.small_ptr[sm]; t < d.small_ptr[sm + 1]; ++t) val += d.small_w[t] * x[d.small_col[t]];
-    }
-    u[g] = val;
+    u[g] = r >= 0 ? x[r] : 0.0;
   }
+}
+
+// u = E x, small dofs: one wave per small dof, its stencil entries spread
+// over the lanes (a stencil holds up to a few tens of entries) and summed by
+// a butterfly, so a small dof costs two memory latencies, not one per entry.
+__global__ __launch_bounds__(256) void k_mf_gather_small(cc_sumfact_desc d, const double* __restrict__ x,
+                                                         double* __restrict__ u) {
+  const int lane = threadIdx.x & 63;
+  const int sm = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (sm >= d.n_small) return;
+  double s = 0.0;
+  for (int t = d.small_ptr[sm] + lane; t < d.small_ptr[sm + 1]; t += 64) s += d.small_w[t] * x[d.small_col[t]];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) u[d.small_dof[sm]] = s;
 }
 
 template <int P1>
@@ -202,25 +210,50 @@ __global__ __launch_bounds__(256) void k_mf_shell(cc_sumfact_desc d, const doubl
   }
 }
 
+// v[g] += sum of the shell-cell outputs W that belong to dof g; a 16-lane
+// team per touched dof (a dof lies in up to (p+1)^3 shell cells).
 __global__ __launch_bounds__(256) void k_mf_touch(cc_sumfact_desc d, const double* __restrict__ Win,
                                                   double* __restrict__ v) {
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < d.n_touch; t += gridDim.x * blockDim.x) {
+  const int tl = threadIdx.x & 15;
+  const int team = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int nteams = (gridDim.x * blockDim.x) >> 4;
+  for (int t = team; t < d.n_touch; t += nteams) {
     double s = 0.0;
-    for (int i = d.touch_ptr[t]; i < d.touch_ptr[t + 1]; ++i) s += Win[d.touch_idx[i]];
-    v[d.touch_dof[t]] += s;
+    for (int i = d.touch_ptr[t] + tl; i < d.touch_ptr[t + 1]; i += 16) s += Win[d.touch_idx[i]];
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (tl == 0) v[d.touch_dof[t]] += s;
   }
 }
 
-// y = E^T v; partial[block] = sum_j x_j y_j (when partial != null).
+// The E^T part of the small dofs: extra[j] = sum_b E[b, j] v[small dof b]
+// for the unknowns j that have such entries (16-lane team per unknown).
+__global__ __launch_bounds__(256) void k_mf_et_extra(cc_sumfact_desc d, const double* __restrict__ v,
+                                                     double* __restrict__ extra, const ColState* __restrict__ st) {
+  if (st && st[0].status != CC_RUNNING) return;
+  const int tl = threadIdx.x & 15;
+  const int team = (blockIdx.x * blockDim.x + threadIdx.x) >> 4;
+  const int nteams = (gridDim.x * blockDim.x) >> 4;
+  for (int i = team; i < d.n_et_rows; i += nteams) {
+    const int j = d.et_rows[i];
+    double s = 0.0;
+    for (int t = d.et_ptr[j] + tl; t < d.et_ptr[j + 1]; t += 16) s += d.et_w[t] * v[d.small_dof[d.et_small[t]]];
+    for (int o = 8; o > 0; o >>= 1) s += __shfl_xor(s, o, 16);
+    if (tl == 0) extra[j] = s;
+  }
+}
+
+// y = E^T v (interior part plus extra[] where the unknown has E^T entries);
+// partial[block] = sum_j x_j y_j (when partial != null).
 __global__ __launch_bounds__(256) void k_mf_scatter(cc_sumfact_desc d, const double* __restrict__ v,
-                                                    double* __restrict__ y, const double* __restrict__ x,
-                                                    double* __restrict__ partial, const ColState* __restrict__ st) {
+                                                    const double* __restrict__ extra, double* __restrict__ y,
+                                                    const double* __restrict__ x, double* __restrict__ partial,
+                                                    const ColState* __restrict__ st) {
   __shared__ double red[4];
   if (st && st[0].status != CC_RUNNING) return;
   double dot = 0.0;
   for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < d.n_red; j += gridDim.x * blockDim.x) {
     double s = v[d.red2full[j]];
-    for (int t = d.et_ptr[j]; t < d.et_ptr[j + 1]; ++t) s += d.et_w[t] * v[d.small_dof[d.et_small[t]]];
+    if (d.et_ptr[j + 1] > d.et_ptr[j]) s += extra[j];
     y[j] = s;
     if (partial) dot += x[j] * s;
   }
@@ -258,23 +291,28 @@ __global__ __launch_bounds__(256) void k_mf_residual(int n, const double* __rest
 }
 
 struct MfBuffers {
-  double *u, *v, *W;
+  double *u, *v, *W, *extra;
 };
 
 static size_t mf_bytes(const cc_sumfact_desc* d) {
   const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
   const int P1 = d->degree + 1;
   const size_t pp = (size_t)P1 * P1 * P1;
-  return 2 * n_full * sizeof(double) + (size_t)d->n_corr * pp * sizeof(double) + 3 * 256;
+  return 2 * n_full * sizeof(double) + (size_t)d->n_corr * pp * sizeof(double) + (size_t)d->n_red * sizeof(double) +
+         4 * 256;
 }
 
 static MfBuffers carve_mf(const cc_sumfact_desc* d, char* base) {
   const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
+  const int P1 = d->degree + 1;
+  const size_t pp = (size_t)P1 * P1 * P1;
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
   MfBuffers b;
   b.u = reinterpret_cast<double*>(base);
   b.v = reinterpret_cast<double*>(base + al(n_full * sizeof(double)));
   b.W = reinterpret_cast<double*>(base + 2 * al(n_full * sizeof(double)));
+  b.extra = reinterpret_cast<double*>(base + 2 * al(n_full * sizeof(double)) +
+                                      al((size_t)d->n_corr * pp * sizeof(double)));
   return b;
 }
 
@@ -289,14 +327,17 @@ static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double
                       int* nparts, const ColState* st, hipStream_t s) {
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   hipLaunchKernelGGL(k_mf_gather, dim3(grid1d(n_full)), dim3(256), 0, s, d, x, b.u);
+  if (d.n_small > 0) hipLaunchKernelGGL(k_mf_gather_small, dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
   dim3 tiles((d.nb[0] + kTile - 1) / kTile, (d.nb[1] + kTile - 1) / kTile, (d.nb[2] + kTile - 1) / kTile);
   hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v);
   if (d.n_corr > 0) {
     hipLaunchKernelGGL((k_mf_shell<P1>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
-    hipLaunchKernelGGL(k_mf_touch, dim3(grid1d(d.n_touch)), dim3(256), 0, s, d, b.W, b.v);
+    hipLaunchKernelGGL(k_mf_touch, dim3(grid1d((size_t)d.n_touch * 16)), dim3(256), 0, s, d, b.W, b.v);
   }
+  if (d.n_et_rows > 0)
+    hipLaunchKernelGGL(k_mf_et_extra, dim3(grid1d((size_t)d.n_et_rows * 16)), dim3(256), 0, s, d, b.v, b.extra, st);
   const int gs = grid1d(d.n_red);
-  hipLaunchKernelGGL(k_mf_scatter, dim3(gs), dim3(256), 0, s, d, b.v, y, x, partial, st);
+  hipLaunchKernelGGL(k_mf_scatter, dim3(gs), dim3(256), 0, s, d, b.v, b.extra, y, x, partial, st);
   if (nparts) *nparts = gs;
   return (int)hipGetLastError();
 }
@@ -310,6 +351,28 @@ static int mf_apply(const cc_sumfact_desc& d, const MfBuffers& b, const double* 
   }
 }
 
+// ||b - A x|| / ||b|| (||b|| = 0 -> 1) with the matrix-free operator; the
+// per-block partials are summed on the host in block order.
+static int mf_true_residual(const cc_sumfact_desc& d, const MfBuffers& mb, const double* bvec, const double* X,
+                            double* scratch, double* rpart, double* rel, hipStream_t s) {
+  CC_CHECK(mf_apply(d, mb, X, scratch, nullptr, nullptr, nullptr, s));
+  const int g = grid1d(d.n_red);
+  hipLaunchKernelGGL(k_mf_residual, dim3(g), dim3(256), 0, s, d.n_red, bvec, scratch, rpart);
+  CC_CHECK(hipGetLastError());
+  std::vector<double> hp(2 * (size_t)g);
+  CC_CHECK(hipMemcpyAsync(hp.data(), rpart, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < g; ++i) {
+    rr += hp[2 * i];
+    bb += hp[2 * i + 1];
+  }
+  double bn = std::sqrt(bb);
+  if (bn == 0.0) bn = 1.0;
+  *rel = std::sqrt(rr) / bn;
+  return 0;
+}
+
 static bool desc_ok(const cc_sumfact_desc* d) {
   if (!d || (d->degree != 2 && d->degree != 3) || d->n_red <= 0) return false;
   for (int k = 0; k < 3; ++k) {
@@ -318,6 +381,8 @@ static bool desc_ok(const cc_sumfact_desc* d) {
     if (d->periodic[k] ? d->nb[k] != d->ne[k] : d->nb[k] != d->ne[k] + d->degree) return false;
   }
   if (!d->red2full || !d->full2red || !d->full2small || !d->small_ptr || !d->et_ptr) return false;
+  if (d->n_et_rows < 0 || (d->n_et_rows > 0 && !d->et_rows)) return false;
+  if (d->n_small > 0 && (!d->small_col || !d->small_w || !d->small_dof || !d->et_small || !d->et_w)) return false;
   if (d->n_corr > 0 && (!d->corr_cell || !d->corr_block || !d->blocks || !d->touch_dof || !d->touch_ptr ||
                         !d->touch_idx))
     return false;
@@ -423,6 +488,37 @@ int cc_fcg_sumfact(const cc_sumfact_desc* d, const cc_cnn_desc* net, const doubl
   };
   return fcg_operator(desc.n_red, apply, *net, desc.diag, bvec, x, tol, maxiter, check_every, fws, fb, result,
                       history, s, residual);
+}
+
+size_t cc_fcg_sumfact_ml_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml) {
+  if (!desc_ok(d) || !ml || !ml_desc_ok(*ml)) return 0;
+  return mf_bytes(d) + 1024 + (size_t)kMaxParts * 2 * sizeof(double) + fcg_operator_ml_bytes((size_t)d->n_red, *ml) +
+         512;
+}
+
+int cc_fcg_sumfact_ml(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, const double* bvec, double* x, double tol,
+                      int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
+                      cc_pcg_result* result, double* history, cc_stream_t stream) {
+  if (!desc_ok(d) || !ml || !ml_desc_ok(*ml) || !bvec || !x || !workspace || !result || maxiter < 0 ||
+      ml->net[0].n_red != d->n_red)
+    return CC_E_ARG;
+  if (workspace_bytes < cc_fcg_sumfact_ml_workspace_bytes(d, ml)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  MfBuffers mb = carve_mf(d, base);
+  const size_t off = (mf_bytes(d) + 511) & ~size_t(255);
+  double* rpart = reinterpret_cast<double*>(base + off);
+  char* fws = base + off + (((size_t)kMaxParts * 2 * sizeof(double) + 511) & ~size_t(255));
+  const size_t fb = workspace_bytes - (size_t)(fws - base);
+  const cc_sumfact_desc desc = *d;
+  auto apply = [&](const double* P, double* Q, double* partial, int* gs) {
+    return mf_apply(desc, mb, P, Q, partial, gs, nullptr, s);
+  };
+  auto residual = [&](const double* X, double* scratch, double* rel) {
+    return mf_true_residual(desc, mb, bvec, X, scratch, rpart, rel, s);
+  };
+  return fcg_operator_ml(desc.n_red, apply, *ml, desc.diag, bvec, x, tol, maxiter, check_every, fws, fb, result,
+                         history, s, residual);
 }
 
 size_t cc_fcg_sumfact_workspace_bytes(const cc_sumfact_desc* d, const cc_cnn_desc* net) {

@@ -97,7 +97,7 @@ def spmm_candidates(k: int, mean_row: float, value_bytes: int = 8):
         nzs = nzs[:2] if nzs else [max(1, 64 // kl)]
         for nz in nzs:
             for grid in (512, 1024):
-                for unroll, stream in ((1, 0), (4, 0), (4, 1)):
+                for unroll, stream in ((1, 0), (4, 0), (4, 1), (8, 0), (8, 1)):
                     yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll, stream=stream)
 
 
@@ -108,7 +108,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     lib = hip.require_gpu()
     mean_row = A.nnz / max(A.n, 1)
     vb = 8 if precision == "fp64" else 4
-    key = f"spmm2|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
+    key = f"spmm3|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
     table = _load()
     if key in table and not force:
         cfg = table[key]

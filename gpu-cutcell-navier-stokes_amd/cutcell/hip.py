@@ -185,7 +185,8 @@ def spmv(A: DeviceCSR, x, y=None):
 
 
 def spmm(A: DeviceCSR, X, Y=None):
-    """``Y = A X`` for a row-major ``[n, k]`` float64 block, k in {1,2,4,8,16}."""
+    """``Y = A X`` for a row-major ``[n, k]`` float64 (or float32, with the
+    float32 copy of the values) block, k in {1,2,4,8,16}."""
     lib = require_gpu()
     torch = _torch()
     k = X.shape[1]
@@ -193,9 +194,13 @@ def spmm(A: DeviceCSR, X, Y=None):
         raise ValueError(f"k must be one of {SUPPORTED_K}")
     X = X.contiguous()
     Y = torch.empty_like(X) if Y is None else Y
-    _check(
-        lib.cc_spmm_csr_f64(A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(X), _ptr(Y), _stream()), "spmm"
-    )
+    if X.dtype == torch.float64:
+        fn, vals = lib.cc_spmm_csr_f64, A.vals
+    elif X.dtype == torch.float32:
+        fn, vals = lib.cc_spmm_csr_f32, A.ensure_f32()
+    else:
+        raise TypeError("X must be float64 or float32")
+    _check(fn(A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(vals), _ptr(X), _ptr(Y), _stream()), "spmm")
     return Y
 
 

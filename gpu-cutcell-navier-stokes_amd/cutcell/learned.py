@@ -52,9 +52,27 @@ class CnnDesc(ctypes.Structure):
     ]
 
 
+ML_MAX_LEVELS = 8
+
+
+class MlDesc(ctypes.Structure):
+    _fields_ = [
+        ("levels", ctypes.c_int32),
+        ("has_net", ctypes.c_int32 * ML_MAX_LEVELS),
+        ("net", CnnDesc * ML_MAX_LEVELS),
+    ]
+
+
 def _bind(lib):
     P, I, F, D, SZ = ctypes.c_void_p, ctypes.c_int32, ctypes.c_float, ctypes.c_double, ctypes.c_size_t
     sigs = {
+        "cc_mg_restrict": [P, P, P, P, I, P, I, P],
+        "cc_mg_prolong": [P, P, P, P, I, P, I, P, P],
+        "cc_mg_to_ch0": [ctypes.c_int64, P, I, P, P],
+        "cc_mg_train_input": [I, P, P, P, F, P, P],
+        "cc_mg_train_loss": [I, P, P, P, P, F, P, F, F, P, P, P, P],
+        "cc_mlcnn_precondition": [P, P, P, P, P, SZ, P],
+        "cc_fcg_sumfact_ml": [P, P, P, P, D, I, I, P, SZ, P, P, P],
         "cc_mfma_probe": [P, P, P, P],
         "cc_cnn_conv3_forward": [I, I, I, P, P, P, P, P, I, P, P, P],
         "cc_cnn_flip_weights": [P, P, P],
@@ -75,6 +93,8 @@ def _bind(lib):
         "cc_cnn_wgrad_workspace_bytes": [I, I, I, I],
         "cc_cnn_precondition_workspace_bytes": [P],
         "cc_fcg_sumfact_workspace_bytes": [P, P],
+        "cc_mlcnn_precondition_workspace_bytes": [P],
+        "cc_fcg_sumfact_ml_workspace_bytes": [P, P],
     }.items():
         fn = getattr(lib, name)
         fn.argtypes = args
@@ -112,8 +132,8 @@ class MaskedCNN:
     """Master weights, optimiser state and device buffers of the symmetric
     preconditioner network (two fp32 masters W0, W1; four bf16 layers)."""
 
-    def __init__(self, grid_shape, red2full: np.ndarray, periodic=(False, False, False), seed: int = 0,
-                 device: str = "cuda", init_scale: float = 0.3):
+    def __init__(self, grid_shape, red2full: np.ndarray | None, periodic=(False, False, False), seed: int = 0,
+                 device: str = "cuda", init_scale: float = 0.3, mask: np.ndarray | None = None):
         import torch
 
         self.torch = torch
@@ -122,7 +142,7 @@ class MaskedCNN:
         self.voxels = self.nx * self.ny * self.nz
         self.periodic = (ctypes.c_int32 * 3)(*[int(bool(p)) for p in periodic])
         self.device = device
-        self.n = int(red2full.size)
+        self.n = int(red2full.size) if red2full is not None else 0
         rng = np.random.default_rng(seed)
         w0 = rng.standard_normal((27, 16, 16)) * np.sqrt(1.0 / 27.0) * init_scale
         w0[:, 1:, :] = 0.0  # the input is channel 0 only
@@ -135,17 +155,18 @@ class MaskedCNN:
         self.grad = torch.zeros_like(self.w32)  # masters' gradient
         self.lgrad = torch.zeros((NLAYERS, WSIZE), dtype=torch.float32, device=device)  # per layer
         self.step_count = 0
-        mask = np.zeros(self.voxels, dtype=np.uint8)
-        mask[red2full] = 1
-        self.mask = t(mask, np.uint8)
-        self.red2full = t(red2full, np.int32)
+        if mask is None:
+            mask = np.zeros(self.voxels, dtype=np.uint8)
+            mask[red2full] = 1
+        self.mask = t(np.asarray(mask, dtype=np.uint8).ravel(), np.uint8)
+        self.red2full = t(red2full if red2full is not None else np.zeros(1, np.int64), np.int32)
         # Activation buffers (bf16 [voxel][16]) and outputs.
         z16 = lambda: torch.zeros(self.voxels * 16, dtype=torch.uint16, device=device)
         self.acts = [z16() for _ in range(NLAYERS)]
         self.y = torch.zeros(self.voxels * 16, dtype=torch.float32, device=device)
         self.g = [z16() for _ in range(2)]
         self.gout = z16()
-        self.wgrad_blocks = 512
+        self.wgrad_blocks = max(1, min(512, ((self.nx + 15) // 16) * self.ny * self.nz))
         nb = int(self.lib.cc_cnn_wgrad_workspace_bytes(self.nx, self.ny, self.nz, self.wgrad_blocks))
         self.wgrad_ws = torch.empty(nb, dtype=torch.uint8, device=device)
         self.part = torch.zeros(2 * 1024, dtype=torch.float64, device=device)
@@ -175,7 +196,7 @@ class MaskedCNN:
         for l in range(NLAYERS):
             d.weights[l] = self.wbf[l].data_ptr()
         d.mask = self.mask.data_ptr()
-        d.red2full = self.red2full.data_ptr()
+        d.red2full = self.red2full.data_ptr() if self.n else None
         d.n_red = self.n
         self._desc = d
         return d
@@ -314,6 +335,259 @@ class MaskedCNN:
         return x, res
 
 
+def coarse_dims(dims, periodic):
+    """Grid of the next level: every other point per axis (periodic axes
+    must have an even count)."""
+    out = []
+    for n, per in zip(dims, periodic):
+        if per:
+            if n % 2:
+                raise ValueError("a periodic axis needs an even point count to coarsen")
+            out.append(n // 2)
+        else:
+            out.append((n + 1) // 2)
+    return tuple(out)
+
+
+def _restrict_1d(n: int, per: bool):
+    """Sparse ``R = P^T`` of one axis (the rule of ``csrc/mg.hip``)."""
+    import scipy.sparse as sp
+
+    nc = n // 2 if per else (n + 1) // 2
+    rows, cols, vals = [], [], []
+    for f in range(n):
+        if f % 2 == 0:
+            srcs = [(f // 2, 1.0)]
+        else:
+            c = (f - 1) // 2
+            if c + 1 < nc:
+                srcs = [(c, 0.5), (c + 1, 0.5)]
+            elif per:
+                srcs = [(c, 0.5), (0, 0.5)]
+            else:
+                srcs = [(c, 1.0)]
+        for c, w in srcs:
+            rows.append(c)
+            cols.append(f)
+            vals.append(w)
+    return sp.csr_matrix((vals, (rows, cols)), shape=(nc, n))
+
+
+def restrict_host(a: np.ndarray, dims, periodic) -> np.ndarray:
+    """``R a`` for a grid raveled ``[z, y, x]`` of dims ``(nx, ny, nz)``."""
+    nx, ny, nz = dims
+    g = np.asarray(a, dtype=np.float64).reshape(nz, ny, nx)
+    Rx, Ry, Rz = (_restrict_1d(n, bool(p)) for n, p in zip(dims, periodic))
+    g = (Rx @ g.reshape(-1, nx).T).T.reshape(nz, ny, -1)
+    g = np.einsum("cy,zyx->zcx", Ry.toarray(), g)
+    g = np.einsum("cz,zyx->cyx", Rz.toarray(), g)
+    return g.ravel()
+
+
+class MultilevelCNN:
+    """The multilevel learned preconditioner
+
+        z = D^-1 r + S sum_l Q_l C_l^T C_l Q_l^T S r,
+
+    one symmetric masked network (``MaskedCNN``) per level of the background
+    grid hierarchy (level l + 1 keeps every other point of level l; Q_l
+    prolongates level l to the background grid, Q_l^T restricts). Each level's
+    mask holds the voxels that the restriction of the unknowns reaches. The
+    coarse levels see the smooth error that a 3x3x3 stack on the fine grid
+    cannot; the whole map stays symmetric and never below Jacobi.
+
+    ``net_levels``: the levels that carry a network (default: all). The
+    background grid's network does most of the work on these B-spline
+    operators, whose slow modes under Jacobi are high-frequency coefficient
+    patterns of the p = 3 basis and of the cut cells, not smooth fields; the
+    coarse levels add the smooth part that matters for pure (unscreened)
+    Poisson operators such as the pressure projection.
+    """
+
+    def __init__(self, grid_shape, red2full: np.ndarray, periodic=(False, False, False), levels: int = 5,
+                 net_levels=None, seed: int = 0, device: str = "cuda", init_scale: float = 0.3):
+        import torch
+
+        self.torch = torch
+        self.lib = _lib()
+        self.device = device
+        self.periodic = tuple(bool(p) for p in periodic)
+        self.red2full_np = np.asarray(red2full, dtype=np.int64)
+        self.n = int(self.red2full_np.size)
+        dims = [tuple(int(v) for v in grid_shape)]
+        for _ in range(1, levels):
+            try:
+                nxt = coarse_dims(dims[-1], self.periodic)
+            except ValueError:
+                break
+            if min(nxt) < 2:
+                break
+            dims.append(nxt)
+        self.dims = dims
+        self.levels = len(dims)
+        if net_levels is None:
+            net_levels = list(range(self.levels))
+        self.net_levels = sorted(int(l) for l in net_levels if l < self.levels)
+        if not self.net_levels or self.net_levels[-1] != self.levels - 1:
+            raise ValueError("the coarsest level must carry a network")
+        masks = [np.zeros(int(np.prod(dims[0])), dtype=np.float64)]
+        masks[0][self.red2full_np] = 1.0
+        for l in range(1, self.levels):
+            masks.append((restrict_host(masks[-1], dims[l - 1], self.periodic) > 0).astype(np.float64))
+        self.masks = [m.astype(np.uint8) for m in masks]
+        self.nets = {}
+        for l in self.net_levels:
+            self.nets[l] = MaskedCNN(dims[l], self.red2full_np if l == 0 else None, self.periodic, seed=seed + 17 * l,
+                                     device=device, init_scale=init_scale, mask=self.masks[l])
+        f32 = lambda nvox: torch.zeros(nvox, dtype=torch.float32, device=device)
+        self.u = [f32(int(np.prod(d))) for d in dims]
+        self.t = [f32(int(np.prod(d))) for d in dims]
+        self.g = [f32(int(np.prod(d))) for d in dims]
+        self.red2full = torch.from_numpy(self.red2full_np.astype(np.int32)).to(device)
+        self.part = torch.zeros(2 * 1024, dtype=torch.float64, device=device)
+        self._cdims = [(ctypes.c_int32 * 3)(*d) for d in dims]
+        self._cper = (ctypes.c_int32 * 3)(*[int(p) for p in self.periodic])
+        self._desc = None
+
+    # -- descriptor -------------------------------------------------------------
+    def desc(self) -> MlDesc:
+        d = MlDesc()
+        d.levels = self.levels
+        for l in range(self.levels):
+            nx, ny, nz = self.dims[l]
+            if l in self.nets:
+                d.net[l] = self.nets[l].desc()
+                d.has_net[l] = 1
+            else:
+                d.net[l].nx, d.net[l].ny, d.net[l].nz = nx, ny, nz
+                for k in range(3):
+                    d.net[l].periodic[k] = int(self.periodic[k])
+                d.has_net[l] = 0
+        d.net[0].red2full = self.red2full.data_ptr()
+        d.net[0].n_red = self.n
+        self._desc = d
+        return d
+
+    # -- hierarchy transfers ------------------------------------------------------
+    def _restrict(self, l, fine, fs, coarse, cs):
+        hip._check(self.lib.cc_mg_restrict(self._cdims[l - 1], self._cdims[l], self._cper, hip._ptr(fine), fs,
+                                           hip._ptr(coarse), cs, hip._stream()), "mg_restrict")
+
+    def _prolong(self, l, coarse, cs, add, as_, fine):
+        hip._check(self.lib.cc_mg_prolong(self._cdims[l - 1], self._cdims[l], self._cper, hip._ptr(coarse), cs,
+                                          hip._ptr(add) if add is not None else None, as_, hip._ptr(fine),
+                                          hip._stream()), "mg_prolong")
+
+    def _to_ch0(self, l, src, grid):
+        hip._check(self.lib.cc_mg_to_ch0(int(np.prod(self.dims[l])), hip._ptr(src), 1, hip._ptr(grid),
+                                         hip._stream()), "mg_to_ch0")
+
+    def _forward_levels(self):
+        """u[0] holds the fine input; fills every level's network output and
+        returns (tensor, stride) of the summed correction on the fine grid."""
+        for l in range(1, self.levels):
+            self._restrict(l, self.u[l - 1], 1, self.u[l], 1)
+        for l, net in self.nets.items():
+            self._to_ch0(l, self.u[l], net.acts[0])
+            net.forward()
+        L = self.levels
+        src, ss = self.nets[L - 1].y, 16
+        for l in range(L - 1, 0, -1):
+            add = self.nets[l - 1].y if (l - 1) in self.nets else None
+            self._prolong(l, src, ss, add, 16, self.t[l - 1])
+            src, ss = self.t[l - 1], 1
+        return src, ss
+
+    # -- training -----------------------------------------------------------------
+    def sample_loss_grad(self, r32, e32, diag, scale: float, gscale: float, accumulate: bool,
+                         e_norm2: float):
+        """Forward + backward for one (r, e) pair; returns its loss (device scalar)."""
+        p = hip._ptr
+        s = hip._stream()
+        hip._check(self.lib.cc_mg_train_input(self.n, p(self.red2full), p(r32), p(diag), float(scale), p(self.u[0]),
+                                              s), "mg_train_input")
+        t0, ts = self._forward_levels()
+        if ts != 1:  # a single level: channel 0 of its output
+            self.t[0].copy_(t0.view(-1, 16)[:, 0])
+            t0 = self.t[0]
+        t2 = scale * scale * e_norm2
+        nparts = ctypes.c_int32()
+        hip._check(self.lib.cc_mg_train_loss(self.n, p(self.red2full), p(r32), p(e32), p(diag), float(scale), p(t0),
+                                             float(1.0 / t2), float(gscale), p(self.g[0]), p(self.part),
+                                             ctypes.addressof(nparts), s), "mg_train_loss")
+        for l in range(1, self.levels):
+            self._restrict(l, self.g[l - 1], 1, self.g[l], 1)
+        for l, net in self.nets.items():
+            self._to_ch0(l, self.g[l], net.gout)
+            net.backward(1.0, accumulate)
+        parts = self.part[: 2 * nparts.value].view(-1, 2).sum(dim=0)
+        return parts[0] / parts[1]
+
+    def adam_step(self, lr: float):
+        for net in self.nets.values():
+            net.adam_step(lr)
+
+    def train(self, R, Eerr, diag, *, steps: int = 200, batch: int = 8, lr: float = 2e-3, seed: int = 0,
+              log_every: int = 0) -> TrainReport:
+        """Adam on mini-batches of (residual, error) pairs (float32, [S, n])."""
+        torch = self.torch
+        rng = np.random.default_rng(seed)
+        S = R.shape[0]
+        scales, enorm2 = MaskedCNN._sample_scales(R, Eerr, diag)
+        # The fine input buffer is written only at the unknowns' voxels.
+        self.u[0].zero_()
+        self.g[0].zero_()
+        hist = []
+        t0 = time.perf_counter()
+        for step in range(steps):
+            idx = rng.choice(S, size=min(batch, S), replace=False)
+            losses = [self.sample_loss_grad(R[i], Eerr[i], diag, float(scales[i]), 1.0 / len(idx), k > 0,
+                                            float(enorm2[i])) for k, i in enumerate(idx)]
+            self.adam_step(lr)
+            hist.append(float(torch.stack(losses).mean().item()))
+            if log_every and step % log_every == 0:
+                print(f"[mlcnn] step {step} loss {hist[-1]:.4f}", flush=True)
+        torch.cuda.synchronize()
+        return TrainReport(steps=steps, batch=batch, loss_first=hist[0], loss_last=hist[-1],
+                           seconds=time.perf_counter() - t0, history=hist)
+
+    # -- application ----------------------------------------------------------------
+    def precondition(self, r, diag):
+        torch = self.torch
+        d = self.desc()
+        nb = int(self.lib.cc_mlcnn_precondition_workspace_bytes(ctypes.addressof(d)))
+        if nb == 0:
+            raise ValueError("invalid multilevel descriptor")
+        ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        z = torch.empty_like(r)
+        hip._check(self.lib.cc_mlcnn_precondition(ctypes.addressof(d), hip._ptr(r), hip._ptr(diag), hip._ptr(z),
+                                                  hip._ptr(ws), nb, hip._stream()), "mlcnn_precondition")
+        return z
+
+    def fcg(self, dmf, b, *, tol=1.0e-8, maxiter=5000, check_every=16, history=False, x=None):
+        """Flexible CG with this preconditioner on a ``DeviceMatrixFree`` operator."""
+        torch = self.torch
+        d = self.desc()
+        nb = int(self.lib.cc_fcg_sumfact_ml_workspace_bytes(ctypes.addressof(dmf.desc), ctypes.addressof(d)))
+        if nb == 0:
+            raise ValueError("invalid descriptors")
+        ws = getattr(self, "_fcg_ws", None)
+        if ws is None or ws.numel() < nb:
+            ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            self._fcg_ws = ws
+        x = torch.empty_like(b) if x is None else x
+        res = hip.PcgResult()
+        hist = torch.full((int(maxiter) + 1,), float("nan"), dtype=torch.float64, device=b.device) if history else None
+        hip._check(self.lib.cc_fcg_sumfact_ml(ctypes.addressof(dmf.desc), ctypes.addressof(d), hip._ptr(b),
+                                              hip._ptr(x), float(tol), int(maxiter), int(check_every), hip._ptr(ws),
+                                              nb, ctypes.addressof(res),
+                                              hip._ptr(hist) if hist is not None else None, hip._stream()),
+                   "fcg_sumfact_ml")
+        if history:
+            return x, res, hist.cpu().numpy()
+        return x, res
+
+
 def torch_where_pos_rsqrt(d):
     """1/sqrt(d) where d > 0, else 1 (the kernels' symmetric scaling rule)."""
     import torch
@@ -345,8 +619,9 @@ def generate_pairs(dmf, rhs, snapshots, tol_ref: float = 1.0e-11):
     return torch.stack(Rs), torch.stack(Es)
 
 
-def solve_learned_with_fallback(net: MaskedCNN, dmf, b, *, tol=1.0e-8, maxiter=5000):
-    """Flexible CG with the network; Jacobi-PCG when it does not converge."""
+def solve_learned_with_fallback(net, dmf, b, *, tol=1.0e-8, maxiter=5000):
+    """Flexible CG with the network (``MaskedCNN`` or ``MultilevelCNN``);
+    Jacobi-PCG when it does not converge."""
     x, res = net.fcg(dmf, b, tol=tol, maxiter=maxiter)
     ok = res.status == hip.CC_CONVERGED and res.true_rel <= max(10.0 * tol, 1e-14)
     if ok:

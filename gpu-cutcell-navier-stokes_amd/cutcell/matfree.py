@@ -309,6 +309,8 @@ class SumfactDesc(ctypes.Structure):
         ("et_small", ctypes.c_void_p),
         ("et_w", ctypes.c_void_p),
         ("diag", ctypes.c_void_p),
+        ("n_et_rows", ctypes.c_int32),
+        ("et_rows", ctypes.c_void_p),
     ]
 
 
@@ -366,6 +368,9 @@ class DeviceMatrixFree:
         d.et_small = t(EsT.indices if EsT.nnz else np.zeros(1), np.int32)
         d.et_w = t(EsT.data if EsT.nnz else np.zeros(1), np.float64)
         d.diag = t(mf.diag, np.float64)
+        et_rows = np.nonzero(np.diff(EsT.indptr))[0]
+        d.n_et_rows = et_rows.size
+        d.et_rows = t(et_rows if et_rows.size else np.zeros(1), np.int32)
         self.desc = d
         self._keep = keep
         self.n = mf.interior.size

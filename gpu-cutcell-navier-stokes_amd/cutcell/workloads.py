@@ -10,8 +10,11 @@
   times p in {3, 4, 5} on 48x48 elements, plus 25 p = 4 slivers with cut-cell
   volume fraction 10^U[-5, -1] (the sliver recipe of
   ``code/fast_solver_workbench.py:138-168``).
-- ``c4_assembly``: 3D box, p = 3, 64^3 elements, sphere obstacle with radius
-  U[0.2, 0.4] and a centre keeping the ball inside.
+- ``c4_operator``: 3D box, p = 3, 64^3 elements, sphere obstacle with radius
+  U[0.2, 0.4] and a centre keeping the ball inside (matrix-free).
+- ``c5_operator``: channel 128 x 64 x 64 elements, p = 3, periodic in x and z
+  (extents 9 x 3.036 x 4.5), lower wall cut by a cosine bump of height 1 over
+  length 4 (matrix-free; ``mass_coef = 0`` for pressure projection).
 """
 
 from __future__ import annotations
@@ -61,12 +64,14 @@ def c5_grid(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3, lengths=
                    Axis(nz, degree, lengths[2], True)))
 
 
-def c5_operator(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3):
-    """Pressure-projection operator on the bump channel, matrix-free."""
+def c5_operator(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3, mass_coef: float = 1.0):
+    """Bump-channel operator ``E^T (K + c M) E``, matrix-free; ``mass_coef = 0``
+    is the pressure-projection operator ``E^T K E`` (singular: the constants
+    are its null space, so right-hand sides must sum to zero)."""
     from .matfree import build_matrix_free
 
-    mf = build_matrix_free(c5_grid(nx, ny, nz, degree), bump_levelset())
-    mf.meta.update(kind="bump_channel", shape=[nx, ny, nz])
+    mf = build_matrix_free(c5_grid(nx, ny, nz, degree), bump_levelset(), mass_coef=mass_coef)
+    mf.meta.update(kind="bump_channel", shape=[nx, ny, nz], mass_coef=mass_coef)
     return mf
 
 

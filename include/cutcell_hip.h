@@ -84,8 +84,10 @@ int cc_profile_collect(void);
  * lane, loaded as one vector (1, 2 or 4 with at most 16 bytes; 0 = widest),
  * nz_lanes nonzero slots per row team ((k/cols_per_lane)*nz_lanes <= 64,
  * power of two; 0 = from the mean row length), grid blocks (0 = one block per
- * few rows up to 1024), unroll 1 or 4 (0 = 4), stream_loads 1 = nontemporal
- * loads of the matrix arrays (not with unroll 1), 0 = ordinary loads. */
+ * few rows up to 1024), unroll 1, 4 or 8 (8 = software-pipelined: the next
+ * entries' indices and values load while the current X rows are gathered;
+ * 0 = 4), stream_loads 1 = nontemporal loads of the matrix arrays (not with
+ * unroll 1), 0 = ordinary loads. */
 int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
                        int32_t cols_per_lane, int32_t stream_loads);
 
@@ -181,7 +183,8 @@ int cc_pcg_batched(int32_t nsys, int32_t n_max, const int32_t* sys_n, const int6
  *  - red2full[n_red]: interior dof of each unknown; full2red/full2small:
  *    inverse maps (-1 elsewhere); small_ptr/small_col/small_w: rows of E for
  *    the n_small small dofs (small_dof), columns are unknowns; et_ptr/
- *    et_small/et_w: the same entries grouped by unknown (E^T);
+ *    et_small/et_w: the same entries grouped by unknown (E^T), et_rows the
+ *    unknowns that have any;
  *  - diag[n_red]: diag(E^T A E) for the Jacobi preconditioner. */
 typedef struct cc_sumfact_desc {
   int32_t nb[3];
@@ -214,6 +217,8 @@ typedef struct cc_sumfact_desc {
   const int32_t* et_small;
   const double* et_w;
   const double* diag;
+  int32_t n_et_rows;      /* unknowns with E^T entries (et_ptr[j + 1] > et_ptr[j]) */
+  const int32_t* et_rows; /* their indices, ascending */
 } cc_sumfact_desc;
 
 /* Workspace of cc_sumfact_apply (and, with solve != 0, of cc_pcg_sumfact). */
@@ -301,6 +306,53 @@ size_t cc_fcg_sumfact_workspace_bytes(const cc_sumfact_desc* d, const cc_cnn_des
 int cc_fcg_sumfact(const cc_sumfact_desc* d, const cc_cnn_desc* net, const double* b, double* x, double tol,
                    int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
                    cc_pcg_result* result, double* history, cc_stream_t stream);
+
+/* ---- Multilevel learned preconditioner (no reference counterpart). ------
+ * Level 0 is the background dof grid; level l + 1 keeps every other grid
+ * point of level l (clamped axis: n_c = (n_f + 1) / 2, coarse c on fine 2c;
+ * periodic axis: n_f even, n_c = n_f / 2). P is tensor-product linear
+ * interpolation and the restriction is exactly R = P^T. With Q_l = P_1 ... P_l
+ * and C_l the symmetric network of level l (four layers as above),
+ *     z = D^-1 r + S sum_{l with a net} Q_l C_l^T C_l Q_l^T S r,
+ * symmetric and never below Jacobi. net[l] gives every level's grid and
+ * periodic flags (periodic flags equal on all levels); levels with has_net[l]
+ * also give weights and mask; net[0].red2full / n_red are the unknowns. The
+ * coarsest level must carry a network. */
+#define CC_ML_MAX_LEVELS 8
+typedef struct cc_mlcnn_desc {
+  int32_t levels;
+  int32_t has_net[CC_ML_MAX_LEVELS];
+  cc_cnn_desc net[CC_ML_MAX_LEVELS];
+} cc_mlcnn_desc;
+
+/* coarse = R fine, fine = P coarse (+ add) on single-channel float32 grids of
+ * dims nf[3] -> nc[3] (x fastest), elements read/written with the given
+ * strides (16 reads channel 0 of a [voxel][16] tensor). */
+int cc_mg_restrict(const int32_t* nf, const int32_t* nc, const int32_t* periodic, const float* fine,
+                   int32_t fine_stride, float* coarse, int32_t coarse_stride, cc_stream_t stream);
+int cc_mg_prolong(const int32_t* nf, const int32_t* nc, const int32_t* periodic, const float* coarse,
+                  int32_t coarse_stride, const float* add, int32_t add_stride, float* fine, cc_stream_t stream);
+/* grid[v][0] = bf16(src[v * src_stride]) for every voxel (other channels untouched). */
+int cc_mg_to_ch0(int64_t nvox, const float* src, int32_t src_stride, uint16_t* grid, cc_stream_t stream);
+/* Training helpers of the multilevel network: u0[voxel of j] = scale * S r_j
+ * (float32, other voxels untouched), and the loss of cc_cnn_train_loss with
+ * the summed correction t0 (single-channel float32 on the background grid)
+ * in place of y; its gradient dL/dt0 (times gscale) goes to g0 (float32). */
+int cc_mg_train_input(int32_t n, const int32_t* red2full, const float* r, const double* diag, float scale, float* u0,
+                      cc_stream_t stream);
+int cc_mg_train_loss(int32_t n, const int32_t* red2full, const float* r, const float* e, const double* diag,
+                     float scale, const float* t0, float inv_den, float gscale, float* g0, double* partial,
+                     int32_t* nparts, cc_stream_t stream);
+/* z = M r with the multilevel preconditioner, float64 in/out. */
+size_t cc_mlcnn_precondition_workspace_bytes(const cc_mlcnn_desc* ml);
+int cc_mlcnn_precondition(const cc_mlcnn_desc* ml, const double* r, const double* diag, double* z, void* workspace,
+                          size_t workspace_bytes, cc_stream_t stream);
+/* Flexible CG with the multilevel preconditioner on the matrix-free
+ * operator; semantics as cc_fcg_sumfact. Synchronises. */
+size_t cc_fcg_sumfact_ml_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml);
+int cc_fcg_sumfact_ml(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, const double* b, double* x, double tol,
+                      int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
+                      cc_pcg_result* result, double* history, cc_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -203,3 +203,83 @@ def sym_fd_check(r, e, d, W0, W1, mask, n_probe=6, h=1e-6, seed=0, periodic=(Fal
         an = grads[i][idx]
         worst = max(worst, abs(fd - an) / max(abs(fd), abs(an), 1e-12))
     return worst
+
+
+# ---------------------------------------------------------------------------
+# Multilevel form: grid hierarchy, prolongation, and the summed preconditioner.
+# ---------------------------------------------------------------------------
+
+
+def coarse_len(n: int, periodic: bool) -> int:
+    """Points of the next level: every other point (periodic: n even)."""
+    if periodic:
+        assert n % 2 == 0
+        return n // 2
+    return (n + 1) // 2
+
+
+def prolong_1d(n: int, periodic: bool) -> np.ndarray:
+    """Dense ``[n, n_c]`` linear interpolation: fine 2c <- c (1), fine
+    2c + 1 <- c, c + 1 (1/2 each; c + 1 wraps when periodic; past the end of
+    a clamped axis, c alone with weight 1). Written from the definition,
+    entry by entry."""
+    nc = coarse_len(n, periodic)
+    P = np.zeros((n, nc))
+    for f in range(n):
+        if f % 2 == 0:
+            P[f, f // 2] = 1.0
+        else:
+            c = (f - 1) // 2
+            if c + 1 < nc:
+                P[f, c] += 0.5
+                P[f, c + 1] += 0.5
+            elif periodic:
+                P[f, c] += 0.5
+                P[f, 0] += 0.5
+            else:
+                P[f, c] += 1.0
+    return P
+
+
+def prolong_3d(dims, periodic):
+    """``P`` on grids raveled ``[z, y, x]`` (dims = (nx, ny, nz)): kron(Pz, Py, Px)."""
+    Px, Py, Pz = (prolong_1d(n, p) for n, p in zip(dims, periodic))
+    return np.kron(Pz, np.kron(Py, Px))
+
+
+def ml_network(r, d, levels, periodic=(False, False, False)):
+    """``z = D^-1 r + S sum_l Q_l C_l^T C_l Q_l^T S r`` on the background grid.
+
+    ``levels``: list over grids 0..L-1 of ``(dims, W0, W1, mask)`` or
+    ``(dims, None, None, None)`` for a level without a network; ``r, d`` are
+    ``[z, y, x]`` grids of level 0 (zero off the unknowns). The level
+    network is the symmetric one of :func:`sym_network` with unit diagonal,
+    i.e. ``C^T C`` applied to channel 0.
+    """
+    mask0 = levels[0][3] if levels[0][3] is not None else (d != 0).astype(float)
+    s = _inv_sqrt(d) * (d != 0)
+    u = [(s * r).ravel()]
+    Ps = [None]
+    for l in range(1, len(levels)):
+        P = prolong_3d(levels[l - 1][0], periodic)
+        Ps.append(P)
+        u.append(P.T @ u[-1])
+    t = np.zeros_like(u[0])
+    for l in range(len(levels) - 1, -1, -1):
+        dims, W0, W1, mask = levels[l]
+        y = np.zeros(int(np.prod(dims)))
+        if W0 is not None:
+            shape = (dims[2], dims[1], dims[0])
+            x = np.zeros(shape + (16,))
+            x[..., 0] = u[l].reshape(shape)
+            a = x
+            for W in sym_layers(W0, W1):
+                a, _ = conv3(a, W, mask, relu=False, periodic=periodic)
+            y = a[..., 0].ravel()
+        if l == len(levels) - 1:
+            t = y
+        else:
+            t = Ps[l + 1] @ t + y
+    dinv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    del mask0
+    return dinv * r + s * t.reshape(r.shape)

@@ -64,6 +64,8 @@ def test_spmm_config_is_validated_per_vector_width():
         assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 1) == 0
         assert lib.cc_spmm_set_config(16, 8, 8, 512, 1, 2, 1) == -1  # nontemporal needs unroll 4
         assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 2) == -1
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 8, 2, 1) == 0
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 2, 2, 0) == -1
     finally:
         for k in hip.SUPPORTED_K:
             for vb in (8, 4):

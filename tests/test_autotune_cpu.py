@@ -10,7 +10,7 @@ def test_spmm_candidates_respect_the_wave():
             assert cands
             for c in cands:
                 assert (k // c["cpl"]) * c["nz"] <= 64 and c["cpl"] * vb <= 16
-                assert c["grid"] in (256, 512, 1024) and c["unroll"] in (1, 4)
+                assert c["grid"] in (256, 512, 1024) and c["unroll"] in (1, 4, 8)
 
 
 def test_batched_candidates_fit_lds():

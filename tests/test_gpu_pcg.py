@@ -76,6 +76,50 @@ def test_spmm_columns(hip, systems, k):
     np.testing.assert_allclose(Y, ref, rtol=0, atol=1e-13 * np.abs(ref).max())
 
 
+def _ragged(n=700, seed=5):
+    """Rows of length 0 to ~200 (empty rows included), random columns."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 200, n)
+    lens[::37] = 0
+    rows = np.repeat(np.arange(n), lens)
+    cols = rng.integers(0, n, rows.size)
+    A = sp.csr_matrix((rng.standard_normal(rows.size), (rows, cols)), shape=(n, n))
+    A.sum_duplicates()
+    return A
+
+
+@pytest.mark.parametrize("k,vb", [(1, 8), (16, 8), (16, 4), (4, 4)])
+def test_spmm_every_launch_shape(hip, systems, k, vb):
+    """Each launch shape the autotuner may pick (vector width, nonzero slots,
+    grid, unroll 1/4/8, nontemporal loads) against the oracle SpMV, on a
+    B-spline system and on a ragged matrix with empty rows."""
+    from cutcell.autotune import spmm_candidates
+
+    lib = hip.load_library()
+    mats = {"sliver-16": systems["sliver-16"].A, "ragged": _ragged()}
+    try:
+        for name, A in mats.items():
+            Ad = hip.DeviceCSR.from_scipy(A, with_f32=(vb == 4))
+            X = np.random.default_rng(k).standard_normal((A.shape[0], k))
+            if vb == 4:
+                X = X.astype(np.float32)
+            A_ref = A.astype(np.float32).astype(np.float64) if vb == 4 else A
+            ref = np.stack([oracle.spmv(A_ref.tocsr(), X[:, c].astype(np.float64)) for c in range(k)], axis=1)
+            scale = (np.abs(A_ref) @ np.abs(X.astype(np.float64))).max()
+            tol = 1e-5 if vb == 4 else 1e-13
+            seen = 0
+            for cand in spmm_candidates(k, A.nnz / A.shape[0], vb):
+                if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"],
+                                          cand["stream"]) != 0:
+                    continue
+                Y = hip.spmm(Ad, _dev(X)).cpu().numpy() if k > 1 else hip.spmv(Ad, _dev(X[:, 0])).cpu().numpy()[:, None]
+                assert np.abs(Y - ref).max() <= tol * scale, (name, cand)
+                seen += 1
+            assert seen >= 5
+    finally:
+        lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0)
+
+
 def test_jacobi_diag(hip, systems):
     A = systems["C1-oblique-5deg"].A
     d = hip.jacobi_diag(hip.DeviceCSR.from_scipy(A)).cpu().numpy()
