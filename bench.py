@@ -131,6 +131,35 @@ def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int, tune: 
     )
 
 
+def spmv_bandwidth(hip, torch, Ad, reps: int = 20):
+    """The single-vector CSR SpMV of the path (``A_gpu @ p`` of the reference
+    loop) on the workload matrix, autotuned per precision: algorithmic bytes
+    ``nnz (s_v + 4) + 4 (n + 1) + 2 n s_x`` over the mean launch time (HIP
+    events on the launch stream)."""
+    from cutcell.autotune import spmm_config
+
+    out = {}
+    for prec, dt, vb in (("fp64", torch.float64, 8), ("fp32", torch.float32, 4)):
+        cfg = spmm_config(Ad, 1, "fp64" if prec == "fp64" else "mixed")
+        x = torch.randn(Ad.n, dtype=dt, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            hip.spmv(Ad, x, y)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            hip.spmv(Ad, x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        nbytes = Ad.nnz * (vb + 4) + 4 * (Ad.n + 1) + 2 * Ad.n * vb
+        out[prec] = {"avg_us": sig(us), "bytes": int(nbytes), "GBps": sig(nbytes / (us * 1e-6) / 1e9),
+                     "frac_of_8TBps": sig(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS),
+                     "launch": {k: v for k, v in cfg.items() if k != "ms"}}
+    return out
+
+
 def cpu_baseline(system, b: np.ndarray, budget_s: float = 30.0):
     """Oracle scalar C port of the reference loop, one core, one right-hand side."""
     import oracle
@@ -443,6 +472,10 @@ def main(argv=None) -> int:
                     kernel=f"k_spmm<{'double' if value_bytes == 8 else 'float'},k=16>",
                     bytes_per_launch=traffic_alg, avg_us=round(head["spmm_avg_us"], 2))
 
+    log("single-vector SpMV bandwidth on the workload matrix ...")
+    spmv = spmv_bandwidth(hip, torch, Ad)
+    log(f"spmv: {spmv}")
+
     batched = None
     if not args.no_batched:
         log("batched section: 241 uneven 2D systems in one persistent launch ...")
@@ -493,6 +526,7 @@ def main(argv=None) -> int:
             for m, r in runs.items()
         },
         "roofline": roofline,
+        "spmv_k1": spmv,
         "cpu_baseline": cpu,
         "batched_uneven_2d": batched,
         "learned_sphere_3d": learned,

@@ -63,100 +63,85 @@ struct GridArgs {
   int px, py, pz;  // periodic flags
 };
 
-// Forward (and data-gradient) layer. A wave owns one (y, z) row of the grid
-// and sweeps it in groups of 16 consecutive x voxels; ACT: 0 none, 1 relu.
-// gate (optional, bf16 [voxel][16]): multiply the result by (gate > 0), the
-// ReLU derivative in backward mode.
-// The block stages the layer's weights in LDS transposed to [tap][co][ci],
-// so that a lane's MFMA B operand of a tap (4 consecutive input channels of
-// one output channel) is one 8-byte read; every wave keeps all 27 B operands
-// in registers for its whole row.
+// Forward (and data-gradient) layer. One wave = 16 consecutive x voxels of
+// one (y, z) row; ACT: 0 none, 1 relu. gate (optional, bf16 [voxel][16]):
+// multiply the result by (gate > 0), the ReLU derivative in backward mode.
+// Every load is unconditional (out-of-grid taps read a clamped in-grid
+// voxel and are zeroed by a select), so the 27 taps' weight and input loads
+// are all in flight together instead of forming a chain of branches.
 template <int ACT>
 __global__ __launch_bounds__(256) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
                                                const uint16_t* __restrict__ W, const uint8_t* __restrict__ mask,
                                                const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
                                                float* __restrict__ out_f32) {
-  __shared__ __align__(16) uint16_t wl[27 * 16 * 16];
   __shared__ float tile[4][16][17];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int row = lane % 16;  // voxel within a group (A row / D col index)
+  const int row = lane % 16;  // voxel within the wave's 16 (A row / D col index)
   const int kg = lane / 16;   // k group: channels 4 kg .. 4 kg + 3
-  for (int i = threadIdx.x; i < 27 * 16 * 16; i += blockDim.x) {
-    const int t = i / 256, ci = (i / 16) % 16, co = i % 16;  // W[t][ci][co]
-    wl[(t * 16 + co) * 16 + ci] = W[i];
-  }
-  __syncthreads();
-  v4s bw[27];
-#pragma unroll
-  for (int t = 0; t < 27; ++t) bw[t] = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
-  const long long rows = (long long)g.ny * g.nz;
-  const long long wave_id = (long long)blockIdx.x * 4 + wv;
-  const bool valid = wave_id < rows;
-  const int y = valid ? (int)(wave_id % g.ny) : 0;
-  const int z = valid ? (int)(wave_id / g.ny) : 0;
-  // Row offsets of the 9 (dz, dy) neighbours, -1 where outside a clamped axis.
-  long long roff[9];
-#pragma unroll
-  for (int q = 0; q < 9; ++q) {
-    const int zz = nb_index(z + q / 3 - 1, g.nz, g.pz);
-    const int yy = nb_index(y + q % 3 - 1, g.ny, g.py);
-    roff[q] = (valid && zz >= 0 && yy >= 0) ? ((long long)zz * g.ny + yy) * g.nx : -1;
-  }
   const int xt = (g.nx + 15) / 16;
-  for (int grp = 0; grp < xt; ++grp) {
-    const int xb = grp * 16;
-    const int x = xb + row;
-    int xx[3];
+  const long long wave_id = (long long)blockIdx.x * 4 + wv;
+  const long long n_waves = (long long)xt * g.ny * g.nz;
+  const bool valid = wave_id < n_waves;
+  const long long wid = valid ? wave_id : 0;
+  const int xb = (int)(wid % xt) * 16;
+  const int y = (int)((wid / xt) % g.ny);
+  const int z = (int)(wid / ((long long)xt * g.ny));
+  const int x = xb + row;
+  int zi[3], yi[3], xi[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) xx[k] = (x < g.nx) ? nb_index(x + k - 1, g.nx, g.px) : -1;
-    v4f acc = {0.f, 0.f, 0.f, 0.f};
+  for (int k = 0; k < 3; ++k) {
+    zi[k] = nb_index(z + k - 1, g.nz, g.pz);
+    yi[k] = nb_index(y + k - 1, g.ny, g.py);
+    xi[k] = (x < g.nx) ? nb_index(x + k - 1, g.nx, g.px) : -1;
+  }
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < 27; ++t) {
-      const long long ro = roff[t / 3];
-      const int xi = xx[t % 3];
-      v4s a = {0, 0, 0, 0};
-      if (ro >= 0 && xi >= 0) {
-        const uint2 raw = *reinterpret_cast<const uint2*>(in + (size_t)(ro + xi) * 16 + 4 * kg);
-        a[0] = (short)(raw.x & 0xffff);
-        a[1] = (short)(raw.x >> 16);
-        a[2] = (short)(raw.y & 0xffff);
-        a[3] = (short)(raw.y >> 16);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[t], acc, 0, 0, 0);
+  for (int t = 0; t < 27; ++t) {
+    v4s b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b[i] = (short)W[(t * 16 + 4 * kg + i) * 16 + row];
+    const int zz = zi[t / 9], yy = yi[(t / 3) % 3], xx = xi[t % 3];
+    const bool ok = valid && zz >= 0 && yy >= 0 && xx >= 0;
+    const size_t v = ok ? ((size_t)zz * g.ny + yy) * g.nx + xx : 0;
+    const uint2 raw = *reinterpret_cast<const uint2*>(in + v * 16 + 4 * kg);
+    v4s a;
+    a[0] = ok ? (short)(raw.x & 0xffff) : (short)0;
+    a[1] = ok ? (short)(raw.x >> 16) : (short)0;
+    a[2] = ok ? (short)(raw.y & 0xffff) : (short)0;
+    a[3] = ok ? (short)(raw.y >> 16) : (short)0;
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  // acc[i] = D[voxel 4 kg + i][co = row]; transpose through LDS so that each
+  // lane stores 4 consecutive channels of one voxel. Every wave of the block
+  // reaches the barrier.
+#pragma unroll
+  for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
+  __syncthreads();
+  const int vx = xb + row;
+  if (valid && vx < g.nx) {
+    const size_t v = ((size_t)z * g.ny + y) * g.nx + vx;
+    const bool live = mask[v] != 0;
+    uint16_t h[4];
+    float f[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      float val = tile[wv][row][4 * kg + i];
+      if (ACT == 1) val = val > 0.f ? val : 0.f;
+      if (gate) val = bf2f(gate[v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
+      if (!live) val = 0.f;
+      f[i] = val;
+      h[i] = f2bf(val);
     }
-    // acc[i] = D[voxel 4 kg + i][co = row]; transpose through LDS so that each
-    // lane stores 4 consecutive channels of one voxel. Every wave of the block
-    // runs the same number of groups and reaches both barriers.
-    __syncthreads();
+    if (out) {
+      uint2 raw;
+      raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+      raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+      *reinterpret_cast<uint2*>(out + v * 16 + 4 * kg) = raw;
+    }
+    if (out_f32) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
-    __syncthreads();
-    const int vx = xb + row;
-    if (valid && vx < g.nx) {
-      const size_t v = (size_t)(((long long)z * g.ny + y) * g.nx + vx);
-      const bool live = mask[v] != 0;
-      uint16_t h[4];
-      float f[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float val = tile[wv][row][4 * kg + i];
-        if (ACT == 1) val = val > 0.f ? val : 0.f;
-        if (gate) val = bf2f(gate[v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
-        if (!live) val = 0.f;
-        f[i] = val;
-        h[i] = f2bf(val);
-      }
-      if (out) {
-        uint2 raw;
-        raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-        raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-        *reinterpret_cast<uint2*>(out + v * 16 + 4 * kg) = raw;
-      }
-      if (out_f32) {
-#pragma unroll
-        for (int i = 0; i < 4; ++i) out_f32[v * 16 + 4 * kg + i] = f[i];
-      }
+      for (int i = 0; i < 4; ++i) out_f32[v * 16 + 4 * kg + i] = f[i];
     }
   }
 }
@@ -181,24 +166,39 @@ __global__ __launch_bounds__(256) void k_conv3_wgrad(GridArgs g, const uint16_t*
     const int xb = (int)(r % xt) * 16;
     const int y = (int)((r / xt) % g.ny);
     const int z = (int)(r / ((long long)xt * g.ny));
-    // B operand: gout[voxel xb + 4 kg + i][co = m].
+    // B operand: gout[voxel xb + 4 kg + i][co = m]. Loads are unconditional
+    // (a clamped in-grid address, zeroed by a select) so they all overlap.
     v4s bb;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       const int x = xb + 4 * kg + i;
-      bb[i] = (x < g.nx) ? (short)gout[(((size_t)z * g.ny + y) * g.nx + x) * 16 + m] : (short)0;
+      const bool ok = x < g.nx;
+      const short v = (short)gout[(((size_t)z * g.ny + y) * g.nx + (ok ? x : 0)) * 16 + m];
+      bb[i] = ok ? v : (short)0;
+    }
+    int zi[3], yi[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      zi[k] = nb_index(z + k - 1, g.nz, g.pz);
+      yi[k] = nb_index(y + k - 1, g.ny, g.py);
+    }
+    int xi[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int x = xb + 4 * kg + i;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) xi[i][k] = (x < g.nx) ? nb_index(x + k - 1, g.nx, g.px) : -1;
     }
 #pragma unroll
     for (int t = 0; t < 27; ++t) {
-      const int dz = t / 9 - 1, dy = (t / 3) % 3 - 1, dx = t % 3 - 1;
-      const int zz = nb_index(z + dz, g.nz, g.pz);
-      const int yy = nb_index(y + dy, g.ny, g.py);
+      const int zz = zi[t / 9], yy = yi[(t / 3) % 3];
       v4s aa;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int x = xb + 4 * kg + i;
-        const int xx = (x < g.nx) ? nb_index(x + dx, g.nx, g.px) : -1;
-        aa[i] = (zz >= 0 && yy >= 0 && xx >= 0) ? (short)in[(((size_t)zz * g.ny + yy) * g.nx + xx) * 16 + m] : (short)0;
+        const int xx = xi[i][t % 3];
+        const bool ok = zz >= 0 && yy >= 0 && xx >= 0;
+        const short v = (short)in[((ok ? ((size_t)zz * g.ny + yy) * g.nx + xx : 0)) * 16 + m];
+        aa[i] = ok ? v : (short)0;
       }
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(aa, bb, acc[t], 0, 0, 0);
     }
@@ -320,7 +320,7 @@ __global__ void k_sym_grads(const float* __restrict__ gl, float* __restrict__ gw
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
                        float* out_f32, hipStream_t s) {
   GridArgs g{net.nx, net.ny, net.nz, net.periodic[0], net.periodic[1], net.periodic[2]};
-  const long long waves = (long long)net.ny * net.nz;  // one wave per (y, z) row
+  const long long waves = (long long)((net.nx + 15) / 16) * net.ny * net.nz;
   const unsigned blocks = (unsigned)((waves + 3) / 4);
   uint16_t* acts[4] = {act0, act1, act2, act3};
   for (int l = 0; l < 3; ++l)
@@ -348,7 +348,7 @@ int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* peri
                          uint16_t* out, float* out_f32, cc_stream_t stream) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || !in || !weights || !mask || (!out && !out_f32)) return CC_E_ARG;
   GridArgs g{nx, ny, nz, periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
-  const long long waves = (long long)ny * nz;  // one wave per (y, z) row
+  const long long waves = (long long)((nx + 15) / 16) * ny * nz;
   const long long blocks = (waves + 3) / 4;
   if (blocks > 0x7fffffffLL) return CC_E_ARG;
   hipStream_t s = (hipStream_t)stream;

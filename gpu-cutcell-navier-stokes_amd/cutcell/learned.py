@@ -436,9 +436,13 @@ class MultilevelCNN:
             masks.append((restrict_host(masks[-1], dims[l - 1], self.periodic) > 0).astype(np.float64))
         self.masks = [m.astype(np.uint8) for m in masks]
         self.nets = {}
+        # The restriction sums (1/2 + 1 + 1/2)^3 = 8 fine values per coarse
+        # point, so a smooth input reaches level l amplified by 8^l; starting
+        # level l's first layer 8^(l/2) smaller keeps every level's initial
+        # correction the size of the fine one (training starts near Jacobi).
         for l in self.net_levels:
             self.nets[l] = MaskedCNN(dims[l], self.red2full_np if l == 0 else None, self.periodic, seed=seed + 17 * l,
-                                     device=device, init_scale=init_scale, mask=self.masks[l])
+                                     device=device, init_scale=init_scale * 8.0 ** (-0.5 * l), mask=self.masks[l])
         f32 = lambda nvox: torch.zeros(nvox, dtype=torch.float32, device=device)
         self.u = [f32(int(np.prod(d))) for d in dims]
         self.t = [f32(int(np.prod(d))) for d in dims]
