@@ -44,6 +44,15 @@ for _p in (str(PKG), str(ROOT)):
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the headline SpMM on this workload, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
+# profiles/spmm_planar81k_p3_k16.md (2.0 FETCH + 1.0 WRITE, calibrated on a
+# 256 MiB copy); keyed by kernel, nnz and n so that another workload reports
+# null.
+PROFILED_TRAFFIC = {
+    ("k_spmm<float,k=16>", 27369621, 81195): 237_187_000,
+    ("k_spmm<double,k=16>", 27369621, 81195): 480_284_000,
+}
 TOL = 1.0e-8
 NRHS = 16
 
@@ -276,6 +285,28 @@ def time_solver(hip, torch, solve, B):
             "learned_prec_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None}
 
 
+def time_batched(hip, torch, dmf, B):
+    """All columns of the list B (len in {1, 2, 4, 8, 16}) in one batched
+    matrix-free Jacobi-PCG (each cut block read once per apply for all
+    columns)."""
+    Bk = torch.stack(B, dim=1).contiguous()
+    dmf.pcg_batched(Bk, tol=TOL, maxiter=20000)  # warm-up
+    torch.cuda.synchronize()
+    hip.profile_enable(True)
+    t0 = time.perf_counter()
+    _, res = dmf.pcg_batched(Bk, tol=TOL, maxiter=20000)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    apply_ms, apply_cnt = hip.profile_read("spmm")
+    hip.profile_enable(False)
+    iters = [int(r.iterations) for r in res]
+    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
+    return {"solves_per_s": sig(len(B) / dt), "ms_per_batch": sig(1e3 * dt), "columns": len(B), "converged": ok,
+            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
+            "true_rel_max": sig(max(float(r.true_rel) for r in res)),
+            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1))}
+
+
 def pressure_section(hip, torch, args):
     """Pressure projection on the bump channel: the singular E^T K E
     (periodic in x and z), a stream of zero-sum right-hand sides (the
@@ -305,6 +336,8 @@ def pressure_section(hip, torch, args):
     B = [rhs() for _ in range(args.c5_eval)]
     out["jacobi"] = time_solver(hip, torch, lambda b: dmf.pcg(b, tol=TOL, maxiter=20000), B)
     log(f"pressure, jacobi: {out['jacobi']}")
+    out["jacobi_batched"] = time_batched(hip, torch, dmf, B)
+    log(f"pressure, jacobi batched: {out['jacobi_batched']}")
     t0 = time.perf_counter()
     R, E = generate_pairs(dmf, [rhs() for _ in range(args.c5_pair_rhs)], (5, 20, 60, 150, 400))
     torch.cuda.synchronize()
@@ -322,6 +355,7 @@ def pressure_section(hip, torch, args):
     out["learned_multilevel"] = time_solver(hip, torch, lambda b: mlnet.fcg(dmf, b, tol=TOL, maxiter=20000), B)
     log(f"pressure, learned multilevel: {out['learned_multilevel']}")
     out["speedup_solves_per_s"] = sig(out["learned_multilevel"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
+    out["speedup_batched_jacobi"] = sig(out["jacobi_batched"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
     return out
 
 
@@ -374,6 +408,8 @@ def learned_section(hip, torch, args):
     for method, solve in solvers.items():
         out[method] = time_solver(hip, torch, solve, B)
         log(f"learned section, {method}: {out[method]}")
+    out["jacobi_batched"] = time_batched(hip, torch, dmf, B)
+    log(f"learned section, jacobi batched: {out['jacobi_batched']}")
     _, res, how = solve_learned_with_fallback(net, dmf, B[0], tol=TOL)
     out["fallback_check"] = {"path": how, "converged": res.status == hip.CC_CONVERGED}
     out["speedup_solves_per_s"] = sig(out["learned"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
@@ -398,10 +434,11 @@ def main(argv=None) -> int:
     ap.add_argument("--learned-rhs", type=int, default=200, help="training right-hand sides (10 pairs each)")
     ap.add_argument("--learned-steps", type=int, default=1000, help="Adam steps (8 pairs each)")
     ap.add_argument("--learned-lr", type=float, default=2e-3, help="Adam learning rate")
-    ap.add_argument("--learned-eval", type=int, default=8, help="right-hand sides timed per method")
+    ap.add_argument("--learned-eval", type=int, default=16, help="right-hand sides timed per method (1-16)")
     ap.add_argument("--ml-levels", type=int, default=5, help="grids of the multilevel learned preconditioner")
     ap.add_argument("--no-pressure", action="store_true", help="skip the bump-channel pressure-projection section")
-    ap.add_argument("--c5-eval", type=int, default=8, help="right-hand sides of the pressure stream timed per method")
+    ap.add_argument("--c5-eval", type=int, default=16,
+                    help="right-hand sides of the pressure stream timed per method (1, 2, 4, 8 or 16)")
     ap.add_argument("--c5-pair-rhs", type=int, default=30, help="pressure right-hand sides that give training pairs")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="before timing, copy a buffer of this many MiB once (a known-size kernel that calibrates "
@@ -467,9 +504,10 @@ def main(argv=None) -> int:
     vec_bytes = 8 if head["method"] == "fp64" else 4
     traffic_alg = spmm_bytes(n, nnz, NRHS, value_bytes, vec_bytes)
     achieved = traffic_alg / (head["spmm_avg_us"] * 1e-6) / 1e9
+    kname = f"k_spmm<{'double' if value_bytes == 8 else 'float'},k=16>"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
-                    kernel=f"k_spmm<{'double' if value_bytes == 8 else 'float'},k=16>",
+                    frac=round(achieved / HBM_PEAK_GBS, 4), traffic=PROFILED_TRAFFIC.get((kname, nnz, n)),
+                    kernel=kname,
                     bytes_per_launch=traffic_alg, avg_us=round(head["spmm_avg_us"], 2))
 
     log("single-vector SpMV bandwidth on the workload matrix ...")

@@ -63,64 +63,68 @@ struct GridArgs {
   int px, py, pz;  // periodic flags
 };
 
-// Forward (and data-gradient) layer. One wave = 16 consecutive x voxels of
-// one (y, z) row; ACT: 0 none, 1 relu. gate (optional, bf16 [voxel][16]):
-// multiply the result by (gate > 0), the ReLU derivative in backward mode.
-// Every load is unconditional (out-of-grid taps read a clamped in-grid
-// voxel and are zeroed by a select), so the 27 taps' weight and input loads
-// are all in flight together instead of forming a chain of branches.
+// Forward (and data-gradient) layer; ACT: 0 none, 1 relu. gate (optional,
+// bf16 [voxel][16]): multiply the result by (gate > 0), the ReLU derivative
+// in backward mode.
+// A block of 16 waves owns a 16 x 4 x 4 (x, y, z) tile of output voxels,
+// wave w the 16 x-consecutive voxels of row (y0 + w % 4, z0 + w / 4). The
+// block stages the tile's 18 x 6 x 6 input halo (all 16 channels, zero or
+// wrapped outside the grid) and the layer's weights, transposed to
+// [tap][co][ci], in LDS; then every tap of every wave reads its MFMA A
+// operand (4 channels of one voxel) and B operand (4 input channels of one
+// output channel) as single conflict-free 8-byte LDS reads.
+constexpr int kCx = 16, kCy = 4, kCz = 4;
+constexpr int kHx = kCx + 2, kHy = kCy + 2, kHz = kCz + 2;
+
 template <int ACT>
-__global__ __launch_bounds__(256) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
-                                               const uint16_t* __restrict__ W, const uint8_t* __restrict__ mask,
-                                               const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
-                                               float* __restrict__ out_f32) {
-  __shared__ float tile[4][16][17];
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+__global__ __launch_bounds__(1024) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
+                                                const uint16_t* __restrict__ W, const uint8_t* __restrict__ mask,
+                                                const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
+                                                float* __restrict__ out_f32) {
+  __shared__ __align__(16) uint16_t halo[kHz * kHy * kHx * 16];
+  __shared__ __align__(16) uint16_t wl[27 * 16 * 16];
+  __shared__ float tile[16][16][17];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * kCx, y0 = blockIdx.y * kCy, z0 = blockIdx.z * kCz;
+  // Halo: 16-byte chunks (8 channels) of each voxel.
+  for (int i = tid; i < kHz * kHy * kHx * 2; i += blockDim.x) {
+    const int half = i & 1, hv = i >> 1;
+    const int hx = hv % kHx, hy = (hv / kHx) % kHy, hz = hv / (kHx * kHy);
+    const int gx = nb_index(x0 - 1 + hx, g.nx, g.px);
+    const int gy = nb_index(y0 - 1 + hy, g.ny, g.py);
+    const int gz = nb_index(z0 - 1 + hz, g.nz, g.pz);
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (gx >= 0 && gy >= 0 && gz >= 0)
+      v = *reinterpret_cast<const uint4*>(in + (((size_t)gz * g.ny + gy) * g.nx + gx) * 16 + 8 * half);
+    *reinterpret_cast<uint4*>(&halo[hv * 16 + 8 * half]) = v;
+  }
+  for (int i = tid; i < 27 * 16 * 16; i += blockDim.x) {
+    const int t = i / 256, ci = (i / 16) % 16, co = i % 16;  // W[t][ci][co]
+    wl[(t * 16 + co) * 16 + ci] = W[i];
+  }
+  __syncthreads();
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
   const int row = lane % 16;  // voxel within the wave's 16 (A row / D col index)
   const int kg = lane / 16;   // k group: channels 4 kg .. 4 kg + 3
-  const int xt = (g.nx + 15) / 16;
-  const long long wave_id = (long long)blockIdx.x * 4 + wv;
-  const long long n_waves = (long long)xt * g.ny * g.nz;
-  const bool valid = wave_id < n_waves;
-  const long long wid = valid ? wave_id : 0;
-  const int xb = (int)(wid % xt) * 16;
-  const int y = (int)((wid / xt) % g.ny);
-  const int z = (int)(wid / ((long long)xt * g.ny));
-  const int x = xb + row;
-  int zi[3], yi[3], xi[3];
-#pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    zi[k] = nb_index(z + k - 1, g.nz, g.pz);
-    yi[k] = nb_index(y + k - 1, g.ny, g.py);
-    xi[k] = (x < g.nx) ? nb_index(x + k - 1, g.nx, g.px) : -1;
-  }
+  const int ly = wv % kCy, lz = wv / kCy;
   v4f acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int t = 0; t < 27; ++t) {
-    v4s b;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) b[i] = (short)W[(t * 16 + 4 * kg + i) * 16 + row];
-    const int zz = zi[t / 9], yy = yi[(t / 3) % 3], xx = xi[t % 3];
-    const bool ok = valid && zz >= 0 && yy >= 0 && xx >= 0;
-    const size_t v = ok ? ((size_t)zz * g.ny + yy) * g.nx + xx : 0;
-    const uint2 raw = *reinterpret_cast<const uint2*>(in + v * 16 + 4 * kg);
-    v4s a;
-    a[0] = ok ? (short)(raw.x & 0xffff) : (short)0;
-    a[1] = ok ? (short)(raw.x >> 16) : (short)0;
-    a[2] = ok ? (short)(raw.y & 0xffff) : (short)0;
-    a[3] = ok ? (short)(raw.y >> 16) : (short)0;
+    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // 0..2 in halo coordinates
+    const int hv = ((lz + dz) * kHy + (ly + dy)) * kHx + (row + dx);
+    const v4s a = *reinterpret_cast<const v4s*>(&halo[hv * 16 + 4 * kg]);
+    const v4s b = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
     acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
   }
   // acc[i] = D[voxel 4 kg + i][co = row]; transpose through LDS so that each
-  // lane stores 4 consecutive channels of one voxel. Every wave of the block
-  // reaches the barrier.
+  // lane stores 4 consecutive channels of one voxel.
 #pragma unroll
   for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
   __syncthreads();
-  const int vx = xb + row;
-  if (valid && vx < g.nx) {
-    const size_t v = ((size_t)z * g.ny + y) * g.nx + vx;
+  const int vx = x0 + row, vy = y0 + ly, vz = z0 + lz;
+  if (vx < g.nx && vy < g.ny && vz < g.nz) {
+    const size_t v = ((size_t)vz * g.ny + vy) * g.nx + vx;
     const bool live = mask[v] != 0;
     uint16_t h[4];
     float f[4];
@@ -144,6 +148,10 @@ __global__ __launch_bounds__(256) void k_conv3(GridArgs g, const uint16_t* __res
       for (int i = 0; i < 4; ++i) out_f32[v * 16 + 4 * kg + i] = f[i];
     }
   }
+}
+
+static dim3 conv_grid(int nx, int ny, int nz) {
+  return dim3((unsigned)((nx + kCx - 1) / kCx), (unsigned)((ny + kCy - 1) / kCy), (unsigned)((nz + kCz - 1) / kCz));
 }
 
 // Weight gradient partials: part[block][t][ci][co] = sum over the block's
@@ -320,13 +328,12 @@ __global__ void k_sym_grads(const float* __restrict__ gl, float* __restrict__ gw
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
                        float* out_f32, hipStream_t s) {
   GridArgs g{net.nx, net.ny, net.nz, net.periodic[0], net.periodic[1], net.periodic[2]};
-  const long long waves = (long long)((net.nx + 15) / 16) * net.ny * net.nz;
-  const unsigned blocks = (unsigned)((waves + 3) / 4);
+  const dim3 blocks = conv_grid(net.nx, net.ny, net.nz);
   uint16_t* acts[4] = {act0, act1, act2, act3};
   for (int l = 0; l < 3; ++l)
-    hipLaunchKernelGGL(k_conv3<0>, dim3(blocks), dim3(256), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
                        acts[l + 1], nullptr);
-  hipLaunchKernelGGL(k_conv3<0>, dim3(blocks), dim3(256), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
                      out_f32);
   return (int)hipGetLastError();
 }
@@ -348,14 +355,13 @@ int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* peri
                          uint16_t* out, float* out_f32, cc_stream_t stream) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || !in || !weights || !mask || (!out && !out_f32)) return CC_E_ARG;
   GridArgs g{nx, ny, nz, periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
-  const long long waves = (long long)((nx + 15) / 16) * ny * nz;
-  const long long blocks = (waves + 3) / 4;
-  if (blocks > 0x7fffffffLL) return CC_E_ARG;
+  if ((nz + kCz - 1) / kCz > 65535 || (ny + kCy - 1) / kCy > 65535) return CC_E_ARG;
   hipStream_t s = (hipStream_t)stream;
+  const dim3 blocks = conv_grid(nx, ny, nz);
   if (relu)
-    hipLaunchKernelGGL(k_conv3<1>, dim3((unsigned)blocks), dim3(256), 0, s, g, in, weights, mask, gate, out, out_f32);
+    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32);
   else
-    hipLaunchKernelGGL(k_conv3<0>, dim3((unsigned)blocks), dim3(256), 0, s, g, in, weights, mask, gate, out, out_f32);
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32);
   return (int)hipGetLastError();
 }
 

@@ -82,6 +82,14 @@ int cg_operator_f64(int n, const std::function<int(const double*, double*, doubl
                     const std::function<int(const double*, double*, double*)>& residual,
                     const int* snap_iters = nullptr, int n_snap = 0, double* snapshots = nullptr);
 size_t cg_operator_bytes(size_t n);
+// The same for k in {1, 2, 4, 8, 16} right-hand sides (row-major [n][k]);
+// apply writes per-block, per-column partials [nparts][k]; residual writes k
+// relative residuals to host memory.
+int cg_operator_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& apply,
+                      const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
+                      int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
+                      const std::function<int(const double*, double*, double*)>& residual);
+size_t cg_operator_multi_bytes(size_t n, int k);
 }  // namespace cc
 
 struct cc_cnn_desc;

@@ -461,6 +461,9 @@ static int ml_setup(const cc_mlcnn_desc& ml, const MlBuffers& b, hipStream_t s) 
 }
 
 // z = D^-1 r + S sum_l Q_l C_l^T C_l Q_l^T S r; partial z.q when q is given.
+// (Running the coarse chain on a second stream beside the fine level's
+// network was measured slower: the fine convolutions fill the GPU, and the
+// fork/join events added about 100 us per application.)
 static int ml_prec(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double* r, const double* diag,
                    const double* q, double* z, double* part, const FcgState* st, hipStream_t s) {
   const cc_cnn_desc& f = ml.net[0];

@@ -988,6 +988,58 @@ int cg_operator_f64(int n, const std::function<int(const double*, double*, doubl
 
 size_t cg_operator_bytes(size_t n) { return cg_bytes<double>(n, 1); }
 
+// Float64 Jacobi-PCG for K right-hand sides sharing an operator (row-major
+// [n][K] blocks, as the CSR solver's): apply(P, Q, partial, &nparts) writes
+// Q = A P and per-block, per-column partials of P.Q ([nparts][K]);
+// residual(X, scratch, rel) writes the K true relative residuals (host).
+template <int K>
+static int cg_operator_k(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                         const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
+                         int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
+                         const std::function<int(const double*, double*, double*)>& residual) {
+  Carve cv{(char*)ws, 0, wsb};
+  CgBuffers<double> b = carve_cg<double>(cv, n, K, X);
+  if (!cv.ok()) return CC_E_WORKSPACE;
+  if (precond && diag_src)
+    CC_CHECK(hipMemcpyAsync(b.diag, diag_src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  std::vector<ColState> hs;
+  CC_CHECK((cg_loop<double, K>(n, apply, B, nullptr, nullptr, b, precond && diag_src, tol, maxiter, check_every, s,
+                               hs, nullptr)));
+  CC_CHECK(hipStreamSynchronize(s));
+  std::vector<double> rel(K);
+  CC_CHECK(residual(X, b.R, rel.data()));
+  for (int c = 0; c < K; ++c) {
+    results[c].iterations = hs[c].iterations;
+    results[c].status = hs[c].status == CC_RUNNING ? CC_MAXITER : hs[c].status;
+    results[c].outer = 1;
+    results[c].inner_total = hs[c].iterations;
+    results[c].recursive_rel = hs[c].rel;
+    results[c].true_rel = rel[c];
+  }
+  return 0;
+}
+
+int cg_operator_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& apply,
+                      const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
+                      int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
+                      const std::function<int(const double*, double*, double*)>& residual) {
+  switch (k) {
+    case 1: return cg_operator_k<1>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
+                                    residual);
+    case 2: return cg_operator_k<2>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
+                                    residual);
+    case 4: return cg_operator_k<4>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
+                                    residual);
+    case 8: return cg_operator_k<8>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
+                                    residual);
+    case 16: return cg_operator_k<16>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results,
+                                      s, residual);
+    default: return CC_E_ARG;
+  }
+}
+
+size_t cg_operator_multi_bytes(size_t n, int k) { return cg_bytes<double>(n, k); }
+
 // True relative residuals ||B - A X|| / ||B|| (||B|| = 0 -> 1) of the float64
 // block X, written to rel[K] (host). The per-block partial sums are combined
 // on the host in block order.

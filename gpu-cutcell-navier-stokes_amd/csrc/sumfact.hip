@@ -62,10 +62,15 @@ __global__ __launch_bounds__(256) void k_mf_gather_small(cc_sumfact_desc d, cons
   if (lane == 0) u[d.small_dof[sm]] = s;
 }
 
+// blockIdx.z = column * nzt + z tile: column c reads u + c n_full and writes
+// v + c n_full (one column: nzt = gridDim.z).
 template <int P1>
-__global__ __launch_bounds__(256) void k_mf_tensor(cc_sumfact_desc d, const double* __restrict__ u,
-                                                   double* __restrict__ v) {
+__global__ __launch_bounds__(256) void k_mf_tensor(cc_sumfact_desc d, const double* __restrict__ u_all,
+                                                   double* __restrict__ v_all, int nzt) {
   constexpr int p = P1 - 1;
+  const size_t col_off = (size_t)(blockIdx.z / nzt) * ((size_t)d.nb[0] * d.nb[1] * d.nb[2]);
+  const double* __restrict__ u = u_all + col_off;
+  double* __restrict__ v = v_all + col_off;
   constexpr int W = 2 * p + 1;
   constexpr int T = kTile;
   constexpr int R = T + 2 * p;
@@ -75,7 +80,7 @@ __global__ __launch_bounds__(256) void k_mf_tensor(cc_sumfact_desc d, const doub
   __shared__ double sc[R * T * T];
   __shared__ double sd[R * T * T];
   const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T, z0 = blockIdx.z * T;
+  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T, z0 = (blockIdx.z % nzt) * T;
   const int tid = threadIdx.x;
   const double c = d.mass_coef;
   for (int i = tid; i < R * R * R; i += blockDim.x) {
@@ -290,6 +295,232 @@ __global__ __launch_bounds__(256) void k_mf_residual(int n, const double* __rest
   }
 }
 
+// ---------------------------------------------------------------------------
+// K right-hand sides. Reduced-space blocks are row-major [n_red][K] (as the
+// CSR solver's); the background-grid work arrays u, v are column-major
+// [K][n_full], so that each column's tensor apply reads and writes
+// contiguous memory, and every shell cell reads its cut block once for all K
+// columns. Shell outputs are [n_corr][K][P], extra is [n_red][K].
+// ---------------------------------------------------------------------------
+
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_gather(cc_sumfact_desc d, const double* __restrict__ x,
+                                                    double* __restrict__ u) {
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  const size_t total = n_full * K;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t g = e / K;
+    const int c = (int)(e % K);
+    const int r = d.full2red[g];
+    u[(size_t)c * n_full + g] = r >= 0 ? x[(size_t)r * K + c] : 0.0;
+  }
+}
+
+// One wave per small dof: lane (entry slot e, column c), 64 / K entry slots.
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, const double* __restrict__ x,
+                                                          double* __restrict__ u) {
+  constexpr int EL = 64 / K;
+  const int lane = threadIdx.x & 63;
+  const int c = lane % K, el = lane / K;
+  const int sm = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  if (sm >= d.n_small) return;
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  double s = 0.0;
+  for (int t = d.small_ptr[sm] + el; t < d.small_ptr[sm + 1]; t += EL)
+    s += d.small_w[t] * x[(size_t)d.small_col[t] * K + c];
+  for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
+  if (lane < K) u[(size_t)c * n_full + d.small_dof[sm]] = s;
+}
+
+// One wave per shell cell and all K columns: W[s][c][l] = (C_s - A_s) u_s
+// (cut) or -A_s u_s (empty), columns handled CK at a time in registers.
+template <int P1, int K>
+__global__ __launch_bounds__(256) void k_mfk_shell(cc_sumfact_desc d, const double* __restrict__ u,
+                                                   double* __restrict__ Wout) {
+  constexpr int PP = P1 * P1 * P1;
+  constexpr int WAVES = 4;
+  constexpr int CK = K < 8 ? K : 8;
+  __shared__ double su[WAVES][K][PP];
+  __shared__ double sf[WAVES][6][P1 * P1];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int s = blockIdx.x * WAVES + wv;
+  const bool live = s < d.n_corr;
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  const int ex = live ? d.corr_cell[3 * s + 0] : 0;
+  const int ey = live ? d.corr_cell[3 * s + 1] : 0;
+  const int ez = live ? d.corr_cell[3 * s + 2] : 0;
+  if (live) {
+    for (int l = lane; l < PP; l += 64) {
+      const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
+      const int gx = wrap_or_invalid(ex + lx, d.nb[0], d.periodic[0]);
+      const int gy = wrap_or_invalid(ey + ly, d.nb[1], d.periodic[1]);
+      const int gz = wrap_or_invalid(ez + lz, d.nb[2], d.periodic[2]);
+      const size_t g = ((size_t)gz * d.nb[1] + gy) * d.nb[0] + gx;
+#pragma unroll
+      for (int c = 0; c < K; ++c) su[wv][c][l] = u[(size_t)c * n_full + g];
+    }
+    for (int l = lane; l < P1 * P1; l += 64) {
+      sf[wv][0][l] = d.elem_m[0][(size_t)ex * P1 * P1 + l];
+      sf[wv][1][l] = d.elem_k[0][(size_t)ex * P1 * P1 + l];
+      sf[wv][2][l] = d.elem_m[1][(size_t)ey * P1 * P1 + l];
+      sf[wv][3][l] = d.elem_k[1][(size_t)ey * P1 * P1 + l];
+      sf[wv][4][l] = d.elem_m[2][(size_t)ez * P1 * P1 + l];
+      sf[wv][5][l] = d.elem_k[2][(size_t)ez * P1 * P1 + l];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const int blk = d.corr_block[s];
+  const double cm = d.mass_coef;
+  for (int l = lane; l < PP; l += 64) {
+    const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
+    for (int c0 = 0; c0 < K; c0 += CK) {
+      double acc[CK];
+#pragma unroll
+      for (int c = 0; c < CK; ++c) acc[c] = 0.0;
+      for (int jz = 0; jz < P1; ++jz) {
+        const double mz = sf[wv][4][lz * P1 + jz], kz = sf[wv][5][lz * P1 + jz];
+        for (int jy = 0; jy < P1; ++jy) {
+          const double my = sf[wv][2][ly * P1 + jy], ky = sf[wv][3][ly * P1 + jy];
+          const double cyz = kz * my + mz * ky + cm * mz * my, myz = mz * my;
+          double sm[CK], sk[CK];
+#pragma unroll
+          for (int c = 0; c < CK; ++c) sm[c] = sk[c] = 0.0;
+#pragma unroll
+          for (int jx = 0; jx < P1; ++jx) {
+            const double mx = sf[wv][0][lx * P1 + jx], kx = sf[wv][1][lx * P1 + jx];
+            const int j = (jz * P1 + jy) * P1 + jx;
+#pragma unroll
+            for (int c = 0; c < CK; ++c) {
+              const double uj = su[wv][c0 + c][j];
+              sm[c] += mx * uj;
+              sk[c] += kx * uj;
+            }
+          }
+#pragma unroll
+          for (int c = 0; c < CK; ++c) acc[c] += cyz * sm[c] + myz * sk[c];
+        }
+      }
+      double w[CK];
+#pragma unroll
+      for (int c = 0; c < CK; ++c) w[c] = -acc[c];
+      if (blk >= 0) {
+        const double* __restrict__ Cb = d.blocks + (size_t)blk * PP * PP;
+        for (int j = 0; j < PP; ++j) {
+          const double cb = Cb[(size_t)j * PP + l];  // symmetric: column l = row l
+#pragma unroll
+          for (int c = 0; c < CK; ++c) w[c] += cb * su[wv][c0 + c][j];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < CK; ++c) Wout[((size_t)s * K + c0 + c) * PP + l] = w[c];
+    }
+  }
+}
+
+// v[c][g] += sum of the shell outputs of dof g: lane (entry slot, column),
+// 64 / K entry slots per touched dof, summed by a butterfly.
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const double* __restrict__ Win,
+                                                   double* __restrict__ v, int PP) {
+  constexpr int EL = 64 / K;
+  const int lane = threadIdx.x & 63;
+  const int c = lane % K, el = lane / K;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  for (int t = wave; t < d.n_touch; t += nwaves) {
+    double s = 0.0;
+    for (int i = d.touch_ptr[t] + el; i < d.touch_ptr[t + 1]; i += EL) {
+      const int idx = d.touch_idx[i];
+      const int sc = idx / PP, l = idx - sc * PP;
+      s += Win[((size_t)sc * K + c) * PP + l];
+    }
+    for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
+    if (lane < K) v[(size_t)c * n_full + d.touch_dof[t]] += s;
+  }
+}
+
+// extra[j][c] = sum_b E[b, j] v[c][small dof b] for the unknowns with E^T entries.
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_et_extra(cc_sumfact_desc d, const double* __restrict__ v,
+                                                      double* __restrict__ extra) {
+  constexpr int EL = 64 / K;
+  const int lane = threadIdx.x & 63;
+  const int c = lane % K, el = lane / K;
+  const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int nwaves = (gridDim.x * blockDim.x) >> 6;
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  for (int i = wave; i < d.n_et_rows; i += nwaves) {
+    const int j = d.et_rows[i];
+    double s = 0.0;
+    for (int t = d.et_ptr[j] + el; t < d.et_ptr[j + 1]; t += EL)
+      s += d.et_w[t] * v[(size_t)c * n_full + d.small_dof[d.et_small[t]]];
+    for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
+    if (lane < K) extra[(size_t)j * K + c] = s;
+  }
+}
+
+// y = E^T v for K columns; partial[block][c] = sum_j x[j][c] y[j][c].
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_scatter(cc_sumfact_desc d, const double* __restrict__ v,
+                                                     const double* __restrict__ extra, double* __restrict__ y,
+                                                     const double* __restrict__ x, double* __restrict__ partial) {
+  __shared__ double red[4 * K];
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  const size_t total = (size_t)d.n_red * K;
+  const int c = threadIdx.x % K;  // fixed per thread: the stride is a multiple of K
+  double dot = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const int j = (int)(e / K);
+    double sv = v[(size_t)c * n_full + d.red2full[j]];
+    if (d.et_ptr[j + 1] > d.et_ptr[j]) sv += extra[e];
+    y[e] = sv;
+    if (partial) dot += x[e] * sv;
+  }
+  if (partial) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int o = K; o < 64; o <<= 1) dot += __shfl_xor(dot, o, 64);
+    if (lane < K) red[wave * K + lane] = dot;
+    __syncthreads();
+    if (threadIdx.x < K)
+      partial[(size_t)blockIdx.x * K + threadIdx.x] =
+          (red[threadIdx.x] + red[K + threadIdx.x]) + (red[2 * K + threadIdx.x] + red[3 * K + threadIdx.x]);
+  }
+}
+
+// Per-block, per-column partials of ((b - y).(b - y), b.b): [block][2][K].
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_residual(int n, const double* __restrict__ b, const double* __restrict__ y,
+                                                      double* __restrict__ partial) {
+  __shared__ double red[2][4 * K];
+  const size_t total = (size_t)n * K;
+  const int c = threadIdx.x % K;
+  double rr = 0.0, bb = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const double r = b[e] - y[e];
+    rr += r * r;
+    bb += b[e] * b[e];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = K; o < 64; o <<= 1) {
+    rr += __shfl_xor(rr, o, 64);
+    bb += __shfl_xor(bb, o, 64);
+  }
+  if (lane < K) {
+    red[0][wave * K + lane] = rr;
+    red[1][wave * K + lane] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * K) {
+    const int q = threadIdx.x / K, cc = threadIdx.x % K;
+    partial[((size_t)blockIdx.x * 2 + q) * K + cc] =
+        (red[q][cc] + red[q][K + cc]) + (red[q][2 * K + cc] + red[q][3 * K + cc]);
+  }
+}
+
 struct MfBuffers {
   double *u, *v, *W, *extra;
 };
@@ -322,6 +553,15 @@ static int grid1d(size_t n) {
   return (int)(g < 1 ? 1 : g);
 }
 
+// Blocks for kernels without per-block partials whose items (one per 256 /
+// per_block threads) are latency-bound chains: one item per wave or team,
+// no cap below the grid limit.
+static int grid_items(size_t items, int per_block) {
+  size_t g = (items + per_block - 1) / per_block;
+  if (g > 0x7fffffffull) g = 0x7fffffffull;
+  return (int)(g < 1 ? 1 : g);
+}
+
 template <int P1>
 static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double* x, double* y, double* partial,
                       int* nparts, const ColState* st, hipStream_t s) {
@@ -329,13 +569,13 @@ static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double
   hipLaunchKernelGGL(k_mf_gather, dim3(grid1d(n_full)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0) hipLaunchKernelGGL(k_mf_gather_small, dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
   dim3 tiles((d.nb[0] + kTile - 1) / kTile, (d.nb[1] + kTile - 1) / kTile, (d.nb[2] + kTile - 1) / kTile);
-  hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v);
+  hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v, (int)tiles.z);
   if (d.n_corr > 0) {
     hipLaunchKernelGGL((k_mf_shell<P1>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
-    hipLaunchKernelGGL(k_mf_touch, dim3(grid1d((size_t)d.n_touch * 16)), dim3(256), 0, s, d, b.W, b.v);
+    hipLaunchKernelGGL(k_mf_touch, dim3(grid_items(d.n_touch, 16)), dim3(256), 0, s, d, b.W, b.v);
   }
   if (d.n_et_rows > 0)
-    hipLaunchKernelGGL(k_mf_et_extra, dim3(grid1d((size_t)d.n_et_rows * 16)), dim3(256), 0, s, d, b.v, b.extra, st);
+    hipLaunchKernelGGL(k_mf_et_extra, dim3(grid_items(d.n_et_rows, 16)), dim3(256), 0, s, d, b.v, b.extra, st);
   const int gs = grid1d(d.n_red);
   hipLaunchKernelGGL(k_mf_scatter, dim3(gs), dim3(256), 0, s, d, b.v, b.extra, y, x, partial, st);
   if (nparts) *nparts = gs;
@@ -370,6 +610,102 @@ static int mf_true_residual(const cc_sumfact_desc& d, const MfBuffers& mb, const
   double bn = std::sqrt(bb);
   if (bn == 0.0) bn = 1.0;
   *rel = std::sqrt(rr) / bn;
+  return 0;
+}
+
+// K-column buffers: u, v [K][n_full]; W [n_corr][K][P]; extra [n_red][K].
+static size_t mfk_bytes(const cc_sumfact_desc* d, int K) {
+  const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
+  const int P1 = d->degree + 1;
+  const size_t pp = (size_t)P1 * P1 * P1;
+  return (2 * n_full + (size_t)d->n_corr * pp + (size_t)d->n_red) * K * sizeof(double) + 4 * 256;
+}
+
+static MfBuffers carve_mfk(const cc_sumfact_desc* d, int K, char* base) {
+  const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
+  const int P1 = d->degree + 1;
+  const size_t pp = (size_t)P1 * P1 * P1;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  MfBuffers b;
+  size_t off = 0;
+  b.u = reinterpret_cast<double*>(base + off);
+  off += al(n_full * K * sizeof(double));
+  b.v = reinterpret_cast<double*>(base + off);
+  off += al(n_full * K * sizeof(double));
+  b.W = reinterpret_cast<double*>(base + off);
+  off += al((size_t)d->n_corr * pp * K * sizeof(double));
+  b.extra = reinterpret_cast<double*>(base + off);
+  return b;
+}
+
+template <int P1, int K>
+static int mfk_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double* x, double* y, double* partial,
+                       int* nparts, hipStream_t s) {
+  const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
+  constexpr int PP = P1 * P1 * P1;
+  hipLaunchKernelGGL((k_mfk_gather<K>), dim3(grid1d(n_full * K)), dim3(256), 0, s, d, x, b.u);
+  if (d.n_small > 0)
+    hipLaunchKernelGGL((k_mfk_gather_small<K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
+  const int nzt = (d.nb[2] + kTile - 1) / kTile;
+  dim3 tiles((d.nb[0] + kTile - 1) / kTile, (d.nb[1] + kTile - 1) / kTile, nzt * K);
+  hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v, nzt);
+  if (d.n_corr > 0) {
+    hipLaunchKernelGGL((k_mfk_shell<P1, K>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
+    hipLaunchKernelGGL((k_mfk_touch<K>), dim3(grid_items(d.n_touch, 4)), dim3(256), 0, s, d, b.W, b.v, PP);
+  }
+  if (d.n_et_rows > 0)
+    hipLaunchKernelGGL((k_mfk_et_extra<K>), dim3(grid_items(d.n_et_rows, 4)), dim3(256), 0, s, d, b.v, b.extra);
+  const int gs = grid1d((size_t)d.n_red * K);
+  hipLaunchKernelGGL((k_mfk_scatter<K>), dim3(gs), dim3(256), 0, s, d, b.v, b.extra, y, x, partial);
+  if (nparts) *nparts = gs;
+  return (int)hipGetLastError();
+}
+
+static int mfk_apply(const cc_sumfact_desc& d, int K, const MfBuffers& b, const double* x, double* y,
+                     double* partial, int* nparts, hipStream_t s) {
+#define CC_MFK(P1V)                                                              \
+  switch (K) {                                                                  \
+    case 1: return mfk_apply_t<P1V, 1>(d, b, x, y, partial, nparts, s);         \
+    case 2: return mfk_apply_t<P1V, 2>(d, b, x, y, partial, nparts, s);         \
+    case 4: return mfk_apply_t<P1V, 4>(d, b, x, y, partial, nparts, s);         \
+    case 8: return mfk_apply_t<P1V, 8>(d, b, x, y, partial, nparts, s);         \
+    case 16: return mfk_apply_t<P1V, 16>(d, b, x, y, partial, nparts, s);       \
+    default: return CC_E_ARG;                                                   \
+  }
+  if (d.degree == 2) CC_MFK(3)
+  if (d.degree == 3) CC_MFK(4)
+#undef CC_MFK
+  return CC_E_ARG;
+}
+
+// ||b_c - (A x)_c|| / ||b_c|| for the K columns (||b_c|| = 0 -> 1); partials
+// summed on the host in block order.
+static int mfk_true_residual(const cc_sumfact_desc& d, int K, const MfBuffers& mb, const double* B, const double* X,
+                             double* scratch, double* rpart, double* rel, hipStream_t s) {
+  CC_CHECK(mfk_apply(d, K, mb, X, scratch, nullptr, nullptr, s));
+  const int g = grid1d((size_t)d.n_red * K);
+  switch (K) {
+    case 1: hipLaunchKernelGGL((k_mfk_residual<1>), dim3(g), dim3(256), 0, s, d.n_red, B, scratch, rpart); break;
+    case 2: hipLaunchKernelGGL((k_mfk_residual<2>), dim3(g), dim3(256), 0, s, d.n_red, B, scratch, rpart); break;
+    case 4: hipLaunchKernelGGL((k_mfk_residual<4>), dim3(g), dim3(256), 0, s, d.n_red, B, scratch, rpart); break;
+    case 8: hipLaunchKernelGGL((k_mfk_residual<8>), dim3(g), dim3(256), 0, s, d.n_red, B, scratch, rpart); break;
+    case 16: hipLaunchKernelGGL((k_mfk_residual<16>), dim3(g), dim3(256), 0, s, d.n_red, B, scratch, rpart); break;
+    default: return CC_E_ARG;
+  }
+  CC_CHECK(hipGetLastError());
+  std::vector<double> hp((size_t)g * 2 * K);
+  CC_CHECK(hipMemcpyAsync(hp.data(), rpart, hp.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c) {
+    double rr = 0.0, bb = 0.0;
+    for (int i = 0; i < g; ++i) {
+      rr += hp[((size_t)i * 2 + 0) * K + c];
+      bb += hp[((size_t)i * 2 + 1) * K + c];
+    }
+    double bn = std::sqrt(bb);
+    if (bn == 0.0) bn = 1.0;
+    rel[c] = std::sqrt(rr) / bn;
+  }
   return 0;
 }
 
@@ -488,6 +824,47 @@ int cc_fcg_sumfact(const cc_sumfact_desc* d, const cc_cnn_desc* net, const doubl
   };
   return fcg_operator(desc.n_red, apply, *net, desc.diag, bvec, x, tol, maxiter, check_every, fws, fb, result,
                       history, s, residual);
+}
+
+static bool k_ok(int k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16; }
+
+size_t cc_sumfact_k_workspace_bytes(const cc_sumfact_desc* d, int32_t k, int32_t solve) {
+  if (!desc_ok(d) || !k_ok(k)) return 0;
+  size_t b = mfk_bytes(d, k) + 512;
+  if (solve) b += cg_operator_multi_bytes((size_t)d->n_red, k) + (size_t)kMaxParts * 2 * k * sizeof(double) + 1024;
+  return b;
+}
+
+int cc_sumfact_apply_k(const cc_sumfact_desc* d, int32_t k, const double* X, double* Y, void* workspace,
+                       size_t workspace_bytes, cc_stream_t stream) {
+  if (!desc_ok(d) || !k_ok(k) || !X || !Y || !workspace) return CC_E_ARG;
+  if (workspace_bytes < cc_sumfact_k_workspace_bytes(d, k, 0)) return CC_E_WORKSPACE;
+  MfBuffers b = carve_mfk(d, k, (char*)workspace);
+  return mfk_apply(*d, k, b, X, Y, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int cc_pcg_sumfact_k(const cc_sumfact_desc* d, int32_t k, const double* B, double* X, double tol, int32_t maxiter,
+                     int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                     cc_pcg_result* results, cc_stream_t stream) {
+  if (!desc_ok(d) || !k_ok(k) || !B || !X || !workspace || !results || maxiter < 0) return CC_E_ARG;
+  if (workspace_bytes < cc_sumfact_k_workspace_bytes(d, k, 1)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  MfBuffers mb = carve_mfk(d, k, base);
+  const size_t off = (mfk_bytes(d, k) + 511) & ~size_t(255);
+  double* rpart = reinterpret_cast<double*>(base + off);
+  char* cgws = base + off + (((size_t)kMaxParts * 2 * k * sizeof(double) + 511) & ~size_t(255));
+  const size_t cgb = workspace_bytes - (size_t)(cgws - base);
+  const cc_sumfact_desc desc = *d;
+  const int K = k;
+  auto apply = [&](const double* P, double* Q, double* partial, int* gs) {
+    return mfk_apply(desc, K, mb, P, Q, partial, gs, s);
+  };
+  auto residual = [&](const double* Xv, double* scratch, double* rel) {
+    return mfk_true_residual(desc, K, mb, B, Xv, scratch, rpart, rel, s);
+  };
+  return cg_operator_multi(desc.n_red, k, apply, preconditioner ? desc.diag : nullptr, B, X, tol, maxiter,
+                           preconditioner, check_every, cgws, cgb, results, s, residual);
 }
 
 size_t cc_fcg_sumfact_ml_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml) {

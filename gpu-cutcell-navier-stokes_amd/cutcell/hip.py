@@ -65,6 +65,9 @@ _SIGNATURES = {
     "cc_sumfact_workspace_bytes": ([_P, _I32], _SZ),
     "cc_sumfact_apply": ([_P, _P, _P, _P, _SZ, _P], ctypes.c_int),
     "cc_pcg_sumfact": ([_P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P, _I32, _P, _P], ctypes.c_int),
+    "cc_sumfact_k_workspace_bytes": ([_P, _I32, _I32], _SZ),
+    "cc_sumfact_apply_k": ([_P, _I32, _P, _P, _P, _SZ, _P], ctypes.c_int),
+    "cc_pcg_sumfact_k": ([_P, _I32, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
     "cc_pcg_batched": (
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _F64, _I32, _I32, _I32,
          _I32, _P, _P, _P],

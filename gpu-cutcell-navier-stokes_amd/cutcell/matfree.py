@@ -433,3 +433,52 @@ class DeviceMatrixFree:
         if n_snap:
             out += (snaps,)
         return out
+
+    # -- k right-hand sides at once -----------------------------------------------
+    def _ws_k(self, k: int, solve: bool):
+        import torch
+
+        from . import hip
+
+        lib = hip.require_gpu()
+        nbytes = int(lib.cc_sumfact_k_workspace_bytes(ctypes.addressof(self.desc), int(k), 1 if solve else 0))
+        if nbytes == 0:
+            raise ValueError(f"invalid matrix-free descriptor or k={k} (k in {hip.SUPPORTED_K})")
+        key = f"ws_k{k}_{int(solve)}"
+        buf = getattr(self, key, None)
+        if buf is None or buf.numel() < nbytes:
+            buf = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            setattr(self, key, buf)
+        return lib, buf, nbytes
+
+    def apply_k(self, X, Y=None):
+        """``Y = E^T (K + c M) E X`` for a row-major ``[n, k]`` float64 block."""
+        import torch
+
+        from . import hip
+
+        k = X.shape[1]
+        lib, ws, nbytes = self._ws_k(k, False)
+        X = X.contiguous()
+        Y = torch.empty_like(X) if Y is None else Y
+        hip._check(lib.cc_sumfact_apply_k(ctypes.addressof(self.desc), int(k), hip._ptr(X), hip._ptr(Y), hip._ptr(ws),
+                                          nbytes, hip._stream()), "sumfact_apply_k")
+        return Y
+
+    def pcg_batched(self, B, *, tol=1.0e-8, maxiter=5000, preconditioner="jacobi", check_every=16, X=None):
+        """Jacobi-PCG on the ``k`` columns of ``B`` (``[n, k]``, k in {1, 2, 4,
+        8, 16}) in one device loop, each cut block read once per apply for all
+        columns. Returns ``(X, [result per column])``."""
+        import torch
+
+        from . import hip
+
+        k = B.shape[1]
+        lib, ws, nbytes = self._ws_k(k, True)
+        B = B.contiguous()
+        X = torch.empty_like(B) if X is None else X
+        res = (hip.PcgResult * k)()
+        hip._check(lib.cc_pcg_sumfact_k(ctypes.addressof(self.desc), int(k), hip._ptr(B), hip._ptr(X), float(tol),
+                                        int(maxiter), {"none": 0, "jacobi": 1}[preconditioner], int(check_every),
+                                        hip._ptr(ws), nbytes, res, hip._stream()), "pcg_sumfact_k")
+        return X, list(res)

@@ -238,6 +238,19 @@ int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* b, double* x, double 
                    cc_pcg_result* result, double* history, const int32_t* snap_iters, int32_t n_snap,
                    double* snapshots, cc_stream_t stream);
 
+/* The same operator for k in {1, 2, 4, 8, 16} right-hand sides at once (X,
+ * Y, B row-major [n_red][k]); the cut blocks are read once per apply for all
+ * k columns. cc_pcg_sumfact_k runs Jacobi-PCG on the k columns (semantics of
+ * cc_pcg_jacobi_f64 per column; no history or snapshots); synchronises;
+ * results[k] host memory. (No reference counterpart: the reference solves
+ * one right-hand side per call, code/fast_solver_backends.py:194-289.) */
+size_t cc_sumfact_k_workspace_bytes(const cc_sumfact_desc* d, int32_t k, int32_t solve);
+int cc_sumfact_apply_k(const cc_sumfact_desc* d, int32_t k, const double* X, double* Y, void* workspace,
+                       size_t workspace_bytes, cc_stream_t stream);
+int cc_pcg_sumfact_k(const cc_sumfact_desc* d, int32_t k, const double* B, double* X, double tol, int32_t maxiter,
+                     int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                     cc_pcg_result* results, cc_stream_t stream);
+
 /* ---- Learned preconditioner (no reference counterpart: the reference has
  * only Jacobi, code/fast_solver_backends.py:74-87 and :237-238). ---------
  * Activations are channel-last bfloat16 grids [voxel][16] on the background

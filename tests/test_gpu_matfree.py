@@ -70,3 +70,34 @@ def test_snapshots_are_the_capped_iterates(case):
         assert rk.iterations == k
         assert torch.equal(snaps[j], xk), (name, k)
     assert torch.equal(snaps[3], x), name
+
+
+@pytest.mark.parametrize("k", [2, 16])
+def test_batched_apply_and_pcg_match_single_column(case, k):
+    """k right-hand sides at once: the apply equals k single applies and the
+    explicit operator (to rounding), and the batched Jacobi-PCG gives each
+    column the single-column solve's solution and stop. The batched kernels
+    add the shell, small-dof and E^T contributions in another fixed order
+    than the single-column ones, so the stop may move by the same few
+    iterations the float summation order moves it elsewhere in this suite
+    (at most three, where the residual hovers near tol)."""
+    name, mf, A, dmf = case
+    rng = np.random.default_rng(k)
+    X = rng.standard_normal((mf.n, k))
+    if "bump" in name:  # keep the stiffness-only variant's columns compatible too
+        X -= X.mean(axis=0)
+    Xd = torch.from_numpy(X).to("cuda")
+    Y = dmf.apply_k(Xd).cpu().numpy()
+    ref = np.stack([dmf.apply(Xd[:, c].contiguous()).cpu().numpy() for c in range(k)], axis=1)
+    scale = (abs(A) @ np.abs(X)).max()  # size of the summed terms (as in the apply test above)
+    assert np.abs(Y - ref).max() <= 1e-11 * scale
+    assert np.abs(Y - A @ X).max() <= 1e-11 * scale
+    Xs, res = dmf.pcg_batched(Xd, tol=1e-8, maxiter=3000)
+    Xs = Xs.cpu().numpy()
+    for c in range(k):
+        x1, r1 = dmf.pcg(Xd[:, c].contiguous(), tol=1e-8, maxiter=3000)
+        assert res[c].status == 1 and r1.status == 1
+        assert abs(res[c].iterations - r1.iterations) <= 3, (c, res[c].iterations, r1.iterations)
+        assert res[c].true_rel <= 1e-7
+        x1 = x1.cpu().numpy()
+        assert np.linalg.norm(Xs[:, c] - x1) <= 1e-6 * np.linalg.norm(x1)
