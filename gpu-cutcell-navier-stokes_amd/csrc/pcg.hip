@@ -101,10 +101,14 @@ void prof_collect() {
 struct SpmmConfig {
   int nz;      // nonzero slots per row team ((K/cpl)*nz <= 64); 0 = pick from the row length
   int grid;    // blocks; 0 = min(kMaxParts, rows / teams per block)
-  int unroll;  // 1, 4 or 8 (software-pipelined) loads in flight per lane; 0 = 4
+  int unroll;  // 1, 4 or 8 (software-pipelined) loads in flight per lane; 0 = 4;
+               // kUnrollWindow: the nonzero-window kernel (k = 1 only)
   int cpl;     // right-hand sides per lane (vector width); 0 = widest 16-byte load
-  int stream;  // 1: nontemporal matrix loads (unroll 4 only)
+  int stream;  // 1: nontemporal matrix loads (not with unroll 1)
+  int nsub;    // sweep steps: chunks per block, taken grid-strided; 0 = 1
 };
+constexpr int kUnrollWindow = 16;
+constexpr int kMaxSweep = 64;
 static SpmmConfig g_spmm_cfg[2][17] = {};
 
 template <typename VT>
@@ -365,11 +369,16 @@ __device__ inline T stream_ld(const T* p) {
 
 // NTL: load the matrix (read once per product) with nontemporal loads so it
 // does not evict the gathered X rows from L2; chosen by the autotuner.
+// nsub: the rows are cut into gridDim.x * nsub chunks of equal nonzero
+// counts; block b takes chunks b, b + G, b + 2G, ... in that order, so at any
+// time the whole grid streams one compact stretch of the matrix (DRAM pages
+// stay open) instead of G stretches far apart.
 template <typename MT, typename VT, int K, int NZ, int MODE, int UNR, int CPL, bool NTL>
 __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ indptr, const int* __restrict__ indices,
                                               const MT* __restrict__ vals, const VT* __restrict__ X,
                                               VT* __restrict__ Y, const double* __restrict__ B,
-                                              double* __restrict__ partial, const ColState* __restrict__ st) {
+                                              double* __restrict__ partial, const ColState* __restrict__ st,
+                                              int nsub) {
   constexpr int KL = K / CPL;
   constexpr int T = KL * NZ;
   constexpr int TPW = 64 / T;
@@ -383,32 +392,69 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
   const int wpb = blockDim.x >> 6;
   const int tpb = wpb * TPW;
   const int team = wave * TPW + lane / T;
-  // XCD-aware chunk of rows for this block: chunks hold equal shares of the
-  // nonzeros (rows along a cut are up to three times longer than the rest),
-  // and the blocks one XCD runs (block id mod 8) take adjacent chunks.
+  // XCD-aware chunks of rows: chunks hold equal shares of the nonzeros (rows
+  // along a cut are up to three times longer than the rest), and within one
+  // sweep step the blocks one XCD runs (block id mod 8) take adjacent chunks.
   const int G = gridDim.x;
   const int b = blockIdx.x;
-  const int chunk = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
-  __shared__ int s_rows[2];
-  if (wave == 0) {
-    const long long base = indptr[0];
-    const long long total = (long long)indptr[n] - base;
-    const int a0 = chunk == 0 ? 0 : row_lower_bound(indptr, n, base + total * chunk / G);
-    const int a1 = chunk == G - 1 ? n : row_lower_bound(indptr, n, base + total * (chunk + 1) / G);
-    if (lane == 0) {
-      s_rows[0] = a0;
-      s_rows[1] = a1;
-    }
-  }
-  __syncthreads();
-  const int r0 = s_rows[0];
-  const int r1 = s_rows[1];
+  const int chunk0 = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
+  const int NC = G * nsub;
+  __shared__ int s_rows[2][2];
   double dot[CPL];
 #pragma unroll
   for (int m = 0; m < CPL; ++m) dot[m] = 0.0;
+  for (int sub = 0; sub < nsub; ++sub) {
+  const int chunk = sub * G + chunk0;
+  if (wave == 0) {
+    const long long base = indptr[0];
+    const long long total = (long long)indptr[n] - base;
+    const int a0 = chunk == 0 ? 0 : row_lower_bound(indptr, n, base + total * chunk / NC);
+    const int a1 = chunk == NC - 1 ? n : row_lower_bound(indptr, n, base + total * (chunk + 1) / NC);
+    if (lane == 0) {
+      s_rows[sub & 1][0] = a0;
+      s_rows[sub & 1][1] = a1;
+    }
+  }
+  // Double-buffered: the next step's bounds go to the other slot, which every
+  // wave finished reading before this barrier.
+  __syncthreads();
+  const int r0 = s_rows[sub & 1][0];
+  const int r1 = s_rows[sub & 1][1];
+  // UNR == 8 carries state from one row of the team to the next: the next
+  // row's bounds (loaded when the current row starts) and its first chunk of
+  // indices and values (loaded during the current row's last chunk), so a row
+  // never starts with a cold chain of row pointer, entries and gathers.
+  constexpr int U = 8;
+  int qa[U];
+  AT va[U];
+  int nbeg = 0, nlen = 0;
+  // The next 8 entries of this lane at j0 of a row of length len (entries past
+  // the end read X row `dummy`, always in range, and carry a zero value).
+  auto load_chunk = [&](const int* ip, const MT* vp, int j0, int len_, int dummy, int* q, AT* v) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int jj = j0 + u * NZ;
+      q[u] = jj < len_ ? stream_ld<NTL>(ip + jj) : dummy;
+      v[u] = jj < len_ ? (AT)stream_ld<NTL>(vp + jj) : (AT)0;
+    }
+  };
+  if constexpr (UNR == 8) {
+    const int row = r0 + team;
+    if (row < r1) {
+      nbeg = indptr[row];
+      nlen = indptr[row + 1] - nbeg;
+      load_chunk(indices + nbeg, vals + nbeg, zs, nlen, row, qa, va);
+    }
+  }
   for (int row = r0 + team; row < r1; row += tpb) {
-    const int beg = indptr[row];
-    const int len = indptr[row + 1] - beg;
+    int beg, len;
+    if constexpr (UNR == 8) {
+      beg = nbeg;
+      len = nlen;
+    } else {
+      beg = indptr[row];
+      len = indptr[row + 1] - beg;
+    }
     const int* __restrict__ ixp = indices + beg;
     const MT* __restrict__ vap = vals + beg;
     AT acc[CPL];
@@ -442,49 +488,49 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
     }
     if constexpr (UNR == 8) {
       // Software-pipelined: the indices and values of the next 8 entries of
-      // this lane are in flight while the X rows of the current 8 are
-      // gathered, so each step waits on one memory latency (the gather),
-      // not on the index load and then the gather. Entries past the row end
-      // read the row's own X row (always in range) and contribute zero.
-      constexpr int U = 8;
-      int qa[U], qb[U];
-      AT va[U], vb[U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        const int jj = zs + u * NZ;
-        qa[u] = jj < len ? stream_ld<NTL>(ixp + jj) : row;
-        va[u] = jj < len ? (AT)stream_ld<NTL>(vap + jj) : (AT)0;
-      }
+      // this lane (or, at the row's last chunk, the next row's first 8) are
+      // in flight while the X rows of the current 8 are gathered, so each
+      // step waits on one memory latency (the gather).
+      const int nrow = row + tpb;
+      const bool has_next = nrow < r1;
+      nbeg = has_next ? indptr[nrow] : 0;
+      nlen = has_next ? indptr[nrow + 1] - nbeg : 0;
+      const int* __restrict__ nixp = indices + nbeg;
+      const MT* __restrict__ nvap = vals + nbeg;
+      int qb[U];
+      AT vb[U];
       AT a1[CPL];
 #pragma unroll
       for (int m = 0; m < CPL; ++m) a1[m] = 0;
-      for (int j0 = zs; j0 < len; j0 += U * NZ) {
-        const int jn = j0 + U * NZ;
+      if (zs >= len) {
+        load_chunk(nixp, nvap, zs, nlen, row, qa, va);
+      } else {
+        for (int j0 = zs; j0 < len; j0 += U * NZ) {
+          const int jn = j0 + U * NZ;
+          if (jn < len)
+            load_chunk(ixp, vap, jn, len, row, qb, vb);
+          else
+            load_chunk(nixp, nvap, zs, nlen, row, qb, vb);
+          VT xv[U][CPL];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const int jj = jn + u * NZ;
-          qb[u] = jj < len ? stream_ld<NTL>(ixp + jj) : row;
-          vb[u] = jj < len ? (AT)stream_ld<NTL>(vap + jj) : (AT)0;
-        }
-        VT xv[U][CPL];
+          for (int u = 0; u < U; ++u) vload<VT, CPL>(X + (size_t)qa[u] * K + c0, xv[u]);
 #pragma unroll
-        for (int u = 0; u < U; ++u) vload<VT, CPL>(X + (size_t)qa[u] * K + c0, xv[u]);
+          for (int u = 0; u < U; ++u) {
+            const bool live = j0 + u * NZ < len;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const bool live = j0 + u * NZ < len;
-#pragma unroll
-          for (int m = 0; m < CPL; ++m) {
-            const AT t = live ? va[u] * (AT)xv[u][m] : (AT)0;
-            if (u & 1)
-              a1[m] += t;
-            else
-              acc[m] += t;
+            for (int m = 0; m < CPL; ++m) {
+              const AT t = live ? va[u] * (AT)xv[u][m] : (AT)0;
+              if (u & 1)
+                a1[m] += t;
+              else
+                acc[m] += t;
+            }
           }
-        }
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          qa[u] = qb[u];
-          va[u] = vb[u];
+          for (int u = 0; u < U; ++u) {
+            qa[u] = qb[u];
+            va[u] = vb[u];
+          }
         }
       }
 #pragma unroll
@@ -526,6 +572,7 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
       }
     }
   }
+  }  // sweep step
   if (partial) {
     // Sum over the lanes that own the same columns, then over waves.
 #pragma unroll
@@ -540,6 +587,243 @@ __global__ __launch_bounds__(256) void k_spmm(int n, const int* __restrict__ ind
       double s = 0.0;
       for (int w = 0; w < wpb; ++w) s += smem[w * K + threadIdx.x];
       partial[(size_t)b * K + threadIdx.x] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Window kernel for one right-hand side (k = 1). A block streams the nonzeros
+// of its row range (an equal share of the nonzeros, as in k_spmm) in windows
+// of kWin entries: every lane loads 4 consecutive entries per step with
+// 16-byte loads, whatever the row lengths, gathers their X values and writes
+// the products to LDS in nonzero order. Then 16-lane teams sum each row's
+// segment of the window in a fixed order (lane-strided, then a shuffle tree).
+// A row that crosses the window's end hands its partial sum to the next
+// window, so a row of any length is summed without atomics. The next
+// window's indices and values are loaded before the row sums of the current
+// one, so the matrix stream stays in flight through the sum phase.
+// ---------------------------------------------------------------------------
+
+typedef int ivec4 __attribute__((ext_vector_type(4)));
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+constexpr int kWinBlock = 512;                  // threads per block
+constexpr int kWinReps = 2;                     // 4-entry groups per lane and window
+constexpr int kWin = kWinBlock * 4 * kWinReps;  // 4096 nonzeros
+constexpr int kRowLanes = 16;                   // lanes per row team in the sum phase
+constexpr int kWinTeams = kWinBlock / kRowLanes;
+
+// Four consecutive matrix values (16-byte aligned).
+template <bool NTL>
+__device__ inline void load_vals4(const double* p, double* v) {
+  const dvec2 a = stream_ld<NTL>(reinterpret_cast<const dvec2*>(p));
+  const dvec2 c = stream_ld<NTL>(reinterpret_cast<const dvec2*>(p) + 1);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = c.x;
+  v[3] = c.y;
+}
+template <bool NTL>
+__device__ inline void load_vals4(const float* p, float* v) {
+  const fvec4 a = stream_ld<NTL>(reinterpret_cast<const fvec4*>(p));
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+}
+
+// One window's entries held by a lane: indices (-1 = no entry) and values.
+template <typename MT>
+struct WinRegs {
+  ivec4 q[kWinReps];
+  MT v[kWinReps][4];
+};
+
+// Load the lane's entries of the window [s, e) (s a multiple of 4); entries
+// before nb (the block's first) or at or past e are not loaded.
+template <typename MT, bool NTL>
+__device__ inline void win_load(WinRegs<MT>& w, const int* __restrict__ indices, const MT* __restrict__ vals, int s,
+                                int nb, int e, int t) {
+#pragma unroll
+  for (int rep = 0; rep < kWinReps; ++rep) {
+    const int j = s + (rep * kWinBlock + t) * 4;
+    if (s + rep * kWinBlock * 4 >= e) {
+      // A step past the window's end (uniform over the block).
+      w.q[rep] = ivec4{-1, -1, -1, -1};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) w.v[rep][u] = (MT)0;
+    } else if (j >= nb && j + 4 <= e) {
+      w.q[rep] = stream_ld<NTL>(reinterpret_cast<const ivec4*>(indices + j));
+      load_vals4<NTL>(vals + j, w.v[rep]);
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int jj = j + u;
+        const bool ok = jj >= nb && jj < e;
+        w.q[rep][u] = ok ? indices[jj] : -1;
+        w.v[rep][u] = ok ? vals[jj] : (MT)0;
+      }
+    }
+  }
+}
+
+template <typename MT, typename VT, int MODE, bool NTL>
+__global__ __launch_bounds__(kWinBlock) void k_spmv_win(int n, const int* __restrict__ indptr,
+                                                        const int* __restrict__ indices, const MT* __restrict__ vals,
+                                                        const VT* __restrict__ X, VT* __restrict__ Y,
+                                                        const double* __restrict__ B, double* __restrict__ partial,
+                                                        const ColState* __restrict__ st, int nsub) {
+  using AT = typename Accum<VT>::type;
+  __shared__ __align__(16) AT prod[kWin];
+  __shared__ double smem[kWinBlock / 64];
+  __shared__ int s_rows[2][2];
+  __shared__ int s_next[2];  // first unfinished row, the row s_carry belongs to (-1: none)
+  __shared__ AT s_carry;
+  if (all_stopped<1>(st)) return;
+  const int t = threadIdx.x;
+  const int lane = t & 63;
+  const int wave = t >> 6;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int chunk0 = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
+  const int NC = G * nsub;
+  const int team = t / kRowLanes;
+  const int tl = t % kRowLanes;
+  double dot = 0.0;
+  // Sweep steps as in k_spmm: chunk sub * G + chunk0 at step sub.
+  for (int sub = 0; sub < nsub; ++sub) {
+    const int chunk = sub * G + chunk0;
+    if (wave == 0) {
+      const long long base = indptr[0];
+      const long long total = (long long)indptr[n] - base;
+      const int a0 = chunk == 0 ? 0 : row_lower_bound(indptr, n, base + total * chunk / NC);
+      const int a1 = chunk == NC - 1 ? n : row_lower_bound(indptr, n, base + total * (chunk + 1) / NC);
+      if (lane == 0) {
+        s_rows[sub & 1][0] = a0;
+        s_rows[sub & 1][1] = a1;
+      }
+    }
+    __syncthreads();
+    const int r0 = s_rows[sub & 1][0];
+    const int r1 = s_rows[sub & 1][1];
+    if (r0 < r1) {
+      const int nb = indptr[r0];
+      const int ne = indptr[r1];
+      int rc = r0;          // first row not yet finished
+      int carry_row = -1;   // row whose partial sum carry holds
+      AT carry = 0;
+      int s = nb & ~3;      // window start, a multiple of 4 (16-byte loads)
+      int e = (ne - s > kWin) ? s + kWin : ne;
+      WinRegs<MT> w;
+      win_load<MT, NTL>(w, indices, vals, s, nb, e, t);
+      while (true) {
+        const bool last = e == ne;
+        VT xv[kWinReps][4];
+#pragma unroll
+        for (int rep = 0; rep < kWinReps; ++rep)
+#pragma unroll
+          for (int u = 0; u < 4; ++u) xv[rep][u] = w.q[rep][u] >= 0 ? X[w.q[rep][u]] : (VT)0;
+#pragma unroll
+        for (int rep = 0; rep < kWinReps; ++rep) {
+          AT* dst = prod + (rep * kWinBlock + t) * 4;
+#pragma unroll
+          for (int u = 0; u < 4; ++u) dst[u] = (AT)w.v[rep][u] * (AT)xv[rep][u];
+        }
+        __syncthreads();
+        // The first pass's row data, loaded before the next window's entries
+        // so that waiting for it does not wait for them. Row rc is always in
+        // this window (it starts before e), so the first pass always runs.
+        const int rf = rc + team;
+        int rbf = 0, ref = 0;
+        VT xf = (VT)0;
+        double bf = 0.0;
+        if (rf < r1) {
+          rbf = indptr[rf];
+          ref = indptr[rf + 1];
+          if (MODE == 0 && partial) xf = X[rf];
+          if (MODE == 1) bf = B[rf];
+        }
+        const int sn = e;
+        const int en = last ? e : ((ne - e > kWin) ? e + kWin : ne);
+        if (!last) win_load<MT, NTL>(w, indices, vals, sn, nb, en, t);
+        // Rows that start before the window's end (in the last window, every
+        // remaining row, empty trailing rows included). The valid rows form a
+        // prefix of [rc, r1); exactly one of them, the last, reaches past e
+        // or ends at e when this is not the last window.
+        for (int base = rc;; base += kWinTeams) {
+          const int r = base + team;
+          int rb, re;
+          VT xr;
+          double br;
+          if (base == rc) {
+            rb = rbf;
+            re = ref;
+            xr = xf;
+            br = bf;
+          } else {
+            if (base >= r1 || (!last && indptr[base] >= e)) break;
+            rb = r < r1 ? indptr[r] : 0;
+            re = r < r1 ? indptr[r + 1] : 0;
+            xr = (MODE == 0 && partial && r < r1) ? X[r] : (VT)0;
+            br = (MODE == 1 && r < r1) ? B[r] : 0.0;
+          }
+          const bool valid = r < r1 && (last || rb < e);
+          AT acc = 0;
+          if (valid) {
+            const int lo = max(rb, s) - s;
+            const int hi = min(re, e) - s;
+            for (int i = lo + tl; i < hi; i += kRowLanes) acc += prod[i];
+          }
+#pragma unroll
+          for (int o = kRowLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+          if (valid && tl == 0) {
+            if (r == carry_row) acc = carry + acc;
+            if (re > e) {
+              s_carry = acc;
+              s_next[0] = r;
+              s_next[1] = r;
+            } else {
+              if (!last && re == e) {
+                s_next[0] = r + 1;
+                s_next[1] = -1;
+              }
+              if (MODE == 0) {
+                const VT y = (VT)acc;
+                Y[r] = y;
+                if (partial) dot += (double)xr * (double)y;
+              } else {
+                const double y = br - (double)acc;
+                Y[r] = (VT)y;
+                dot += y * y;
+              }
+            }
+          }
+        }
+        if (last) break;
+        __syncthreads();
+        rc = s_next[0];
+        carry_row = s_next[1];
+        carry = s_carry;
+        s = sn;
+        e = en;
+        // The next window's products are written only after every thread has
+        // read s_next and s_carry: they are rewritten only after its barrier.
+      }
+      // The next step's products must not overwrite this window's before
+      // every team has summed its rows.
+      __syncthreads();
+    }
+  }
+  if (partial) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) dot += __shfl_xor(dot, o, 64);
+    if (lane == 0) smem[wave] = dot;
+    __syncthreads();
+    if (t == 0) {
+      double acc = 0.0;
+      for (int v = 0; v < kWinBlock / 64; ++v) acc += smem[v];
+      partial[b] = acc;
     }
   }
 }
@@ -756,9 +1040,9 @@ static int default_nz(double avg_row) {
 }
 
 template <typename MT, typename VT, int K, int NZ, int MODE, int CPL>
-static int spmm_launch_nz(int n, int unroll, int grid_req, int stream, const int* indptr, const int* indices,
-                          const MT* vals, const VT* X, VT* Y, const double* B, double* partial, const ColState* st,
-                          hipStream_t s, int* grid_out) {
+static int spmm_launch_nz(int n, int unroll, int grid_req, int stream, int nsub, const int* indptr,
+                          const int* indices, const MT* vals, const VT* X, VT* Y, const double* B, double* partial,
+                          const ColState* st, hipStream_t s, int* grid_out) {
   constexpr int T = (K / CPL) * NZ;
   const int teams_per_block = (kBlock / 64) * (64 / T);
   int grid = grid_req;
@@ -768,19 +1052,19 @@ static int spmm_launch_nz(int n, int unroll, int grid_req, int stream, const int
   if (grid < 1) grid = 1;
   if (unroll == 8 && stream)
     hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 8, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
-                       vals, X, Y, B, partial, st);
+                       vals, X, Y, B, partial, st, nsub);
   else if (unroll == 8)
     hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 8, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
-                       indices, vals, X, Y, B, partial, st);
+                       indices, vals, X, Y, B, partial, st, nsub);
   else if (unroll >= 4 && stream)
     hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
-                       vals, X, Y, B, partial, st);
+                       vals, X, Y, B, partial, st, nsub);
   else if (unroll >= 4)
     hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
-                       indices, vals, X, Y, B, partial, st);
+                       indices, vals, X, Y, B, partial, st, nsub);
   else
     hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
-                       indices, vals, X, Y, B, partial, st);
+                       indices, vals, X, Y, B, partial, st, nsub);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
@@ -794,11 +1078,12 @@ static int spmm_launch_cpl(int n, double avg_row, const int* indptr, const int* 
   const int nz = (cfg.nz > 0 && KL * cfg.nz <= 64) ? cfg.nz : default_nz<KL>(avg_row);
   const int unroll = cfg.unroll > 0 ? cfg.unroll : 4;
   const int grid = cfg.grid;
+  const int nsub = cfg.nsub > 0 ? cfg.nsub : 1;
 #define CC_NZ(NZV)                                                                                                  \
   if (nz == NZV && KL * NZV <= 64)                                                                                  \
-    return spmm_launch_nz<MT, VT, K, (KL * NZV <= 64 ? NZV : 1), MODE, CPL>(n, unroll, grid, cfg.stream, indptr,    \
-                                                                          indices, vals, X, Y, B, partial, st, s, \
-                                                                          grid_out);
+    return spmm_launch_nz<MT, VT, K, (KL * NZV <= 64 ? NZV : 1), MODE, CPL>(n, unroll, grid, cfg.stream, nsub,      \
+                                                                          indptr, indices, vals, X, Y, B, partial, \
+                                                                          st, s, grid_out);
   CC_NZ(64)
   CC_NZ(32)
   CC_NZ(16)
@@ -810,9 +1095,35 @@ static int spmm_launch_cpl(int n, double avg_row, const int* indptr, const int* 
   return CC_E_ARG;
 }
 
+template <typename MT, typename VT, int MODE>
+static int spmv_win_launch(int n, double avg_row, int grid_req, int stream, int nsub, const int* indptr,
+                           const int* indices, const MT* vals, const VT* X, VT* Y, const double* B, double* partial,
+                           const ColState* st, hipStream_t s, int* grid_out) {
+  int grid = grid_req;
+  if (grid <= 0) grid = (int)(avg_row * n / (2.0 * kWin)) + 1;
+  if (grid > kMaxParts) grid = kMaxParts;
+  if (grid >= 2 * kXcds) grid -= grid % kXcds;
+  if (grid < 1) grid = 1;
+  if (stream)
+    hipLaunchKernelGGL((k_spmv_win<MT, VT, MODE, true>), dim3(grid), dim3(kWinBlock), 0, s, n, indptr, indices, vals,
+                       X, Y, B, partial, st, nsub);
+  else
+    hipLaunchKernelGGL((k_spmv_win<MT, VT, MODE, false>), dim3(grid), dim3(kWinBlock), 0, s, n, indptr, indices, vals,
+                       X, Y, B, partial, st, nsub);
+  if (grid_out) *grid_out = grid;
+  return (int)hipGetLastError();
+}
+
 template <typename MT, typename VT, int K, int MODE>
 static int spmm_launch(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const VT* X,
                        VT* Y, const double* B, double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  if constexpr (K == 1) {
+    // The window kernel loads 4 entries at a time with 16-byte loads.
+    const SpmmConfig& cfg = spmm_cfg<VT>(1);
+    if (cfg.unroll == kUnrollWindow && ((uintptr_t)indices & 15) == 0 && ((uintptr_t)vals & 15) == 0)
+      return spmv_win_launch<MT, VT, MODE>(n, avg_row, cfg.grid, cfg.stream, cfg.nsub > 0 ? cfg.nsub : 1, indptr,
+                                           indices, vals, X, Y, B, partial, st, s, grid_out);
+  }
   int cpl = spmm_cfg<VT>(K).cpl > 0 ? spmm_cfg<VT>(K).cpl : default_cpl<VT, K>();
   if (cpl > K || cpl * (int)sizeof(VT) > 16) cpl = default_cpl<VT, K>();
   if (cpl >= 4 && sizeof(VT) == 4 && K >= 4)
@@ -830,6 +1141,16 @@ static double avg_row_len(int n, const int* indptr, hipStream_t s) {
   if (hipMemcpyAsync(&ends[1], indptr + n, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess) return 64.0;
   if (hipStreamSynchronize(s) != hipSuccess) return 64.0;
   return n > 0 ? double(ends[1] - ends[0]) / n : 0.0;
+}
+
+// The mean row length only chooses a default launch shape; reading it costs
+// a stream synchronisation, so the single-product entry points skip it when
+// the configured shape (set by the autotuner) does not need it.
+template <typename VT>
+static double avg_row_if_needed(int n, int k, const int* indptr, hipStream_t s) {
+  const SpmmConfig& cfg = spmm_cfg<VT>(k);
+  if (cfg.grid > 0 && (cfg.nz > 0 || (k == 1 && cfg.unroll == kUnrollWindow))) return 0.0;
+  return avg_row_len(n, indptr, s);
 }
 
 // Workspace carving.
@@ -1222,7 +1543,7 @@ int cc_profile_collect(void) {
 }
 
 int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
-                       int32_t cols_per_lane, int32_t stream_loads) {
+                       int32_t cols_per_lane, int32_t stream_loads, int32_t sweep_steps) {
   if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
   if (value_bytes != 4 && value_bytes != 8) return CC_E_ARG;
   if (cols_per_lane * value_bytes > 16) return CC_E_ARG;
@@ -1230,9 +1551,11 @@ int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t
   if (cols_per_lane > k) return CC_E_ARG;
   const int kl = k / (cols_per_lane > 0 ? cols_per_lane : 1);
   if (nz_lanes < 0 || kl * nz_lanes > 64 || (nz_lanes & (nz_lanes - 1)) != 0) return CC_E_ARG;
-  if (grid < 0 || grid > kMaxParts || (unroll != 0 && unroll != 1 && unroll != 4 && unroll != 8)) return CC_E_ARG;
+  if (grid < 0 || grid > kMaxParts) return CC_E_ARG;
+  if (unroll != 0 && unroll != 1 && unroll != 4 && unroll != 8 && !(unroll == kUnrollWindow && k == 1)) return CC_E_ARG;
   if (stream_loads != 0 && (stream_loads != 1 || unroll == 1)) return CC_E_ARG;
-  g_spmm_cfg[value_bytes == 4 ? 1 : 0][k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane, stream_loads};
+  if (sweep_steps < 0 || sweep_steps > kMaxSweep) return CC_E_ARG;
+  g_spmm_cfg[value_bytes == 4 ? 1 : 0][k] = SpmmConfig{nz_lanes, grid, unroll, cols_per_lane, stream_loads, sweep_steps};
   return 0;
 }
 
@@ -1241,8 +1564,8 @@ int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, co
   if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !x || !y))) return CC_E_ARG;
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  return spmm_launch<double, double, 1, 0>(n, avg_row_len(n, indptr, s), indptr, indices, vals, x, y, nullptr,
-                                           nullptr, nullptr, s, nullptr);
+  return spmm_launch<double, double, 1, 0>(n, avg_row_if_needed<double>(n, 1, indptr, s), indptr, indices, vals, x,
+                                           y, nullptr, nullptr, nullptr, s, nullptr);
 }
 
 int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, const float* vals, const float* x,
@@ -1250,8 +1573,8 @@ int cc_spmv_csr_f32(int32_t n, const int32_t* indptr, const int32_t* indices, co
   if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !x || !y))) return CC_E_ARG;
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  return spmm_launch<float, float, 1, 0>(n, avg_row_len(n, indptr, s), indptr, indices, vals, x, y, nullptr, nullptr,
-                                         nullptr, s, nullptr);
+  return spmm_launch<float, float, 1, 0>(n, avg_row_if_needed<float>(n, 1, indptr, s), indptr, indices, vals, x, y,
+                                         nullptr, nullptr, nullptr, s, nullptr);
 }
 
 int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
@@ -1259,7 +1582,8 @@ int cc_spmm_csr_f64(int32_t n, int32_t k, const int32_t* indptr, const int32_t* 
   if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !X || !Y))) return CC_E_ARG;
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const double avg = avg_row_len(n, indptr, s);
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
+  const double avg = avg_row_if_needed<double>(n, k, indptr, s);
   CC_DISPATCH_K(k, (spmm_launch<double, double, KK, 0>(n, avg, indptr, indices, vals, X, Y, nullptr, nullptr, nullptr,
                                                        s, nullptr)));
 }
@@ -1269,7 +1593,8 @@ int cc_spmm_csr_f32(int32_t n, int32_t k, const int32_t* indptr, const int32_t* 
   if (n < 0 || (n > 0 && (!indptr || !indices || !vals || !X || !Y))) return CC_E_ARG;
   if (n == 0) return 0;
   hipStream_t s = (hipStream_t)stream;
-  const double avg = avg_row_len(n, indptr, s);
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
+  const double avg = avg_row_if_needed<float>(n, k, indptr, s);
   CC_DISPATCH_K(k, (spmm_launch<float, float, KK, 0>(n, avg, indptr, indices, vals, X, Y, nullptr, nullptr, nullptr,
                                                      s, nullptr)));
 }

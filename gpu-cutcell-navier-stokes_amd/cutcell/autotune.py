@@ -27,6 +27,9 @@ from . import hip
 
 _MEM: dict = {}
 
+# unroll value that selects the nonzero-window SpMV kernel (k = 1).
+WINDOW_UNROLL = 16
+
 
 def cache_path() -> Path:
     p = os.environ.get("CUTCELL_AUTOTUNE_CACHE")
@@ -87,18 +90,25 @@ def _time_ms(fn, reps: int) -> float:
 # ---------------------------------------------------------------------------
 
 
-def spmm_candidates(k: int, mean_row: float, value_bytes: int = 8):
+def spmm_candidates(k: int, mean_row: float, value_bytes: int = 8, sweeps=(1, 8, 32)):
     """Launch shapes: vector width (right-hand sides per lane), nonzero slots
-    per row team, grid, unroll depth and nontemporal matrix loads."""
+    per row team, grid, unroll depth, nontemporal matrix loads and sweep steps
+    (chunks per block, taken grid-strided)."""
     cpls = [c for c in (1, 2, 4) if c <= k and c * value_bytes <= 16]
-    for cpl in cpls:
-        kl = k // cpl
-        nzs = [nz for nz in (64, 32, 16, 8, 4, 2, 1) if kl * nz <= 64 and nz <= max(mean_row, 1)]
-        nzs = nzs[:2] if nzs else [max(1, 64 // kl)]
-        for nz in nzs:
-            for grid in (512, 1024):
-                for unroll, stream in ((1, 0), (4, 0), (4, 1), (8, 0), (8, 1)):
-                    yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll, stream=stream)
+    for sweep in sweeps:
+        for cpl in cpls:
+            kl = k // cpl
+            nzs = [nz for nz in (64, 32, 16, 8, 4, 2, 1) if kl * nz <= 64 and nz <= max(mean_row, 1)]
+            nzs = nzs[:2] if nzs else [max(1, 64 // kl)]
+            for nz in nzs:
+                for grid in (512, 1024):
+                    for unroll, stream in ((1, 0), (4, 0), (4, 1), (8, 0), (8, 1)):
+                        yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll, stream=stream, sweep=sweep)
+        if k == 1:
+            # The nonzero-window kernel (row shape does not matter to it).
+            for grid in (512, 768, 1024):
+                for stream in (0, 1):
+                    yield dict(cpl=1, nz=64, grid=grid, unroll=WINDOW_UNROLL, stream=stream, sweep=sweep)
 
 
 def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool = False) -> dict:
@@ -108,11 +118,12 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     lib = hip.require_gpu()
     mean_row = A.nnz / max(A.n, 1)
     vb = 8 if precision == "fp64" else 4
-    key = f"spmm3|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
+    key = f"spmm5|{_arch()}|{precision}|k{k}|n{_bucket(A.n)}|row{_bucket(mean_row)}"
     table = _load()
     if key in table and not force:
         cfg = table[key]
-        lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"], cfg.get("stream", 0))
+        lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"], cfg.get("stream", 0),
+                               cfg.get("sweep", 0))
         return cfg
     dt = torch.float64 if precision == "fp64" else torch.float32
     X = torch.randn(A.n, k, dtype=dt, device=A.vals.device)
@@ -122,7 +133,8 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     fn = getattr(lib, fn_name)
     best = None
     for cand in spmm_candidates(k, mean_row, vb):
-        if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"], cand["stream"]) != 0:
+        if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"], cand["stream"],
+                                  cand["sweep"]) != 0:
             continue
         call = lambda: fn(A.n, k, hip._ptr(A.indptr), hip._ptr(A.indices), hip._ptr(vals), hip._ptr(X), hip._ptr(Y),
                           hip._stream())
@@ -130,7 +142,7 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
         if best is None or ms < best[0]:
             best = (ms, cand)
     cfg = dict(best[1], ms=best[0])
-    lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"], cfg["stream"])
+    lib.cc_spmm_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"], cfg["stream"], cfg["sweep"])
     _store(key, cfg)
     return cfg
 
@@ -139,7 +151,7 @@ def reset_spmm_config() -> None:
     lib = hip.load_library()
     for k in hip.SUPPORTED_K:
         for vb in (8, 4):
-            lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0)
+            lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0, 0)
 
 
 # ---------------------------------------------------------------------------

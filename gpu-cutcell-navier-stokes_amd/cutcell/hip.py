@@ -60,7 +60,7 @@ _SIGNATURES = {
     "cc_profile_reset": ([], ctypes.c_int),
     "cc_profile_read": ([_I32, _P, _P], ctypes.c_int),
     "cc_profile_collect": ([], ctypes.c_int),
-    "cc_spmm_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),
+    "cc_spmm_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),
     "cc_spmm_csr_f32": ([_I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_sumfact_workspace_bytes": ([_P, _I32], _SZ),
     "cc_sumfact_apply": ([_P, _P, _P, _P, _SZ, _P], ctypes.c_int),

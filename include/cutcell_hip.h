@@ -86,10 +86,15 @@ int cc_profile_collect(void);
  * power of two; 0 = from the mean row length), grid blocks (0 = one block per
  * few rows up to 1024), unroll 1, 4 or 8 (8 = software-pipelined: the next
  * entries' indices and values load while the current X rows are gathered;
- * 0 = 4), stream_loads 1 = nontemporal loads of the matrix arrays (not with
- * unroll 1), 0 = ordinary loads. */
+ * 16 = for k = 1 only, the nonzero-window kernel: 4 entries per lane per step
+ * with 16-byte loads and per-row sums of the products in LDS, used when
+ * indices and vals are 16-byte aligned; 0 = 4), stream_loads 1 = nontemporal
+ * loads of the matrix arrays (not with unroll 1), 0 = ordinary loads;
+ * sweep_steps (0..64, 0 = 1): the rows are cut into grid * sweep_steps
+ * chunks of equal nonzero counts and block b takes chunks b, b + grid, ...,
+ * so that the grid streams one compact stretch of the matrix at a time. */
 int cc_spmm_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
-                       int32_t cols_per_lane, int32_t stream_loads);
+                       int32_t cols_per_lane, int32_t stream_loads, int32_t sweep_steps);
 
 /* y = A x, float64 (replaces the cuSPARSE product A_gpu @ p). */
 int cc_spmv_csr_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,

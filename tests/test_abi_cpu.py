@@ -54,22 +54,27 @@ def test_spmm_config_is_validated_per_vector_width():
 
     lib = hip.load_library()
     try:
-        assert lib.cc_spmm_set_config(16, 8, 16, 0, 0, 4, 0) == -1
-        assert lib.cc_spmm_set_config(16, 4, 16, 0, 0, 4, 0) == 0
-        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 0) == 0
-        assert lib.cc_spmm_set_config(16, 8, 16, 512, 4, 2, 0) == -1  # 8 x 16 lanes > 64
-        assert lib.cc_spmm_set_config(3, 8, 0, 0, 0, 0, 0) == -1
-        assert lib.cc_spmm_set_config(16, 2, 0, 0, 0, 0, 0) == -1
-        assert lib.cc_spmm_set_config(16, 8, 0, 0, 3, 0, 0) == -1
-        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 1) == 0
-        assert lib.cc_spmm_set_config(16, 8, 8, 512, 1, 2, 1) == -1  # nontemporal needs unroll 4
-        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 2) == -1
-        assert lib.cc_spmm_set_config(16, 8, 8, 512, 8, 2, 1) == 0
-        assert lib.cc_spmm_set_config(16, 8, 8, 512, 2, 2, 0) == -1
+        assert lib.cc_spmm_set_config(16, 8, 16, 0, 0, 4, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 4, 16, 0, 0, 4, 0, 0) == 0
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 0, 0) == 0
+        assert lib.cc_spmm_set_config(16, 8, 16, 512, 4, 2, 0, 0) == -1  # 8 x 16 lanes > 64
+        assert lib.cc_spmm_set_config(3, 8, 0, 0, 0, 0, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 2, 0, 0, 0, 0, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 8, 0, 0, 3, 0, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 1, 0) == 0
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 1, 2, 1, 0) == -1  # nontemporal needs unroll 4
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 2, 0) == -1
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 8, 2, 1, 0) == 0
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 2, 2, 0, 0) == -1
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 0, 64) == 0  # sweep steps 0..64
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 0, 65) == -1
+        assert lib.cc_spmm_set_config(16, 8, 8, 512, 4, 2, 0, -1) == -1
+        assert lib.cc_spmm_set_config(1, 8, 64, 0, 16, 1, 1, 8) == 0  # window kernel: k = 1 only
+        assert lib.cc_spmm_set_config(2, 8, 32, 0, 16, 1, 0, 0) == -1
     finally:
         for k in hip.SUPPORTED_K:
             for vb in (8, 4):
-                assert lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0) == 0
+                assert lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0, 0) == 0
 
 
 def test_gpu_backend_without_gpu_reports_unavailable():

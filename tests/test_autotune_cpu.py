@@ -10,7 +10,10 @@ def test_spmm_candidates_respect_the_wave():
             assert cands
             for c in cands:
                 assert (k // c["cpl"]) * c["nz"] <= 64 and c["cpl"] * vb <= 16
-                assert c["grid"] in (256, 512, 1024) and c["unroll"] in (1, 4, 8)
+                assert c["grid"] in (256, 512, 768, 1024)
+                assert c["unroll"] in (1, 4, 8) or (c["unroll"] == autotune.WINDOW_UNROLL and k == 1)
+            # the nonzero-window kernel is a candidate exactly for one right-hand side
+            assert any(c["unroll"] == autotune.WINDOW_UNROLL for c in cands) == (k == 1)
 
 
 def test_batched_candidates_fit_lds():

@@ -88,15 +88,30 @@ def _ragged(n=700, seed=5):
     return A
 
 
-@pytest.mark.parametrize("k,vb", [(1, 8), (16, 8), (16, 4), (4, 4)])
+def _long_rows(n=12000, seed=9):
+    """Short rows, empty rows and a few rows longer than the window kernel's
+    4096-entry window (their sums cross two or more windows)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 12, n)
+    lens[::53] = 0
+    lens[[7, 2500, 2501, n - 1]] = [5000, 4096, 9000, 4500]
+    rows = np.repeat(np.arange(n), lens)
+    cols = np.concatenate([rng.choice(n, size=m, replace=False) for m in lens])
+    A = sp.csr_matrix((rng.standard_normal(rows.size), (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("k,vb", [(1, 8), (1, 4), (16, 8), (16, 4), (4, 4)])
 def test_spmm_every_launch_shape(hip, systems, k, vb):
     """Each launch shape the autotuner may pick (vector width, nonzero slots,
-    grid, unroll 1/4/8, nontemporal loads) against the oracle SpMV, on a
-    B-spline system and on a ragged matrix with empty rows."""
+    grid, unroll 1/4/8, nontemporal loads, and for k = 1 the nonzero-window
+    kernel) against the oracle SpMV, on a B-spline system, on a ragged matrix
+    with empty rows and on one with rows longer than a window."""
     from cutcell.autotune import spmm_candidates
 
     lib = hip.load_library()
-    mats = {"sliver-16": systems["sliver-16"].A, "ragged": _ragged()}
+    mats = {"sliver-16": systems["sliver-16"].A, "ragged": _ragged(), "long-rows": _long_rows()}
     try:
         for name, A in mats.items():
             Ad = hip.DeviceCSR.from_scipy(A, with_f32=(vb == 4))
@@ -110,14 +125,42 @@ def test_spmm_every_launch_shape(hip, systems, k, vb):
             seen = 0
             for cand in spmm_candidates(k, A.nnz / A.shape[0], vb):
                 if lib.cc_spmm_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"],
-                                          cand["stream"]) != 0:
+                                          cand["stream"], cand["sweep"]) != 0:
                     continue
                 Y = hip.spmm(Ad, _dev(X)).cpu().numpy() if k > 1 else hip.spmv(Ad, _dev(X[:, 0])).cpu().numpy()[:, None]
                 assert np.abs(Y - ref).max() <= tol * scale, (name, cand)
                 seen += 1
             assert seen >= 5
     finally:
-        lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0)
+        lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("stream,sweep", [(0, 1), (1, 8)])
+def test_window_spmv_in_pcg_and_residual(hip, systems, stream, sweep):
+    """The nonzero-window kernel as the product of the single-right-hand-side
+    CG loop (with the fused p.Ap partials) and of the float64 true residual:
+    the same trajectory bar as the row kernels, and the device residual equal
+    to the host one."""
+    from cutcell.autotune import WINDOW_UNROLL
+
+    lib = hip.load_library()
+    s = systems["C1-oblique-5deg"]
+    Ad = hip.DeviceCSR.from_scipy(s.A)
+    try:
+        assert lib.cc_spmm_set_config(1, 8, 64, 0, WINDOW_UNROLL, 1, stream, sweep) == 0
+        xd, res, hist = hip.pcg(Ad, _dev(s.b), tol=1e-8, maxiter=5000, history=True)
+        r = res[0]
+        x_or, it_or, st_or, _, h_or = oracle.pcg_gpu_loop(s.A, s.b, tol=1e-8, history=True)
+        assert r.status == hip.CC_CONVERGED == st_or
+        check_trajectory(hist[:, 0], r.iterations, h_or, it_or, 1e-8)
+        x = xd.cpu().numpy()
+        assert np.linalg.norm(x - x_or) / np.linalg.norm(x_or) < 1e-6
+        assert r.true_rel == pytest.approx(np.linalg.norm(s.A @ x - s.b) / np.linalg.norm(s.b), rel=1e-6)
+        X = np.random.default_rng(3).standard_normal(s.A.shape[0])
+        rel = hip.residual(Ad, _dev(s.b), _dev(X))[0]
+        assert rel == pytest.approx(np.linalg.norm(s.b - s.A @ X) / np.linalg.norm(s.b), rel=1e-12)
+    finally:
+        lib.cc_spmm_set_config(1, 8, 0, 0, 0, 0, 0, 0)
 
 
 def test_jacobi_diag(hip, systems):
