@@ -11,8 +11,12 @@ Methods timed:
 - ``fp64``: float64 Jacobi-PCG, the 16 columns share each SpMM;
 - ``mixed``: float32 values/vectors in the inner Jacobi-PCG, float64 residual,
   dots and correction (iterative refinement); stops on the float64 true
-  residual <= 1e-8.
-``value`` is the faster of the two among those whose every column reaches the
+  residual <= 1e-8;
+- ``fp64+rb`` and ``mixed+rb``: the same solves with the CG products taken
+  from the row-block form of the matrix (groups of 4 rows sharing one gather
+  of each X row; built once per matrix on the device, its build time is
+  reported beside the method and is not part of a step).
+``value`` is the fastest method among those whose every column reaches the
 float64 true residual <= 1e-8.
 
 Also reported: the dominant kernel's roofline (algorithmic bytes per SpMM
@@ -50,9 +54,17 @@ HBM_PEAK_GBS = 8000.0
 # 256 MiB copy); keyed by kernel, nnz and n so that another workload reports
 # null.
 PROFILED_TRAFFIC = {
-    ("k_spmm<float,k=16>", 27369621, 81195): 237_187_000,
-    ("k_spmm<double,k=16>", 27369621, 81195): 480_284_000,
+    ("k_spmm<float,k=16>", 27369621, 81195): 242_892_000,
+    ("k_spmm<double,k=16>", 27369621, 81195): 481_145_000,
+    ("k_spmm_rb<float,k=16>", 27369621, 81195): 225_142_000,
+    ("k_spmm_rb<double,k=16>", 27369621, 81195): 453_754_000,
 }
+
+
+def rb_bytes(rb, k: int, value_bytes: int, vec_bytes: int) -> int:
+    """Algorithmic bytes of one row-block product: group offsets, union
+    columns and padded values once, x read and y written once."""
+    return int(rb.bytes(value_bytes) + 2 * k * rb.n * vec_bytes)
 TOL = 1.0e-8
 NRHS = 16
 
@@ -96,17 +108,23 @@ def sig(v, digits: int = 4):
     return v
 
 
-def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int, tune: bool = True):
-    kw = dict(tol=TOL, maxiter=20000, precision=method)
-    if method == "mixed":
+def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int, tune: bool = True, rb=None):
+    precision = method.split("+")[0]
+    use_rb = method.endswith("+rb")
+    kw = dict(tol=TOL, maxiter=20000, precision=precision)
+    if precision == "mixed":
         kw.update(inner_tol=1e-4, max_outer=12)
+    if use_rb:
+        kw.update(rb=rb)
     ws = hip.Workspace()
     X = torch.empty_like(Bd)
     tuned = None
     if tune:
-        from cutcell.autotune import spmm_config
+        from cutcell.autotune import rb_config, spmm_config
 
-        tuned = spmm_config(Ad, Bd.shape[1], method)
+        tuned = spmm_config(Ad, Bd.shape[1], precision)
+        if use_rb:
+            tuned = dict(csr=tuned, rb=rb_config(rb, Bd.shape[1], precision))
     for _ in range(warmup):
         hip.pcg(Ad, Bd, X=X, workspace=ws, **kw)
     torch.cuda.synchronize()
@@ -121,7 +139,24 @@ def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int, tune: 
     upd_ms, upd_cnt = hip.profile_read("update")
     dir_ms, dir_cnt = hip.profile_read("direction")
     hip.profile_enable(False)
-    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= TOL * (1.0 if method == "mixed" else 10.0) for r in results)
+    # The per-launch events above add their own dispatch latency (a few us)
+    # to each interval. The same product launched back to back between two
+    # events on the launch stream is reported beside it; it reads a matrix
+    # that the solve's other kernels have not just evicted from the caches.
+    P = (Bd if precision == "fp64" else Bd.to(torch.float32)).contiguous()
+    Q = torch.empty_like(P)
+    prod = (lambda: hip.spmm_rb(rb, P, Q)) if use_rb else (lambda: hip.spmm(Ad, P, Q))
+    for _ in range(3):
+        prod()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(20):
+        prod()
+    e1.record()
+    torch.cuda.synchronize()
+    spmm_b2b_us = 1e3 * e0.elapsed_time(e1) / 20
+    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= TOL * (1.0 if precision == "mixed" else 10.0)
+             for r in results)
     return dict(
         method=method,
         elapsed_s=elapsed,
@@ -132,6 +167,7 @@ def time_method(hip, torch, Ad, Bd, method: str, steps: int, warmup: int, tune: 
         outer=[int(r.outer) for r in results],
         true_rel_max=float(max(r.true_rel for r in results)),
         spmm_avg_us=1e3 * spmm_ms / max(spmm_cnt, 1),
+        spmm_back_to_back_us=spmm_b2b_us,
         spmm_launches=spmm_cnt,
         update_avg_us=1e3 * upd_ms / max(upd_cnt, 1),
         direction_avg_us=1e3 * dir_ms / max(dir_cnt, 1),
@@ -424,7 +460,7 @@ def main(argv=None) -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--target", type=int, default=81_000)
-    ap.add_argument("--methods", default="fp64,mixed")
+    ap.add_argument("--methods", default="fp64,mixed,fp64+rb,mixed+rb")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-systems section")
@@ -481,10 +517,20 @@ def main(argv=None) -> int:
         log(f"copy probe: {words * 8} bytes read and written once")
         del src, dst
 
+    rb, rb_build_s = None, None
+    if any(m.endswith("+rb") for m in args.methods.split(",")):
+        t0 = time.perf_counter()
+        rb = hip.DeviceRowBlocks.from_csr(Ad)
+        torch.cuda.synchronize()
+        rb_build_s = time.perf_counter() - t0
+        log(f"row-block form: {rb.nu} union columns ({rb.nu / nnz:.3f} of nnz), built in {rb_build_s * 1e3:.1f} ms")
+
     runs = {}
     for m in args.methods.split(","):
         log(f"timing {m}: warmup={args.warmup} steps={args.steps}")
-        runs[m] = time_method(hip, torch, Ad, Bd, m, args.steps, args.warmup, tune=not args.no_tune)
+        runs[m] = time_method(hip, torch, Ad, Bd, m, args.steps, args.warmup, tune=not args.no_tune, rb=rb)
+        if m.endswith("+rb"):
+            runs[m]["rb_build_s"] = rb_build_s
         r = runs[m]
         log(f"{m}: {r['solves_per_s']:.1f} solves/s, {r['ms_per_step']:.1f} ms/step, converged={r['converged']}, "
             f"iters {min(r['iterations'])}-{max(r['iterations'])}, spmm {r['spmm_avg_us']:.1f} us")
@@ -500,15 +546,27 @@ def main(argv=None) -> int:
 
     valid = {m: r for m, r in runs.items() if r["converged"]}
     head = max(valid.values(), key=lambda r: r["solves_per_s"]) if valid else next(iter(runs.values()))
-    value_bytes = 8 if head["method"] == "fp64" else 4
-    vec_bytes = 8 if head["method"] == "fp64" else 4
-    traffic_alg = spmm_bytes(n, nnz, NRHS, value_bytes, vec_bytes)
+    head_prec = head["method"].split("+")[0]
+    value_bytes = 8 if head_prec == "fp64" else 4
+    vec_bytes = 8 if head_prec == "fp64" else 4
+    csr_alg = spmm_bytes(n, nnz, NRHS, value_bytes, vec_bytes)
+    tname = "double" if value_bytes == 8 else "float"
+    if head["method"].endswith("+rb"):
+        # The product reads the row-block form: its own stored bytes are the
+        # algorithmic bytes; the CSR figure is kept for comparison.
+        traffic_alg = rb_bytes(rb, NRHS, value_bytes, vec_bytes)
+        kname = f"k_spmm_rb<{tname},k=16>"
+    else:
+        traffic_alg = csr_alg
+        kname = f"k_spmm<{tname},k=16>"
     achieved = traffic_alg / (head["spmm_avg_us"] * 1e-6) / 1e9
-    kname = f"k_spmm<{'double' if value_bytes == 8 else 'float'},k=16>"
     roofline = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=PROFILED_TRAFFIC.get((kname, nnz, n)),
                     kernel=kname,
-                    bytes_per_launch=traffic_alg, avg_us=round(head["spmm_avg_us"], 2))
+                    bytes_per_launch=traffic_alg, avg_us=round(head["spmm_avg_us"], 2),
+                    avg_us_back_to_back=round(head["spmm_back_to_back_us"], 2),
+                    csr_bytes_per_launch=csr_alg,
+                    csr_equivalent_GBps=round(csr_alg / (head["spmm_avg_us"] * 1e-6) / 1e9, 1))
 
     log("single-vector SpMV bandwidth on the workload matrix ...")
     spmv = spmv_bandwidth(hip, torch, Ad)
@@ -546,7 +604,7 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head["method"] == "fp64" else "f32+f64",
+        "dtype": "f64" if head_prec == "fp64" else "f32+f64",
         "data": "synthetic: seeded random planar cut, Gaussian right-hand sides",
         "config": {
             "workload": f"3D cut-box screened Poisson, p=3, planar cut seed {args.seed}, N={system.meta['n_elements']}^3 "

@@ -113,6 +113,16 @@ size_t fcg_operator_ml_bytes(size_t n, const cc_mlcnn_desc& ml);
 bool ml_desc_ok(const cc_mlcnn_desc& ml);
 }  // namespace cc
 
+struct cc_rb_desc;
+namespace cc {
+// Row-block product Y = A X (rowblock.hip); partial[block][c] gets X.Y sums
+// when not NULL. MT = VT = double or float; K in {1, 2, 4, 8, 16}.
+template <typename MT, typename VT, int K>
+int rb_spmm(const cc_rb_desc& rb, const VT* X, VT* Y, double* partial, const ColState* st, hipStream_t s,
+            int* grid_out);
+bool rb_desc_ok(const cc_rb_desc* rb, int n);
+}  // namespace cc
+
 #define CC_CHECK(expr)             \
   do {                             \
     int cc_err_ = (int)(expr);     \

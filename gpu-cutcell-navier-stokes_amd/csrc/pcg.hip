@@ -37,11 +37,10 @@
 
 #include "../../include/cutcell_hip.h"
 #include "common.h"
+#include "spmm_util.h"
 
 namespace cc {
 
-constexpr int kBlock = 256;
-constexpr int kXcds = 8;
 
 // ---------------------------------------------------------------------------
 // Optional per-kernel-class event timing (process-wide, not thread-safe).
@@ -252,15 +251,6 @@ __device__ inline void finalize_beta(const double* __restrict__ partial, int npa
   }
 }
 
-template <int K>
-__device__ inline bool all_stopped(const ColState* st) {
-  if (st == nullptr) return false;
-#pragma unroll
-  for (int c = 0; c < K; ++c)
-    if (st[c].status == CC_RUNNING) return false;
-  return true;
-}
-
 // ---------------------------------------------------------------------------
 // Kernels.
 // ---------------------------------------------------------------------------
@@ -275,98 +265,10 @@ __global__ __launch_bounds__(256) void k_diag(int n, const int* __restrict__ ind
   diag[row] = d;
 }
 
-// Vector access of CPL consecutive right-hand sides (one 8- or 16-byte load).
-template <typename VT, int CPL>
-struct VecT;
-template <>
-struct VecT<double, 1> {
-  using T = double;
-};
-template <>
-struct VecT<double, 2> {
-  using T = double2;
-};
-template <>
-struct VecT<float, 1> {
-  using T = float;
-};
-template <>
-struct VecT<float, 2> {
-  using T = float2;
-};
-template <>
-struct VecT<float, 4> {
-  using T = float4;
-};
-
-template <typename VT, int CPL>
-__device__ inline void vload(const VT* __restrict__ p, VT* out) {
-  if constexpr (CPL == 1) {
-    out[0] = *p;
-  } else {
-    using V = typename VecT<VT, CPL>::T;
-    const V v = *reinterpret_cast<const V*>(p);
-    out[0] = v.x;
-    out[1] = v.y;
-    if constexpr (CPL == 4) {
-      out[2] = v.z;
-      out[3] = v.w;
-    }
-  }
-}
-
-template <typename VT, int CPL>
-__device__ inline void vstore(VT* __restrict__ p, const VT* in) {
-  if constexpr (CPL == 1) {
-    *p = in[0];
-  } else {
-    using V = typename VecT<VT, CPL>::T;
-    V v;
-    v.x = in[0];
-    v.y = in[1];
-    if constexpr (CPL == 4) {
-      v.z = in[2];
-      v.w = in[3];
-    }
-    *reinterpret_cast<V*>(p) = v;
-  }
-}
-
 // MODE 0: Y = A X, partial[block][c] = sum_rows X.Y      (p.q of CG)
 // MODE 1: Y = B - A X, partial[block][c] = sum_rows Y.Y  (true residual)
 // A row team has KL = K/CPL lanes per nonzero slot (each lane owns CPL
 // consecutive right-hand sides, loaded as one vector) and NZ slots.
-// First row r in [0, n] with indptr[r] >= target (indptr is nondecreasing and
-// indptr[n] >= target), found by one wave with a 64-ary search: every round
-// probes 64 evenly spaced row pointers and keeps the bracket of the first
-// one at or past the target. Three rounds cover 262k rows. Every lane of the
-// wave returns the same row.
-__device__ inline int row_lower_bound(const int* __restrict__ indptr, int n, long long target) {
-  const int lane = threadIdx.x & 63;
-  int lo = 0, hi = n;
-  while (hi - lo > 64) {
-    const int step = (hi - lo + 63) / 64;
-    const int idx = min(lo + lane * step, hi);
-    const unsigned long long m = __ballot(indptr[idx] >= target);
-    const int f = m ? __ffsll((long long)m) - 1 : 64;
-    const int nlo = f == 0 ? lo : lo + (f - 1) * step;
-    const int nhi = f == 64 ? hi : min(lo + f * step, hi);
-    lo = nlo;
-    hi = nhi;
-  }
-  const int idx = lo + lane;
-  const unsigned long long m = __ballot(idx <= hi && indptr[min(idx, hi)] >= target);
-  return m ? lo + __ffsll((long long)m) - 1 : hi;
-}
-
-template <bool NTL, typename T>
-__device__ inline T stream_ld(const T* p) {
-  if constexpr (NTL)
-    return __builtin_nontemporal_load(p);
-  else
-    return *p;
-}
-
 // NTL: load the matrix (read once per product) with nontemporal loads so it
 // does not evict the gathered X rows from L2; chosen by the autotuner.
 // nsub: the rows are cut into gridDim.x * nsub chunks of equal nonzero
@@ -1253,13 +1155,15 @@ static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_
   return 0;
 }
 
+// rb (may be NULL): take the products from the row-block form instead.
 template <typename MT, typename VT, int K>
 static int cg_run(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const double* src,
                   const double* inv_scale, const int* run_mask, CgBuffers<VT>& b, int precond, double tol,
                   int maxiter, int check_every, hipStream_t s, std::vector<ColState>& host_st,
-                  double* hist = nullptr) {
+                  double* hist = nullptr, const cc_rb_desc* rb = nullptr) {
   const ColState* st = b.st;
   auto apply = [&](const VT* P, VT* Q, double* partial, int* gs) {
+    if (rb) return rb_spmm<MT, VT, K>(*rb, P, Q, partial, st, s, gs);
     return spmm_launch<MT, VT, K, 0>(n, avg_row, indptr, indices, vals, P, Q, nullptr, partial, st, s, gs);
   };
   return cg_loop<VT, K>(n, apply, src, inv_scale, run_mask, b, precond, tol, maxiter, check_every, s, host_st, hist);
@@ -1391,7 +1295,7 @@ static int true_residual(int n, double avg_row, const int* indptr, const int* in
 template <int K>
 static int pcg_f64(int n, const int* indptr, const int* indices, const double* vals, const double* B, double* X,
                    double tol, int maxiter, int precond, int check_every, void* ws, size_t wsb,
-                   cc_pcg_result* results, double* hist, hipStream_t s) {
+                   cc_pcg_result* results, double* hist, hipStream_t s, const cc_rb_desc* rb = nullptr) {
   Carve cv{(char*)ws, 0, wsb};
   CgBuffers<double> b = carve_cg<double>(cv, n, K, X);
   if (!cv.ok()) return CC_E_WORKSPACE;
@@ -1402,7 +1306,7 @@ static int pcg_f64(int n, const int* indptr, const int* indices, const double* v
   }
   std::vector<ColState> hs;
   CC_CHECK((cg_run<double, double, K>(n, avg, indptr, indices, vals, B, nullptr, nullptr, b, precond, tol, maxiter,
-                                      check_every, s, hs, hist)));
+                                      check_every, s, hs, hist, rb)));
   std::vector<double> rel(K);
   CC_CHECK(true_residual<K>(n, avg, indptr, indices, vals, B, X, b.R, b.pa, s, rel.data()));
   for (int c = 0; c < K; ++c) {
@@ -1431,7 +1335,8 @@ static size_t mixed_bytes(size_t n) {
 template <int K>
 static int pcg_mixed(int n, const int* indptr, const int* indices, const double* vals, const float* vals32,
                      const double* B, double* X, double tol, int maxiter, double inner_tol, int max_outer,
-                     int precond, int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s) {
+                     int precond, int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
+                     const cc_rb_desc* rb = nullptr) {
   Carve cv{(char*)ws, 0, wsb};
   CgBuffers<float> b = carve_cg<float>(cv, n, K, nullptr);
   double* R64 = cv.take<double>((size_t)n * K);
@@ -1460,7 +1365,7 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
     for (int c = 0; c < K; ++c) any |= ho[c].status == CC_RUNNING;
     if (!any || used >= maxiter) break;
     CC_CHECK((cg_run<float, float, K>(n, avg, indptr, indices, vals32, R64, inv_scale, run_mask, b, precond,
-                                      inner_tol, maxiter - used, check_every, s, hs)));
+                                      inner_tol, maxiter - used, check_every, s, hs, nullptr, rb)));
     int most = 0;
     for (int c = 0; c < K; ++c)
       if (ho[c].status == CC_RUNNING && hs[c].iterations > most) most = hs[c].iterations;
@@ -1658,6 +1563,31 @@ int cc_pcg_jacobi_mixed(int32_t n, int32_t k, const int32_t* indptr, const int32
   hipStream_t s = (hipStream_t)stream;
   CC_DISPATCH_K(k, (pcg_mixed<KK>(n, indptr, indices, vals, vals32, B, X, tol, maxiter, inner_tol, max_outer,
                                   preconditioner, check_every, workspace, workspace_bytes, results, s)));
+}
+
+int cc_pcg_jacobi_f64_rb(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                         const cc_rb_desc* rb, const double* B, double* X, double tol, int32_t maxiter,
+                         int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                         cc_pcg_result* results, double* history, cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !B || !X || !results || !workspace || maxiter < 0) return CC_E_ARG;
+  if (!rb_desc_ok(rb, n) || !rb->gval) return CC_E_ARG;
+  if (workspace_bytes < cc_pcg_workspace_bytes(n, k, 0)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  CC_DISPATCH_K(k, (pcg_f64<KK>(n, indptr, indices, vals, B, X, tol, maxiter, preconditioner, check_every, workspace,
+                                workspace_bytes, results, history, s, rb)));
+}
+
+int cc_pcg_jacobi_mixed_rb(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                           const cc_rb_desc* rb, const double* B, double* X, double tol, int32_t maxiter,
+                           double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
+                           void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !B || !X || !results || !workspace || maxiter < 0) return CC_E_ARG;
+  if (!rb_desc_ok(rb, n) || !rb->gval32) return CC_E_ARG;
+  if (workspace_bytes < cc_pcg_workspace_bytes(n, k, 1)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  // The inner solve never reads the CSR float32 values when rb is given.
+  CC_DISPATCH_K(k, (pcg_mixed<KK>(n, indptr, indices, vals, nullptr, B, X, tol, maxiter, inner_tol, max_outer,
+                                  preconditioner, check_every, workspace, workspace_bytes, results, s, rb)));
 }
 
 }  // extern "C"

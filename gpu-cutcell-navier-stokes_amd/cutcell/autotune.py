@@ -18,6 +18,7 @@ solve capped at a few tens of iterations).
 
 from __future__ import annotations
 
+import ctypes
 import json
 import math
 import os
@@ -147,11 +148,55 @@ def spmm_config(A: "hip.DeviceCSR", k: int, precision: str = "fp64", force: bool
     return cfg
 
 
+def rb_candidates(k: int, value_bytes: int = 8):
+    """Row-block product shapes: vector width, union-column slots per group
+    team, grid and unroll depth."""
+    for cpl in [c for c in (1, 2, 4) if c <= k and c * value_bytes <= 16]:
+        kl = k // cpl
+        for nz in [nz for nz in (16, 8, 4, 2) if kl * nz <= 64]:
+            for grid in (512, 1024):
+                for unroll in (1, 4):
+                    yield dict(cpl=cpl, nz=nz, grid=grid, unroll=unroll)
+
+
+def rb_config(rb: "hip.DeviceRowBlocks", k: int, precision: str = "fp64", force: bool = False) -> dict:
+    """Pick and apply the row-block product shape for ``rb`` with ``k``
+    right-hand sides."""
+    import torch
+
+    lib = hip.require_gpu()
+    vb = 8 if precision == "fp64" else 4
+    mean_cols = rb.nu / max(rb.ngroups, 1)
+    key = f"rb1|{_arch()}|{precision}|k{k}|n{_bucket(rb.n)}|cols{_bucket(mean_cols)}"
+    table = _load()
+    if key in table and not force:
+        cfg = table[key]
+        lib.cc_rb_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
+        return cfg
+    dt = torch.float64 if precision == "fp64" else torch.float32
+    X = torch.randn(rb.n, k, dtype=dt, device=rb.gptr.device)
+    Y = torch.empty_like(X)
+    fn = lib.cc_spmm_rb_f64 if precision == "fp64" else lib.cc_spmm_rb_f32
+    best = None
+    for cand in rb_candidates(k, vb):
+        if lib.cc_rb_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) != 0:
+            continue
+        call = lambda: fn(ctypes.addressof(rb.desc), k, hip._ptr(X), hip._ptr(Y), hip._stream())
+        ms = _time_ms(call, 5)
+        if best is None or ms < best[0]:
+            best = (ms, cand)
+    cfg = dict(best[1], ms=best[0])
+    lib.cc_rb_set_config(k, vb, cfg["nz"], cfg["grid"], cfg["unroll"], cfg["cpl"])
+    _store(key, cfg)
+    return cfg
+
+
 def reset_spmm_config() -> None:
     lib = hip.load_library()
     for k in hip.SUPPORTED_K:
         for vb in (8, 4):
             lib.cc_spmm_set_config(k, vb, 0, 0, 0, 0, 0, 0)
+            lib.cc_rb_set_config(k, vb, 0, 0, 0, 0)
 
 
 # ---------------------------------------------------------------------------

@@ -84,7 +84,34 @@ _SIGNATURES = {
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P],
         ctypes.c_int,
     ),
+    "cc_rb_build_count": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_rb_build_fill": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_rb_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),
+    "cc_spmm_rb_f64": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "cc_spmm_rb_f32": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "cc_pcg_jacobi_f64_rb": (
+        [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int,
+    ),
+    "cc_pcg_jacobi_mixed_rb": (
+        [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P],
+        ctypes.c_int,
+    ),
 }
+
+RB_ROWS = 4  # CC_RB_ROWS of the C header
+
+
+class RbDesc(ctypes.Structure):
+    """``cc_rb_desc`` of the C header."""
+
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("ngroups", ctypes.c_int32),
+        ("gptr", ctypes.c_void_p),
+        ("gcol", ctypes.c_void_p),
+        ("gval", ctypes.c_void_p),
+        ("gval32", ctypes.c_void_p),
+    ]
 
 
 def declared_symbols() -> list[str]:
@@ -170,6 +197,63 @@ class DeviceCSR:
 
     def bytes(self, value_bytes: int = 8) -> int:
         return int(self.nnz * (value_bytes + 4) + 4 * (self.n + 1))
+
+
+@dataclass
+class DeviceRowBlocks:
+    """Row-block form of a device CSR matrix (``cc_rb_desc``): groups of
+    ``RB_ROWS`` consecutive rows, the ascending union of their columns and,
+    per union column, the group's row values (zero where a row has none).
+    Built on the device from ``A`` (column indices must ascend in each row)."""
+
+    n: int
+    ngroups: int
+    nu: int
+    gptr: object
+    gcol: object
+    gval: object | None
+    gval32: object | None
+    desc: RbDesc
+
+    @staticmethod
+    def from_csr(A: DeviceCSR, fp64: bool = True, fp32: bool = True) -> "DeviceRowBlocks":
+        lib = require_gpu()
+        torch = _torch()
+        dev = A.vals.device
+        ng = -(-A.n // RB_ROWS)
+        gptr = torch.empty(ng + 1, dtype=torch.int32, device=dev)
+        nu = ctypes.c_int64()
+        _check(lib.cc_rb_build_count(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(gptr), ctypes.addressof(nu),
+                                     _stream()), "rb_build_count (column indices must ascend within rows)")
+        nu = int(nu.value)
+        gcol = torch.empty(max(nu, 1), dtype=torch.int32, device=dev)
+        gval = torch.empty(max(nu, 1) * RB_ROWS, dtype=torch.float64, device=dev) if fp64 else None
+        gval32 = torch.empty(max(nu, 1) * RB_ROWS, dtype=torch.float32, device=dev) if fp32 else None
+        _check(lib.cc_rb_build_fill(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(gptr), _ptr(gcol),
+                                    _ptr(gval) if fp64 else None, _ptr(gval32) if fp32 else None, _stream()),
+               "rb_build_fill")
+        desc = RbDesc(A.n, ng, _ptr(gptr), _ptr(gcol), _ptr(gval) if fp64 else None,
+                      _ptr(gval32) if fp32 else None)
+        return DeviceRowBlocks(A.n, ng, nu, gptr, gcol, gval, gval32, desc)
+
+    def bytes(self, value_bytes: int = 8) -> int:
+        """Stored bytes one product reads: group offsets, union columns and
+        the padded values of one precision."""
+        return int(4 * (self.ngroups + 1) + self.nu * (4 + RB_ROWS * value_bytes))
+
+
+def spmm_rb(rb: DeviceRowBlocks, X, Y=None):
+    """``Y = A X`` from the row-block form (``X`` float64 or float32, ``[n, k]``)."""
+    lib = require_gpu()
+    torch = _torch()
+    X2 = X.reshape(rb.n, -1).contiguous()
+    k = X2.shape[1]
+    if k not in SUPPORTED_K:
+        raise ValueError(f"k must be one of {SUPPORTED_K}")
+    Y = torch.empty_like(X2) if Y is None else Y
+    fn = lib.cc_spmm_rb_f64 if X2.dtype == torch.float64 else lib.cc_spmm_rb_f32
+    _check(fn(ctypes.addressof(rb.desc), k, _ptr(X2), _ptr(Y), _stream()), "spmm_rb")
+    return Y.reshape(X.shape)
 
 
 def spmv(A: DeviceCSR, x, y=None):
@@ -290,6 +374,7 @@ def pcg(
     workspace: Workspace | None = None,
     history: bool = False,
     tune: bool = False,
+    rb: DeviceRowBlocks | None = None,
 ):
     """Device Jacobi-PCG on ``k`` right-hand sides (``B`` is ``[n]`` or ``[n, k]``).
 
@@ -297,6 +382,7 @@ def pcg(
     (and, with ``history=True`` for ``precision="fp64"``, a ``[maxiter+1, k]``
     array of ``||r_j||/||b||`` per iteration, NaN past each column's stop).
     ``precision="mixed"`` runs float32 inner PCG under float64 refinement.
+    With ``rb`` (the row-block form of ``A``) the CG products use it.
     """
     lib = require_gpu()
     torch = _torch()
@@ -309,9 +395,11 @@ def pcg(
     prec = {"none": 0, "jacobi": 1}[preconditioner]
     mode = {"fp64": 0, "mixed": 1}[precision]
     if tune:
-        from .autotune import spmm_config
+        from .autotune import rb_config, spmm_config
 
         spmm_config(A, k, precision)
+        if rb is not None:
+            rb_config(rb, k, precision)
     nbytes = int(lib.cc_pcg_workspace_bytes(A.n, k, mode))
     ws = (workspace or _default_ws).get(nbytes, A.vals.device)
     res = (PcgResult * k)()
@@ -319,10 +407,23 @@ def pcg(
     if mode == 0:
         if history:
             hist = torch.full((int(maxiter) + 1, k), float("nan"), dtype=torch.float64, device=A.vals.device)
-        st = lib.cc_pcg_jacobi_f64(
-            A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(B2), _ptr(X2),
-            float(tol), int(maxiter), prec, int(check_every), _ptr(ws), nbytes, ctypes.addressof(res),
-            _ptr(hist) if hist is not None else None, _stream(),
+        if rb is not None:
+            st = lib.cc_pcg_jacobi_f64_rb(
+                A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), ctypes.addressof(rb.desc), _ptr(B2),
+                _ptr(X2), float(tol), int(maxiter), prec, int(check_every), _ptr(ws), nbytes,
+                ctypes.addressof(res), _ptr(hist) if hist is not None else None, _stream(),
+            )
+        else:
+            st = lib.cc_pcg_jacobi_f64(
+                A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(B2), _ptr(X2),
+                float(tol), int(maxiter), prec, int(check_every), _ptr(ws), nbytes, ctypes.addressof(res),
+                _ptr(hist) if hist is not None else None, _stream(),
+            )
+    elif rb is not None:
+        st = lib.cc_pcg_jacobi_mixed_rb(
+            A.n, k, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), ctypes.addressof(rb.desc), _ptr(B2), _ptr(X2),
+            float(tol), int(maxiter), float(inner_tol), int(max_outer), prec, int(check_every),
+            _ptr(ws), nbytes, ctypes.addressof(res), _stream(),
         )
     else:
         v32 = A.ensure_f32()

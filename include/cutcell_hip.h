@@ -147,6 +147,58 @@ int cc_pcg_jacobi_mixed(int32_t n, int32_t k, const int32_t* indptr, const int32
                         double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
                         void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream);
 
+/* Row-block form of a CSR matrix (no reference counterpart: a derived layout
+ * for products with several right-hand sides, built once per matrix). Rows
+ * are taken in groups of CC_RB_ROWS consecutive rows; group g stores the
+ * ascending union of its rows' columns, gcol[gptr[g] .. gptr[g+1]), and per
+ * union column u the CC_RB_ROWS row values gval[u * CC_RB_ROWS + r] (zero
+ * where row r has no entry at that column), so one gathered row of X serves
+ * all rows of the group. ngroups = ceil(n / CC_RB_ROWS). */
+#define CC_RB_ROWS 4
+typedef struct cc_rb_desc {
+  int32_t n;
+  int32_t ngroups;
+  const int32_t* gptr;  /* [ngroups + 1] */
+  const int32_t* gcol;  /* [nu] */
+  const double* gval;   /* [nu * CC_RB_ROWS] float64 values, or NULL */
+  const float* gval32;  /* [nu * CC_RB_ROWS] float32 values, or NULL */
+} cc_rb_desc;
+
+/* Build step 1: from a CSR matrix whose column indices ascend within each
+ * row (repeated columns are summed), write the group offsets to gptr
+ * (device, ngroups + 1 entries) and the union size to *nu (host).
+ * Synchronises the stream; returns CC_E_ARG if a row's indices descend. */
+int cc_rb_build_count(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* gptr, int64_t* nu,
+                      cc_stream_t stream);
+/* Build step 2: fill gcol [nu] and gval / gval32 [nu * CC_RB_ROWS] (either
+ * may be NULL, not both) from the same CSR matrix and the gptr of step 1. */
+int cc_rb_build_fill(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                     const int32_t* gptr, int32_t* gcol, double* gval, float* gval32, cc_stream_t stream);
+/* Launch shape of the row-block product for k right-hand sides of
+ * value_bytes-wide vectors (autotuner hook): nz_lanes union-column slots per
+ * group team ((k/cols_per_lane)*nz_lanes <= 64, power of two, 0 = default),
+ * grid blocks (0 = default, at most 1024), unroll 1 or 4 (0 = 4),
+ * cols_per_lane 1, 2 or 4 right-hand sides per lane (at most 16 bytes,
+ * 0 = widest). */
+int cc_rb_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t grid, int32_t unroll,
+                     int32_t cols_per_lane);
+/* Y = A X for k in {1,2,4,8,16} right-hand sides (row-major [n][k]) from the
+ * row-block form (the same product as cc_spmm_csr_f64 / _f32). */
+int cc_spmm_rb_f64(const cc_rb_desc* rb, int32_t k, const double* X, double* Y, cc_stream_t stream);
+int cc_spmm_rb_f32(const cc_rb_desc* rb, int32_t k, const float* X, float* Y, cc_stream_t stream);
+/* cc_pcg_jacobi_f64 / cc_pcg_jacobi_mixed with the CG products taken from
+ * the row-block form rb of the same matrix (rb->gval for float64,
+ * rb->gval32 for the mixed inner solve); the diagonal and the float64 true
+ * residuals still come from the CSR arrays. Same workspace sizes. */
+int cc_pcg_jacobi_f64_rb(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                         const cc_rb_desc* rb, const double* B, double* X, double tol, int32_t maxiter,
+                         int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                         cc_pcg_result* results, double* history, cc_stream_t stream);
+int cc_pcg_jacobi_mixed_rb(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices, const double* vals,
+                           const cc_rb_desc* rb, const double* B, double* X, double tol, int32_t maxiter,
+                           double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
+                           void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream);
+
 /* Batched persistent solve of nsys independent systems of uneven size in one
  * launch (replaces a loop of solve_spd calls over independent cases, the
  * reference's case-level parallelism in code/fast_solver_workbench.py:201-217,
