@@ -58,8 +58,10 @@ static Profiler g_prof;
 
 static void prof_mark(hipStream_t s) {
   if (g_prof.used >= g_prof.pool.size()) {
+    // No system-scope fence at the marks: they only order work on one
+    // stream, and a fence would write back L2 between every two kernels.
     hipEvent_t e;
-    if (hipEventCreate(&e) != hipSuccess) return;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
     g_prof.pool.push_back(e);
   }
   hipEventRecord(g_prof.pool[g_prof.used], s);
