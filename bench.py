@@ -343,6 +343,30 @@ def time_batched(hip, torch, dmf, B):
             "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1))}
 
 
+def time_learned_batched(hip, torch, net, dmf, B):
+    """All columns of the list B in one batched flexible CG with the
+    multilevel learned preconditioner (one K-column operator apply and one
+    batched network pass per iteration)."""
+    Bk = torch.stack(B, dim=1).contiguous()
+    net.fcg_batched(dmf, Bk, tol=TOL, maxiter=20000)  # warm-up
+    torch.cuda.synchronize()
+    hip.profile_enable(True)
+    t0 = time.perf_counter()
+    _, res = net.fcg_batched(dmf, Bk, tol=TOL, maxiter=20000)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    apply_ms, apply_cnt = hip.profile_read("spmm")
+    prec_ms, prec_cnt = hip.profile_read("learned")
+    hip.profile_enable(False)
+    iters = [int(r.iterations) for r in res]
+    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
+    return {"solves_per_s": sig(len(B) / dt), "ms_per_batch": sig(1e3 * dt), "columns": len(B), "converged": ok,
+            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
+            "true_rel_max": sig(max(float(r.true_rel) for r in res)),
+            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1)),
+            "learned_prec_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None}
+
+
 def pressure_section(hip, torch, args):
     """Pressure projection on the bump channel: the singular E^T K E
     (periodic in x and z), a stream of zero-sum right-hand sides (the
@@ -390,8 +414,12 @@ def pressure_section(hip, torch, args):
     torch.cuda.empty_cache()
     out["learned_multilevel"] = time_solver(hip, torch, lambda b: mlnet.fcg(dmf, b, tol=TOL, maxiter=20000), B)
     log(f"pressure, learned multilevel: {out['learned_multilevel']}")
-    out["speedup_solves_per_s"] = sig(out["learned_multilevel"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
-    out["speedup_batched_jacobi"] = sig(out["jacobi_batched"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
+    out["learned_batched"] = time_learned_batched(hip, torch, mlnet, dmf, B)
+    log(f"pressure, learned batched: {out['learned_batched']}")
+    jac = out["jacobi"]["solves_per_s"]
+    out["speedup_solves_per_s"] = sig(out["learned_multilevel"]["solves_per_s"] / jac)
+    out["speedup_batched_jacobi"] = sig(out["jacobi_batched"]["solves_per_s"] / jac)
+    out["speedup_learned_batched_vs_jacobi"] = sig(out["learned_batched"]["solves_per_s"] / jac)
     return out
 
 
@@ -446,9 +474,13 @@ def learned_section(hip, torch, args):
         log(f"learned section, {method}: {out[method]}")
     out["jacobi_batched"] = time_batched(hip, torch, dmf, B)
     log(f"learned section, jacobi batched: {out['jacobi_batched']}")
+    out["learned_batched"] = time_learned_batched(hip, torch, net, dmf, B)
+    log(f"learned section, learned batched: {out['learned_batched']}")
     _, res, how = solve_learned_with_fallback(net, dmf, B[0], tol=TOL)
     out["fallback_check"] = {"path": how, "converged": res.status == hip.CC_CONVERGED}
-    out["speedup_solves_per_s"] = sig(out["learned"]["solves_per_s"] / out["jacobi"]["solves_per_s"])
+    jac = out["jacobi"]["solves_per_s"]
+    out["speedup_solves_per_s"] = sig(out["learned"]["solves_per_s"] / jac)
+    out["speedup_learned_batched_vs_jacobi"] = sig(out["learned_batched"]["solves_per_s"] / jac)
     out["iteration_ratio"] = sig(out["jacobi"]["iterations_mean"] / out["learned"]["iterations_mean"])
     return out
 

@@ -76,82 +76,163 @@ struct GridArgs {
 constexpr int kCx = 16, kCy = 4, kCz = 4;
 constexpr int kHx = kCx + 2, kHy = kCy + 2, kHz = kCz + 2;
 
+// A block owns one (x, y) column of tiles and runs a run of work items
+// (grid b of the batch, z-tile) in turn: items blockIdx.z * ipb ..
+// min(nb * tiles_z, (blockIdx.z + 1) * ipb) - 1, item = b * tiles_z + z-tile.
+// Grid b of in / gate / out / out_f32 starts b * bstride elements in (the
+// mask is shared). The weights are staged once per block, and the halo is
+// double-buffered: the next item's halo is loaded from memory into
+// registers while the current item's MFMAs run, so one barrier per item
+// separates the two.
+constexpr int kHaloChunks = kHz * kHy * kHx * 2;  // 16-byte chunks of a halo
+constexpr int kHaloPerThread = (kHaloChunks + 1023) / 1024;
+
+__device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict__ inb, int x0, int y0, int z0,
+                                  uint4 (&reg)[kHaloPerThread]) {
+#pragma unroll
+  for (int k = 0; k < kHaloPerThread; ++k) {
+    const int i = threadIdx.x + k * 1024;
+    uint4 v = {0u, 0u, 0u, 0u};
+    if (i < kHaloChunks) {
+      const int half = i & 1, hv = i >> 1;
+      const int hx = hv % kHx, hy = (hv / kHx) % kHy, hz = hv / (kHx * kHy);
+      const int gx = nb_index(x0 - 1 + hx, g.nx, g.px);
+      const int gy = nb_index(y0 - 1 + hy, g.ny, g.py);
+      const int gz = nb_index(z0 - 1 + hz, g.nz, g.pz);
+      if (gx >= 0 && gy >= 0 && gz >= 0)
+        v = *reinterpret_cast<const uint4*>(inb + (((size_t)gz * g.ny + gy) * g.nx + gx) * 16 + 8 * half);
+    }
+    reg[k] = v;
+  }
+}
+
+__device__ inline void halo_store(uint16_t* halo, const uint4 (&reg)[kHaloPerThread]) {
+#pragma unroll
+  for (int k = 0; k < kHaloPerThread; ++k) {
+    const int i = threadIdx.x + k * 1024;
+    if (i < kHaloChunks) *reinterpret_cast<uint4*>(&halo[(i >> 1) * 16 + 8 * (i & 1)]) = reg[k];
+  }
+}
+
 template <int ACT>
 __global__ __launch_bounds__(1024) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
                                                 const uint16_t* __restrict__ W, const uint8_t* __restrict__ mask,
                                                 const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
-                                                float* __restrict__ out_f32) {
-  __shared__ __align__(16) uint16_t halo[kHz * kHy * kHx * 16];
+                                                float* __restrict__ out_f32, int tiles_z, size_t bstride, int nb,
+                                                int ipb) {
+  __shared__ __align__(16) uint16_t halo[2][kHz * kHy * kHx * 16];
   __shared__ __align__(16) uint16_t wl[27 * 16 * 16];
   __shared__ float tile[16][16][17];
   const int tid = threadIdx.x;
-  const int x0 = blockIdx.x * kCx, y0 = blockIdx.y * kCy, z0 = blockIdx.z * kCz;
-  // Halo: 16-byte chunks (8 channels) of each voxel.
-  for (int i = tid; i < kHz * kHy * kHx * 2; i += blockDim.x) {
-    const int half = i & 1, hv = i >> 1;
-    const int hx = hv % kHx, hy = (hv / kHx) % kHy, hz = hv / (kHx * kHy);
-    const int gx = nb_index(x0 - 1 + hx, g.nx, g.px);
-    const int gy = nb_index(y0 - 1 + hy, g.ny, g.py);
-    const int gz = nb_index(z0 - 1 + hz, g.nz, g.pz);
-    uint4 v = {0u, 0u, 0u, 0u};
-    if (gx >= 0 && gy >= 0 && gz >= 0)
-      v = *reinterpret_cast<const uint4*>(in + (((size_t)gz * g.ny + gy) * g.nx + gx) * 16 + 8 * half);
-    *reinterpret_cast<uint4*>(&halo[hv * 16 + 8 * half]) = v;
+  const int x0 = blockIdx.x * kCx, y0 = blockIdx.y * kCy;
+  const int it0 = blockIdx.z * ipb;
+  const int it1 = min(nb * tiles_z, it0 + ipb);
+  // Weights W[t][ci][co] -> wl[t][co][ci]: 16-byte loads of 8 output
+  // channels, scattered into LDS.
+  for (int i = tid; i < 27 * 16 * 2; i += blockDim.x) {
+    const int half = i & 1, tc = i >> 1;  // tc = t * 16 + ci
+    const int t = tc / 16, ci = tc % 16;
+    const uint4 v = *reinterpret_cast<const uint4*>(W + (size_t)tc * 16 + 8 * half);
+    const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int co = 8 * half + k;
+      wl[(t * 16 + co) * 16 + ci] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
+    }
   }
-  for (int i = tid; i < 27 * 16 * 16; i += blockDim.x) {
-    const int t = i / 256, ci = (i / 16) % 16, co = i % 16;  // W[t][ci][co]
-    wl[(t * 16 + co) * 16 + ci] = W[i];
-  }
-  __syncthreads();
   const int lane = tid & 63;
   const int wv = tid >> 6;
   const int row = lane % 16;  // voxel within the wave's 16 (A row / D col index)
   const int kg = lane / 16;   // k group: channels 4 kg .. 4 kg + 3
   const int ly = wv % kCy, lz = wv / kCy;
-  v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int t = 0; t < 27; ++t) {
-    const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // 0..2 in halo coordinates
-    const int hv = ((lz + dz) * kHy + (ly + dy)) * kHx + (row + dx);
-    const v4s a = *reinterpret_cast<const v4s*>(&halo[hv * 16 + 4 * kg]);
-    const v4s b = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  uint4 reg[kHaloPerThread];
+  if (it0 < it1) {
+    const int b = it0 / tiles_z, zt = it0 - b * tiles_z;
+    halo_fetch(g, in + (size_t)b * bstride, x0, y0, zt * kCz, reg);
+    halo_store(halo[0], reg);
   }
-  // acc[i] = D[voxel 4 kg + i][co = row]; transpose through LDS so that each
-  // lane stores 4 consecutive channels of one voxel.
-#pragma unroll
-  for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
   __syncthreads();
-  const int vx = x0 + row, vy = y0 + ly, vz = z0 + lz;
-  if (vx < g.nx && vy < g.ny && vz < g.nz) {
-    const size_t v = ((size_t)vz * g.ny + vy) * g.nx + vx;
-    const bool live = mask[v] != 0;
-    uint16_t h[4];
-    float f[4];
+  for (int it = it0; it < it1; ++it) {
+    const int buf = (it - it0) & 1;
+    const int b = it / tiles_z, zt = it - b * tiles_z;
+    const int z0 = zt * kCz;
+    const size_t off = (size_t)b * bstride;
+    // The next item's halo is in flight while this item's MFMAs run.
+    if (it + 1 < it1) {
+      const int bn = (it + 1) / tiles_z, ztn = (it + 1) - bn * tiles_z;
+      halo_fetch(g, in + (size_t)bn * bstride, x0, y0, ztn * kCz, reg);
+    }
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    // Unrolled by planes of 9 taps: fewer operands held at once (two
+    // 1024-thread blocks fit a CU only at <= 64 VGPRs).
+#pragma unroll 9
+    for (int t = 0; t < 27; ++t) {
+      const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // 0..2 in halo coordinates
+      const int hv = ((lz + dz) * kHy + (ly + dy)) * kHx + (row + dx);
+      const v4s a = *reinterpret_cast<const v4s*>(&halo[buf][hv * 16 + 4 * kg]);
+      const v4s bw = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
+    }
+    // acc[i] = D[voxel 4 kg + i][co = row]; transpose through this wave's own
+    // slice of tile (LDS accesses of one wave complete in order, and no other
+    // wave touches the slice) so that each lane stores 4 consecutive
+    // channels of one voxel.
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      float val = tile[wv][row][4 * kg + i];
-      if (ACT == 1) val = val > 0.f ? val : 0.f;
-      if (gate) val = bf2f(gate[v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
-      if (!live) val = 0.f;
-      f[i] = val;
-      h[i] = f2bf(val);
-    }
-    if (out) {
-      uint2 raw;
-      raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-      raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-      *reinterpret_cast<uint2*>(out + v * 16 + 4 * kg) = raw;
-    }
-    if (out_f32) {
+    for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
+    __builtin_amdgcn_wave_barrier();
+    const int vx = x0 + row, vy = y0 + ly, vz = z0 + lz;
+    if (vx < g.nx && vy < g.ny && vz < g.nz) {
+      const size_t v = ((size_t)vz * g.ny + vy) * g.nx + vx;
+      const bool live = mask[v] != 0;
+      uint16_t h[4];
+      float f[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) out_f32[v * 16 + 4 * kg + i] = f[i];
+      for (int i = 0; i < 4; ++i) {
+        float val = tile[wv][row][4 * kg + i];
+        if (ACT == 1) val = val > 0.f ? val : 0.f;
+        if (gate) val = bf2f(gate[off + v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
+        if (!live) val = 0.f;
+        f[i] = val;
+        h[i] = f2bf(val);
+      }
+      if (out) {
+        uint2 raw;
+        raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+        raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+        *reinterpret_cast<uint2*>(out + off + v * 16 + 4 * kg) = raw;
+      }
+      if (out_f32) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) out_f32[off + v * 16 + 4 * kg + i] = f[i];
+      }
     }
+    __builtin_amdgcn_wave_barrier();
+    // The other buffer was last read by the previous item, which every wave
+    // finished before the barrier that ended it.
+    if (it + 1 < it1) halo_store(halo[buf ^ 1], reg);
+    __syncthreads();
   }
 }
 
-static dim3 conv_grid(int nx, int ny, int nz) {
-  return dim3((unsigned)((nx + kCx - 1) / kCx), (unsigned)((ny + kCy - 1) / kCy), (unsigned)((nz + kCz - 1) / kCz));
+static dim3 conv_grid(int nx, int ny, int nz, int nb = 1, int ipb = 1) {
+  const int items = ((nz + kCz - 1) / kCz) * nb;
+  return dim3((unsigned)((nx + kCx - 1) / kCx), (unsigned)((ny + kCy - 1) / kCy), (unsigned)((items + ipb - 1) / ipb));
+}
+static int conv_tiles_z(int nz) { return (nz + kCz - 1) / kCz; }
+
+// Work items (grid of the batch, z-tile) per block: about kConvBlocks blocks
+// in all (two fit a CU), each running its items in turn with the next
+// item's halo loaded during the current one's MFMAs. g_conv_ipb > 0
+// (cc_cnn_set_config) fixes it instead.
+constexpr int kConvBlocks = 1024;
+static int g_conv_ipb = 0;
+static int conv_ipb(int nx, int ny, int nz, int nb) {
+  const long long xy = (long long)((nx + kCx - 1) / kCx) * ((ny + kCy - 1) / kCy);
+  const long long items = (long long)((nz + kCz - 1) / kCz) * nb;
+  long long ipb = g_conv_ipb > 0 ? g_conv_ipb : (xy * items + kConvBlocks - 1) / kConvBlocks;
+  if (ipb < 1) ipb = 1;
+  if (ipb > items) ipb = items;
+  return (int)ipb;
 }
 
 // Weight gradient partials: part[block][t][ci][co] = sum over the block's
@@ -325,16 +406,25 @@ __global__ void k_sym_grads(const float* __restrict__ gl, float* __restrict__ gw
 
 // The preconditioner network: four masked linear layers (weights W0, W1,
 // flip W1, flip W0 in the descriptor), the last one written in fp32.
+// The four layers on nb grids at once (each act / out_f32 buffer holds nb
+// grids of [voxel][16], one after the other).
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
-                       float* out_f32, hipStream_t s) {
+                       float* out_f32, hipStream_t s, int nb) {
+  // The weights are staged with 16-byte loads.
+  for (int l = 0; l < 4; ++l)
+    if (((uintptr_t)net.weights[l] & 15) != 0) return CC_E_ARG;
   GridArgs g{net.nx, net.ny, net.nz, net.periodic[0], net.periodic[1], net.periodic[2]};
-  const dim3 blocks = conv_grid(net.nx, net.ny, net.nz);
+  const int tz = conv_tiles_z(net.nz);
+  const int ipb = conv_ipb(net.nx, net.ny, net.nz, nb);
+  const dim3 blocks = conv_grid(net.nx, net.ny, net.nz, nb, ipb);
+  if (blocks.z > 65535 || blocks.y > 65535) return CC_E_ARG;
+  const size_t bs = (size_t)net.nx * net.ny * net.nz * 16;
   uint16_t* acts[4] = {act0, act1, act2, act3};
   for (int l = 0; l < 3; ++l)
     hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
-                       acts[l + 1], nullptr);
+                       acts[l + 1], nullptr, tz, bs, nb, ipb);
   hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
-                     out_f32);
+                     out_f32, tz, bs, nb, ipb);
   return (int)hipGetLastError();
 }
 
@@ -350,18 +440,28 @@ int cc_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, cc_stream_t st
   return (int)hipGetLastError();
 }
 
+int cc_cnn_set_config(int32_t items_per_block) {
+  if (items_per_block < 0 || items_per_block > 4096) return CC_E_ARG;
+  g_conv_ipb = items_per_block;
+  return 0;
+}
+
 int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
                          const uint16_t* weights, const uint8_t* mask, const uint16_t* gate, int32_t relu,
                          uint16_t* out, float* out_f32, cc_stream_t stream) {
   if (nx <= 0 || ny <= 0 || nz <= 0 || !in || !weights || !mask || (!out && !out_f32)) return CC_E_ARG;
+  // The weights are staged with 16-byte loads.
+  if (((uintptr_t)weights & 15) != 0) return CC_E_ARG;
   GridArgs g{nx, ny, nz, periodic ? periodic[0] : 0, periodic ? periodic[1] : 0, periodic ? periodic[2] : 0};
   if ((nz + kCz - 1) / kCz > 65535 || (ny + kCy - 1) / kCy > 65535) return CC_E_ARG;
   hipStream_t s = (hipStream_t)stream;
-  const dim3 blocks = conv_grid(nx, ny, nz);
+  const int tz = conv_tiles_z(nz);
+  const int ipb = conv_ipb(nx, ny, nz, 1);
+  const dim3 blocks = conv_grid(nx, ny, nz, 1, ipb);
   if (relu)
-    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32);
+    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
   else
-    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32);
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
   return (int)hipGetLastError();
 }
 

@@ -111,6 +111,14 @@ int fcg_operator_ml(int n, const std::function<int(const double*, double*, doubl
                     hipStream_t s, const std::function<int(const double*, double*, double*)>& residual);
 size_t fcg_operator_ml_bytes(size_t n, const cc_mlcnn_desc& ml);
 bool ml_desc_ok(const cc_mlcnn_desc& ml);
+// The same on k in {1, 2, 4, 8, 16} right-hand sides (row-major [n][k]);
+// apply writes per-block, per-column partials [nparts][k] of p.q; residual
+// writes k relative residuals to host memory.
+int fcg_operator_ml_k(int n, int k, const std::function<int(const double*, double*, double*, int*)>& apply,
+                      const cc_mlcnn_desc& ml, const double* diag, const double* bvec, double* x, double tol,
+                      int maxiter, int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
+                      const std::function<int(const double*, double*, double*)>& residual);
+size_t fcg_operator_ml_k_bytes(size_t n, const cc_mlcnn_desc& ml, int k);
 }  // namespace cc
 
 struct cc_rb_desc;

@@ -35,11 +35,15 @@
 namespace cc {
 
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
-                       float* out_f32, hipStream_t s);
+                       float* out_f32, hipStream_t s, int nb = 1);
 
-int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s);
-int mg_prolong(const MgDims& g, const float* coarse, int cs, const float* add, int as, float* fine, hipStream_t s);
-int mg_to_ch0(long long nvox, const float* src, int ss, uint16_t* grid, hipStream_t s);
+// Grid transfers (mg.hip); nb grids of a batch, each bs elements after the last.
+int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s, int nb = 1,
+                size_t fbs = 0, size_t cbs = 0);
+int mg_prolong(const MgDims& g, const float* coarse, int cs, const float* add, int as, float* fine, hipStream_t s,
+               int nb = 1, size_t cbs = 0, size_t abs_ = 0, size_t fbs = 0);
+int mg_to_ch0(long long nvox, const float* src, int ss, uint16_t* grid, hipStream_t s, int nb = 1, size_t sbs = 0,
+              size_t gbs = 0);
 bool mg_level_dims(const cc_cnn_desc& fine, const cc_cnn_desc& coarse, MgDims* g);
 
 struct FcgState {
@@ -511,6 +515,403 @@ size_t fcg_operator_ml_bytes(size_t n, const cc_mlcnn_desc& ml) {
 }
 
 size_t fcg_operator_bytes(size_t n, size_t voxels) { return fcg_bytes(n, voxels); }
+
+// ---------------------------------------------------------------------------
+// Flexible CG on K right-hand sides at once (row-major [n][K] vectors, one
+// FcgState per column), with the multilevel learned preconditioner applied
+// to all K columns in the same launches: the grid transfers and the
+// convolutions run over K grids per launch (batched), so the operator's
+// K-column apply and the network's weights are shared by the batch. Each
+// column follows its own FCG(1) recurrence and stops on its own; stopped
+// columns are left untouched.
+// ---------------------------------------------------------------------------
+
+// Sum v over the threads of a 256-thread block that share column
+// threadIdx.x % K; thread c < K writes the block's total to out[c].
+template <int K>
+__device__ inline void block_cols(double v, double* out, double* red) {
+#pragma unroll
+  for (int o = K; o < 64; o <<= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane < K) red[w * K + lane] = v;
+  __syncthreads();
+  if (threadIdx.x < K)
+    out[threadIdx.x] = (red[threadIdx.x] + red[K + threadIdx.x]) + (red[2 * K + threadIdx.x] + red[3 * K + threadIdx.x]);
+}
+
+// Per-column fixed-order sum of part[nparts][K] by one 256-thread block.
+template <int K>
+__device__ inline void reduce_cols(const double* __restrict__ part, int nparts, double* out, double* smem) {
+  const int c = threadIdx.x % K, r0 = threadIdx.x / K, R = blockDim.x / K;
+  double acc = 0.0;
+  for (int b = r0; b < nparts; b += R) acc += part[(size_t)b * K + c];
+  smem[threadIdx.x] = acc;
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+    for (int i = 0; i < R; ++i) t += smem[i * K + threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+  __syncthreads();
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_setup(int n, const double* __restrict__ b, double* __restrict__ x,
+                                                    double* __restrict__ r, double* __restrict__ part) {
+  __shared__ double red[4 * K];
+  double bb = 0.0;
+  const size_t total = (size_t)n * K;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    x[e] = 0.0;
+    r[e] = b[e];
+    bb += b[e] * b[e];
+  }
+  block_cols<K>(bb, part + (size_t)blockIdx.x * K, red);
+}
+
+// A zero right-hand side is solved by x = 0 (converged at iteration 0).
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_init(const double* __restrict__ part, int nparts, FcgState* st) {
+  __shared__ double smem[256];
+  __shared__ double s[K];
+  reduce_cols<K>(part, nparts, s, smem);
+  if (threadIdx.x < K) {
+    const double bb = s[threadIdx.x];
+    FcgState f;
+    f.bnorm = bb > 0.0 ? sqrt(bb) : 1.0;
+    f.rel = sqrt(bb) / f.bnorm;
+    f.alpha = f.beta = f.pq = f.pr = f.zq = 0.0;
+    f.status = bb > 0.0 ? CC_RUNNING : CC_CONVERGED;
+    f.iterations = 0;
+    st[threadIdx.x] = f;
+  }
+}
+
+// p = z + beta p (beta = 0 at the start) on running columns; partials p.r.
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_dir(int n, double* __restrict__ p, const double* __restrict__ z,
+                                                  const double* __restrict__ r, double* __restrict__ part,
+                                                  const FcgState* __restrict__ st, int first) {
+  __shared__ double red[4 * K];
+  const int c = threadIdx.x % K;
+  const bool run = st[c].status == CC_RUNNING;
+  const double beta = first ? 0.0 : st[c].beta;
+  double pr = 0.0;
+  const size_t total = (size_t)n * K;
+  if (run)
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+      const double pp = z[e] + (first ? 0.0 : beta * p[e]);
+      p[e] = pp;
+      pr += pp * r[e];
+    }
+  block_cols<K>(pr, part + (size_t)blockIdx.x * K, red);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_alpha(const double* __restrict__ part_pq, int n_pq,
+                                                    const double* __restrict__ part_pr, int n_pr, FcgState* st,
+                                                    int iter) {
+  __shared__ double smem[256];
+  __shared__ double a[K], b[K];
+  reduce_cols<K>(part_pq, n_pq, a, smem);
+  reduce_cols<K>(part_pr, n_pr, b, smem);
+  if (threadIdx.x < K) {
+    FcgState& f = st[threadIdx.x];
+    if (f.status == CC_RUNNING) {
+      const double pq = a[threadIdx.x];
+      f.pq = pq;
+      f.pr = b[threadIdx.x];
+      if (!(pq > 0.0 && isfinite(pq))) {
+        f.status = CC_BREAKDOWN;
+        f.iterations = iter;
+        f.alpha = 0.0;
+      } else {
+        f.alpha = b[threadIdx.x] / pq;
+      }
+    }
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_update(int n, double* __restrict__ x, double* __restrict__ r,
+                                                     const double* __restrict__ p, const double* __restrict__ q,
+                                                     double* __restrict__ part, const FcgState* __restrict__ st) {
+  __shared__ double red[4 * K];
+  const int c = threadIdx.x % K;
+  const bool run = st[c].status == CC_RUNNING;
+  const double a = st[c].alpha;
+  double rr = 0.0;
+  const size_t total = (size_t)n * K;
+  if (run)
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+      x[e] += a * p[e];
+      const double rn = r[e] - a * q[e];
+      r[e] = rn;
+      rr += rn * rn;
+    }
+  block_cols<K>(rr, part + (size_t)blockIdx.x * K, red);
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_rel(const double* __restrict__ part, int nparts, FcgState* st, int iter,
+                                                  double tol, int maxiter) {
+  __shared__ double smem[256];
+  __shared__ double s[K];
+  reduce_cols<K>(part, nparts, s, smem);
+  if (threadIdx.x < K) {
+    FcgState& f = st[threadIdx.x];
+    if (f.status == CC_RUNNING) {
+      const double rel = sqrt(s[threadIdx.x]) / f.bnorm;
+      f.rel = rel;
+      f.iterations = iter;
+      if (rel <= tol)
+        f.status = CC_CONVERGED;
+      else if (iter >= maxiter)
+        f.status = CC_MAXITER;
+    }
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_beta(const double* __restrict__ part_zq, int n_zq, FcgState* st) {
+  __shared__ double smem[256];
+  __shared__ double s[K];
+  reduce_cols<K>(part_zq, n_zq, s, smem);
+  if (threadIdx.x < K) {
+    FcgState& f = st[threadIdx.x];
+    if (f.status == CC_RUNNING) {
+      f.zq = s[threadIdx.x];
+      f.beta = -s[threadIdx.x] / f.pq;
+    }
+  }
+}
+
+// S r of every column as single-channel float32 grids u0[c][voxel].
+template <int K>
+__global__ __launch_bounds__(256) void k_preck_in(int n, const int* __restrict__ red2full, const double* __restrict__ r,
+                                                  const double* __restrict__ diag, float* __restrict__ u0,
+                                                  size_t vox) {
+  const size_t total = (size_t)n * K;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t j = e / K;
+    const int c = (int)(e % K);
+    u0[(size_t)c * vox + red2full[j]] = (float)(sym_scale(diag[j]) * r[e]);
+  }
+}
+
+// z = D^-1 r + S y (y: column c's grid at c * ybs, element stride ys);
+// partials z.q per column.
+template <int K>
+__global__ __launch_bounds__(256) void k_preck_out(int n, const int* __restrict__ red2full, const double* __restrict__ r,
+                                                   const double* __restrict__ diag, const float* __restrict__ y,
+                                                   int ys, size_t ybs, double* __restrict__ z,
+                                                   const double* __restrict__ q, double* __restrict__ part) {
+  __shared__ double red[4 * K];
+  const int c = threadIdx.x % K;
+  double zq = 0.0;
+  const size_t total = (size_t)n * K;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t j = e / K;
+    const double d = diag[j];
+    const double zz = jacobi_part(r[e], d) + sym_scale(d) * (double)y[(size_t)c * ybs + (size_t)red2full[j] * ys];
+    z[e] = zz;
+    if (q) zq += zz * q[e];
+  }
+  if (part) block_cols<K>(zq, part + (size_t)blockIdx.x * K, red);
+}
+
+static int gk(size_t n, int K) {
+  size_t g = (n * K + 1023) / 1024;
+  if (g > (size_t)kMaxParts) g = kMaxParts;
+  return (int)(g < 1 ? 1 : g);
+}
+
+struct FcgKBuffers {
+  double *r, *z, *p, *q, *pa, *pb, *pc;
+  FcgState* st;
+};
+
+static size_t fcgk_bytes(size_t n, int K) {
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
+  for (int i = 0; i < 4; ++i) add(n * K * sizeof(double));
+  for (int i = 0; i < 3; ++i) add((size_t)kMaxParts * K * sizeof(double));
+  add(K * sizeof(FcgState));
+  return b + 256;
+}
+
+static FcgKBuffers carve_fcgk(char* base, size_t n, int K) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    char* p = base + off;
+    off += bytes;
+    return p;
+  };
+  FcgKBuffers b;
+  b.r = (double*)take(n * K * 8);
+  b.z = (double*)take(n * K * 8);
+  b.p = (double*)take(n * K * 8);
+  b.q = (double*)take(n * K * 8);
+  b.pa = (double*)take((size_t)kMaxParts * K * 8);
+  b.pb = (double*)take((size_t)kMaxParts * K * 8);
+  b.pc = (double*)take((size_t)kMaxParts * K * 8);
+  b.st = (FcgState*)take(K * sizeof(FcgState));
+  return b;
+}
+
+// Multilevel buffers for K columns: every per-level array of MlBuffers
+// holds K grids one after the other.
+static size_t ml_bytes_k(const cc_mlcnn_desc& ml, int K) {
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
+  for (int l = 0; l < ml.levels; ++l) {
+    const size_t v = ml_vox(ml.net[l]) * K;
+    add(v * sizeof(float));
+    add(v * sizeof(float));
+    if (ml.has_net[l]) {
+      add(v * 16 * sizeof(float));
+      for (int k = 0; k < 4; ++k) add(v * 16 * sizeof(uint16_t));
+    }
+  }
+  return b + 256;
+}
+
+static MlBuffers carve_ml_k(const cc_mlcnn_desc& ml, char* base, int K) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    off = (off + 255) & ~size_t(255);
+    char* p = base + off;
+    off += bytes;
+    return p;
+  };
+  MlBuffers b{};
+  for (int l = 0; l < ml.levels; ++l) {
+    const size_t v = ml_vox(ml.net[l]) * K;
+    b.u[l] = (float*)take(v * sizeof(float));
+    b.t[l] = (float*)take(v * sizeof(float));
+    if (ml.has_net[l]) {
+      b.y[l] = (float*)take(v * 16 * sizeof(float));
+      for (int k = 0; k < 4; ++k) b.act[l][k] = (uint16_t*)take(v * 16 * sizeof(uint16_t));
+    }
+  }
+  return b;
+}
+
+template <int K>
+static int ml_prec_k(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double* r, const double* diag,
+                     const double* q, double* z, double* part, hipStream_t s) {
+  const cc_cnn_desc& f = ml.net[0];
+  const int n = f.n_red;
+  const int L = ml.levels;
+  size_t vox[CC_ML_MAX_LEVELS];
+  for (int l = 0; l < L; ++l) vox[l] = ml_vox(ml.net[l]);
+  hipLaunchKernelGGL((k_preck_in<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, mb.u[0], vox[0]);
+  MgDims g[CC_ML_MAX_LEVELS];
+  for (int l = 1; l < L; ++l) {
+    mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
+    CC_CHECK(mg_restrict(g[l], mb.u[l - 1], 1, mb.u[l], 1, s, K, vox[l - 1], vox[l]));
+  }
+  for (int l = 0; l < L; ++l) {
+    if (!ml.has_net[l]) continue;
+    CC_CHECK(mg_to_ch0((long long)vox[l], mb.u[l], 1, mb.act[l][0], s, K, vox[l], vox[l] * 16));
+    CC_CHECK(
+        cnn_forward_layers(ml.net[l], mb.act[l][0], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s, K));
+  }
+  const float* src = mb.y[L - 1];
+  int ss = 16;
+  for (int l = L - 1; l >= 1; --l) {
+    CC_CHECK(mg_prolong(g[l], src, ss, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 16, mb.t[l - 1], s, K,
+                        vox[l] * ss, vox[l - 1] * 16, vox[l - 1]));
+    src = mb.t[l - 1];
+    ss = 1;
+  }
+  hipLaunchKernelGGL((k_preck_out<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, src, ss,
+                     vox[0] * ss, z, q, part);
+  return (int)hipGetLastError();
+}
+
+template <int K>
+static int fcg_loop_ml_k(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
+                         const cc_mlcnn_desc& ml, const MlBuffers& mb, const FcgKBuffers& b, const double* diag,
+                         const double* bvec, double* x, double tol, int maxiter, int check_every,
+                         cc_pcg_result* results, hipStream_t s,
+                         const std::function<int(const double*, double*, double*)>& residual) {
+  const int gs = gk(n, K);
+  hipLaunchKernelGGL((k_fcgk_setup<K>), dim3(gs), dim3(256), 0, s, n, bvec, x, b.r, b.pa);
+  hipLaunchKernelGGL((k_fcgk_init<K>), dim3(1), dim3(256), 0, s, b.pa, gs, b.st);
+  CC_CHECK(ml_prec_k<K>(ml, mb, b.r, diag, nullptr, b.z, nullptr, s));
+  hipLaunchKernelGGL((k_fcgk_dir<K>), dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 1);
+  CC_CHECK(hipGetLastError());
+  std::vector<FcgState> hs(K);
+  if (check_every < 1) check_every = 1;
+  for (int it = 1; it <= maxiter; ++it) {
+    int np = 0;
+    prof_begin(kProfSpmm, s);
+    CC_CHECK(apply(b.p, b.q, b.pa, &np));
+    prof_end(s);
+    hipLaunchKernelGGL((k_fcgk_alpha<K>), dim3(1), dim3(256), 0, s, b.pa, np, b.pb, gs, b.st, it);
+    hipLaunchKernelGGL((k_fcgk_update<K>), dim3(gs), dim3(256), 0, s, n, x, b.r, b.p, b.q, b.pc, b.st);
+    hipLaunchKernelGGL((k_fcgk_rel<K>), dim3(1), dim3(256), 0, s, b.pc, gs, b.st, it, tol, maxiter);
+    prof_begin(kProfLearned, s);
+    CC_CHECK(ml_prec_k<K>(ml, mb, b.r, diag, b.q, b.z, b.pa, s));
+    prof_end(s);
+    hipLaunchKernelGGL((k_fcgk_beta<K>), dim3(1), dim3(256), 0, s, b.pa, gs, b.st);
+    hipLaunchKernelGGL((k_fcgk_dir<K>), dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 0);
+    CC_CHECK(hipGetLastError());
+    if (it % check_every == 0 || it == maxiter) {
+      CC_CHECK(hipMemcpyAsync(hs.data(), b.st, K * sizeof(FcgState), hipMemcpyDeviceToHost, s));
+      CC_CHECK(hipStreamSynchronize(s));
+      prof_collect();
+      bool any = false;
+      for (int c = 0; c < K; ++c) any |= hs[c].status == CC_RUNNING;
+      if (!any) break;
+    }
+  }
+  CC_CHECK(hipMemcpyAsync(hs.data(), b.st, K * sizeof(FcgState), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  prof_collect();
+  std::vector<double> rel(K, 0.0);
+  CC_CHECK(residual(x, b.q, rel.data()));
+  for (int c = 0; c < K; ++c) {
+    results[c].iterations = hs[c].iterations;
+    results[c].status = hs[c].status == CC_RUNNING ? CC_MAXITER : hs[c].status;
+    results[c].outer = 1;
+    results[c].inner_total = hs[c].iterations;
+    results[c].recursive_rel = hs[c].rel;
+    results[c].true_rel = rel[c];
+  }
+  return 0;
+}
+
+size_t fcg_operator_ml_k_bytes(size_t n, const cc_mlcnn_desc& ml, int K) {
+  return ((fcgk_bytes(n, K) + 255) & ~size_t(255)) + ml_bytes_k(ml, K);
+}
+
+int fcg_operator_ml_k(int n, int K, const std::function<int(const double*, double*, double*, int*)>& apply,
+                      const cc_mlcnn_desc& ml, const double* diag, const double* bvec, double* x, double tol,
+                      int maxiter, int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
+                      const std::function<int(const double*, double*, double*)>& residual) {
+  if (wsb < fcg_operator_ml_k_bytes(n, ml, K)) return CC_E_WORKSPACE;
+  const size_t core = (fcgk_bytes(n, K) + 255) & ~size_t(255);
+  FcgKBuffers b = carve_fcgk((char*)ws, n, K);
+  MlBuffers mb = carve_ml_k(ml, (char*)ws + core, K);
+  // Zero what is written only in part: u[0] off the unknowns' voxels and
+  // channels 1..15 of each network's input, for all K grids.
+  CC_CHECK(hipMemsetAsync(mb.u[0], 0, ml_vox(ml.net[0]) * K * sizeof(float), s));
+  for (int l = 0; l < ml.levels; ++l)
+    if (ml.has_net[l])
+      CC_CHECK(hipMemsetAsync(mb.act[l][0], 0, ml_vox(ml.net[l]) * K * 16 * sizeof(uint16_t), s));
+  switch (K) {
+    case 1: return fcg_loop_ml_k<1>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
+    case 2: return fcg_loop_ml_k<2>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
+    case 4: return fcg_loop_ml_k<4>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
+    case 8: return fcg_loop_ml_k<8>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
+    case 16:
+      return fcg_loop_ml_k<16>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
+    default: return CC_E_ARG;
+  }
+}
 
 // Training input: grid[red2full[j]][0] = bf16(scale * S r_j).
 __global__ __launch_bounds__(256) void k_train_input(int n, const int* __restrict__ red2full,

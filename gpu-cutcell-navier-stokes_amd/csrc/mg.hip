@@ -68,8 +68,12 @@ __device__ inline int pro_src(int f, const MgAxis& ax, int* c, float* w) {
   return 1;
 }
 
+// Batched over blockIdx.y: grid b of a batch starts b * fbs (fine) and
+// b * cbs (coarse) elements in.
 __global__ __launch_bounds__(256) void k_mg_restrict(MgDims g, const float* __restrict__ fine, int fs,
-                                                     float* __restrict__ coarse, int cs) {
+                                                     float* __restrict__ coarse, int cs, size_t fbs, size_t cbs) {
+  fine += blockIdx.y * fbs;
+  coarse += blockIdx.y * cbs;
   const int ncx = g.a[0].nc, ncy = g.a[1].nc, ncz = g.a[2].nc;
   const long long total = (long long)ncx * ncy * ncz;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
@@ -99,7 +103,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgDims g, const float* __re
 }
 
 __global__ __launch_bounds__(256) void k_mg_prolong(MgDims g, const float* __restrict__ coarse, int cs,
-                                                    const float* __restrict__ add, int as, float* __restrict__ fine) {
+                                                    const float* __restrict__ add, int as, float* __restrict__ fine,
+                                                    size_t cbs, size_t abs_, size_t fbs) {
+  coarse += blockIdx.y * cbs;
+  if (add) add += blockIdx.y * abs_;
+  fine += blockIdx.y * fbs;
   const int nfx = g.a[0].nf, nfy = g.a[1].nf, nfz = g.a[2].nf;
   const long long total = (long long)nfx * nfy * nfz;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
@@ -129,7 +137,9 @@ __device__ inline uint16_t mg_f2bf(float f) {
 }
 
 __global__ __launch_bounds__(256) void k_mg_to_ch0(long long nvox, const float* __restrict__ src, int ss,
-                                                   uint16_t* __restrict__ grid) {
+                                                   uint16_t* __restrict__ grid, size_t sbs, size_t gbs) {
+  src += blockIdx.y * sbs;
+  grid += blockIdx.y * gbs;
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
        v += (long long)gridDim.x * blockDim.x)
     grid[(size_t)v * 16] = mg_f2bf(src[(size_t)v * ss]);
@@ -151,20 +161,24 @@ static bool mg_dims(const int32_t* nf, const int32_t* nc, const int32_t* per, Mg
   return true;
 }
 
-int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s) {
+int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s, int nb,
+                size_t fbs, size_t cbs) {
   const long long nc = (long long)g.a[0].nc * g.a[1].nc * g.a[2].nc;
-  hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(nc)), dim3(256), 0, s, g, fine, fs, coarse, cs);
+  hipLaunchKernelGGL(k_mg_restrict, dim3(mg_grid(nc), nb), dim3(256), 0, s, g, fine, fs, coarse, cs, fbs, cbs);
   return (int)hipGetLastError();
 }
 
-int mg_prolong(const MgDims& g, const float* coarse, int cs, const float* add, int as, float* fine, hipStream_t s) {
+int mg_prolong(const MgDims& g, const float* coarse, int cs, const float* add, int as, float* fine, hipStream_t s,
+               int nb, size_t cbs, size_t abs_, size_t fbs) {
   const long long nf = (long long)g.a[0].nf * g.a[1].nf * g.a[2].nf;
-  hipLaunchKernelGGL(k_mg_prolong, dim3(mg_grid(nf)), dim3(256), 0, s, g, coarse, cs, add, as, fine);
+  hipLaunchKernelGGL(k_mg_prolong, dim3(mg_grid(nf), nb), dim3(256), 0, s, g, coarse, cs, add, as, fine, cbs, abs_,
+                     fbs);
   return (int)hipGetLastError();
 }
 
-int mg_to_ch0(long long nvox, const float* src, int ss, uint16_t* grid, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_to_ch0, dim3(mg_grid(nvox)), dim3(256), 0, s, nvox, src, ss, grid);
+int mg_to_ch0(long long nvox, const float* src, int ss, uint16_t* grid, hipStream_t s, int nb, size_t sbs,
+              size_t gbs) {
+  hipLaunchKernelGGL(k_mg_to_ch0, dim3(mg_grid(nvox), nb), dim3(256), 0, s, nvox, src, ss, grid, sbs, gbs);
   return (int)hipGetLastError();
 }
 
@@ -184,7 +198,7 @@ int cc_mg_restrict(const int32_t* nf, const int32_t* nc, const int32_t* periodic
                    int32_t fine_stride, float* coarse, int32_t coarse_stride, cc_stream_t stream) {
   MgDims g;
   if (!mg_dims(nf, nc, periodic, &g) || !fine || !coarse || fine_stride < 1 || coarse_stride < 1) return CC_E_ARG;
-  return mg_restrict(g, fine, fine_stride, coarse, coarse_stride, (hipStream_t)stream);
+  return mg_restrict(g, fine, fine_stride, coarse, coarse_stride, (hipStream_t)stream, 1, 0, 0);
 }
 
 int cc_mg_prolong(const int32_t* nf, const int32_t* nc, const int32_t* periodic, const float* coarse,
@@ -192,12 +206,12 @@ int cc_mg_prolong(const int32_t* nf, const int32_t* nc, const int32_t* periodic,
   MgDims g;
   if (!mg_dims(nf, nc, periodic, &g) || !coarse || !fine || coarse_stride < 1 || (add && add_stride < 1))
     return CC_E_ARG;
-  return mg_prolong(g, coarse, coarse_stride, add, add_stride, fine, (hipStream_t)stream);
+  return mg_prolong(g, coarse, coarse_stride, add, add_stride, fine, (hipStream_t)stream, 1, 0, 0, 0);
 }
 
 int cc_mg_to_ch0(int64_t nvox, const float* src, int32_t src_stride, uint16_t* grid, cc_stream_t stream) {
   if (nvox <= 0 || !src || !grid || src_stride < 1) return CC_E_ARG;
-  return mg_to_ch0(nvox, src, src_stride, grid, (hipStream_t)stream);
+  return mg_to_ch0(nvox, src, src_stride, grid, (hipStream_t)stream, 1, 0, 0);
 }
 
 }  // extern "C"

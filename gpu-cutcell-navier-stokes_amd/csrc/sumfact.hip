@@ -898,6 +898,38 @@ int cc_fcg_sumfact_ml(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, const d
                          history, s, residual);
 }
 
+size_t cc_fcg_sumfact_ml_k_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32_t k) {
+  if (!desc_ok(d) || !ml || !ml_desc_ok(*ml) || !k_ok(k)) return 0;
+  return mfk_bytes(d, k) + 512 + (size_t)kMaxParts * 2 * k * sizeof(double) + 512 +
+         fcg_operator_ml_k_bytes((size_t)d->n_red, *ml, k) + 512;
+}
+
+int cc_fcg_sumfact_ml_k(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32_t k, const double* B, double* X,
+                        double tol, int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
+                        cc_pcg_result* results, cc_stream_t stream) {
+  if (!desc_ok(d) || !ml || !ml_desc_ok(*ml) || !k_ok(k) || !B || !X || !workspace || !results || maxiter < 0 ||
+      ml->net[0].n_red != d->n_red)
+    return CC_E_ARG;
+  if (workspace_bytes < cc_fcg_sumfact_ml_k_workspace_bytes(d, ml, k)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  MfBuffers mb = carve_mfk(d, k, base);
+  const size_t off = (mfk_bytes(d, k) + 511) & ~size_t(255);
+  double* rpart = reinterpret_cast<double*>(base + off);
+  char* fws = base + off + (((size_t)kMaxParts * 2 * k * sizeof(double) + 511) & ~size_t(255));
+  const size_t fb = workspace_bytes - (size_t)(fws - base);
+  const cc_sumfact_desc desc = *d;
+  const int K = k;
+  auto apply = [&](const double* P, double* Q, double* partial, int* gs) {
+    return mfk_apply(desc, K, mb, P, Q, partial, gs, s);
+  };
+  auto residual = [&](const double* Xv, double* scratch, double* rel) {
+    return mfk_true_residual(desc, K, mb, B, Xv, scratch, rpart, rel, s);
+  };
+  return fcg_operator_ml_k(desc.n_red, k, apply, *ml, desc.diag, B, X, tol, maxiter, check_every, fws, fb, results,
+                           s, residual);
+}
+
 size_t cc_fcg_sumfact_workspace_bytes(const cc_sumfact_desc* d, const cc_cnn_desc* net) {
   if (!desc_ok(d) || !net) return 0;
   return mf_bytes(d) + 1024 + (size_t)kMaxParts * 2 * sizeof(double) +

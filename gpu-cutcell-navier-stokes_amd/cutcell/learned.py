@@ -73,7 +73,9 @@ def _bind(lib):
         "cc_mg_train_loss": [I, P, P, P, P, F, P, F, F, P, P, P, P],
         "cc_mlcnn_precondition": [P, P, P, P, P, SZ, P],
         "cc_fcg_sumfact_ml": [P, P, P, P, D, I, I, P, SZ, P, P, P],
+        "cc_fcg_sumfact_ml_k": [P, P, I, P, P, D, I, I, P, SZ, P, P],
         "cc_mfma_probe": [P, P, P, P],
+        "cc_cnn_set_config": [I],
         "cc_cnn_conv3_forward": [I, I, I, P, P, P, P, P, I, P, P, P],
         "cc_cnn_flip_weights": [P, P, P],
         "cc_cnn_sym_weights": [P, P, P],
@@ -95,6 +97,7 @@ def _bind(lib):
         "cc_fcg_sumfact_workspace_bytes": [P, P],
         "cc_mlcnn_precondition_workspace_bytes": [P],
         "cc_fcg_sumfact_ml_workspace_bytes": [P, P],
+        "cc_fcg_sumfact_ml_k_workspace_bytes": [P, P, I],
     }.items():
         fn = getattr(lib, name)
         fn.argtypes = args
@@ -590,6 +593,30 @@ class MultilevelCNN:
         if history:
             return x, res, hist.cpu().numpy()
         return x, res
+
+    def fcg_batched(self, dmf, B, *, tol=1.0e-8, maxiter=5000, check_every=16, X=None):
+        """Flexible CG with this preconditioner on ``k`` right-hand sides at
+        once (``B`` row-major ``[n, k]``, k in 1, 2, 4, 8, 16): one K-column
+        operator apply and one batched network pass per iteration; each column
+        stops on its own. Returns ``(X, results)``."""
+        torch = self.torch
+        d = self.desc()
+        B = B.contiguous()
+        k = B.shape[1]
+        nb = int(self.lib.cc_fcg_sumfact_ml_k_workspace_bytes(ctypes.addressof(dmf.desc), ctypes.addressof(d), k))
+        if nb == 0:
+            raise ValueError("invalid descriptors or k")
+        ws = getattr(self, "_fcgk_ws", None)
+        if ws is None or ws.numel() < nb:
+            ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            self._fcgk_ws = ws
+        X = torch.empty_like(B) if X is None else X
+        res = (hip.PcgResult * k)()
+        hip._check(self.lib.cc_fcg_sumfact_ml_k(ctypes.addressof(dmf.desc), ctypes.addressof(d), k, hip._ptr(B),
+                                                hip._ptr(X), float(tol), int(maxiter), int(check_every),
+                                                hip._ptr(ws), nb, ctypes.addressof(res), hip._stream()),
+                   "fcg_sumfact_ml_k")
+        return X, list(res)
 
 
 def torch_where_pos_rsqrt(d):

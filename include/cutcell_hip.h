@@ -326,6 +326,12 @@ typedef struct cc_cnn_desc {
 /* D = A B for 16x16 bfloat16 A, B (row-major), fp32 D: the MFMA lane-layout
  * probe the convolution kernels rely on. */
 int cc_mfma_probe(const uint16_t* A, const uint16_t* B, float* D, cc_stream_t stream);
+/* Launch shape of the convolutions (autotuner hook, no reference
+ * counterpart): work items (one grid of a batch times one z-tile of
+ * 16 x 4 x 4 voxels) run in turn by one block, 1..4096; 0 = about 1024
+ * blocks in all. */
+int cc_cnn_set_config(int32_t items_per_block);
+
 /* out = mask * act(conv3(in, weights)) [* (gate > 0)]; out (bfloat16) and/or
  * out_f32 (fp32) may be given. relu: 1 = ReLU. gate may be NULL. */
 int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* periodic, const uint16_t* in,
@@ -423,6 +429,17 @@ size_t cc_fcg_sumfact_ml_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcn
 int cc_fcg_sumfact_ml(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, const double* b, double* x, double tol,
                       int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
                       cc_pcg_result* result, double* history, cc_stream_t stream);
+/* The same on k in {1,2,4,8,16} right-hand sides at once (B, X row-major
+ * [n_red][k], results[k]): one K-column operator apply and one batched
+ * network pass per iteration serve all columns; each column follows its own
+ * flexible-CG recurrence and stops on its own (a zero column converges at
+ * iteration 0). No reference counterpart (the reference solves one
+ * right-hand side per call, code/fast_solver_backends.py:194-289).
+ * Synchronises. */
+size_t cc_fcg_sumfact_ml_k_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32_t k);
+int cc_fcg_sumfact_ml_k(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32_t k, const double* B, double* X,
+                        double tol, int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
+                        cc_pcg_result* results, cc_stream_t stream);
 
 #ifdef __cplusplus
 }
