@@ -66,32 +66,34 @@ struct GridArgs {
 // Forward (and data-gradient) layer; ACT: 0 none, 1 relu. gate (optional,
 // bf16 [voxel][16]): multiply the result by (gate > 0), the ReLU derivative
 // in backward mode.
-// A block of 16 waves owns a 16 x 4 x 4 (x, y, z) tile of output voxels,
-// wave w the 16 x-consecutive voxels of row (y0 + w % 4, z0 + w / 4). The
-// block stages the tile's 18 x 6 x 6 input halo (all 16 channels, zero or
-// wrapped outside the grid) and the layer's weights, transposed to
-// [tap][co][ci], in LDS; then every tap of every wave reads its MFMA A
-// operand (4 channels of one voxel) and B operand (4 input channels of one
-// output channel) as single conflict-free 8-byte LDS reads.
-constexpr int kCx = 16, kCy = 4, kCz = 4;
+// A block of 8 waves owns a 16 x 4 x 8 (x, y, z) tile of output voxels; wave
+// w the 4 rows (y0 .. y0 + 3) of 16 x-consecutive voxels in plane z0 + w,
+// one MFMA accumulator per row. The block stages the tile's 18 x 6 x 10
+// input halo (all 16 channels, zero or wrapped outside the grid) and the
+// layer's weights, transposed to [tap][co][ci], in LDS. For each (dz, dx)
+// a wave reads the A operands (4 channels of one voxel per lane, 8-byte LDS
+// reads) of the 6 halo rows its 4 output rows touch once, and each of the 3
+// dy taps' B operands once, for 12 MFMAs: 9 LDS reads per 12 MFMAs instead
+// of 2 per MFMA.
+constexpr int kCx = 16, kCy = 4, kCz = 8;
 constexpr int kHx = kCx + 2, kHy = kCy + 2, kHz = kCz + 2;
+constexpr int kConvThreads = 64 * kCz;  // one wave per z-plane of the tile
 
 // A block owns one (x, y) column of tiles and runs a run of work items
 // (grid b of the batch, z-tile) in turn: items blockIdx.z * ipb ..
 // min(nb * tiles_z, (blockIdx.z + 1) * ipb) - 1, item = b * tiles_z + z-tile.
 // Grid b of in / gate / out / out_f32 starts b * bstride elements in (the
-// mask is shared). The weights are staged once per block, and the halo is
-// double-buffered: the next item's halo is loaded from memory into
-// registers while the current item's MFMAs run, so one barrier per item
-// separates the two.
+// mask is shared). The weights are staged once per block; the next item's
+// halo is loaded from memory into registers while the current item's MFMAs
+// run.
 constexpr int kHaloChunks = kHz * kHy * kHx * 2;  // 16-byte chunks of a halo
-constexpr int kHaloPerThread = (kHaloChunks + 1023) / 1024;
+constexpr int kHaloPerThread = (kHaloChunks + kConvThreads - 1) / kConvThreads;
 
 __device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict__ inb, int x0, int y0, int z0,
                                   uint4 (&reg)[kHaloPerThread]) {
 #pragma unroll
   for (int k = 0; k < kHaloPerThread; ++k) {
-    const int i = threadIdx.x + k * 1024;
+    const int i = threadIdx.x + k * kConvThreads;
     uint4 v = {0u, 0u, 0u, 0u};
     if (i < kHaloChunks) {
       const int half = i & 1, hv = i >> 1;
@@ -109,20 +111,21 @@ __device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict_
 __device__ inline void halo_store(uint16_t* halo, const uint4 (&reg)[kHaloPerThread]) {
 #pragma unroll
   for (int k = 0; k < kHaloPerThread; ++k) {
-    const int i = threadIdx.x + k * 1024;
+    const int i = threadIdx.x + k * kConvThreads;
     if (i < kHaloChunks) *reinterpret_cast<uint4*>(&halo[(i >> 1) * 16 + 8 * (i & 1)]) = reg[k];
   }
 }
 
 template <int ACT>
-__global__ __launch_bounds__(1024) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
-                                                const uint16_t* __restrict__ W, const uint8_t* __restrict__ mask,
-                                                const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
-                                                float* __restrict__ out_f32, int tiles_z, size_t bstride, int nb,
-                                                int ipb) {
-  __shared__ __align__(16) uint16_t halo[2][kHz * kHy * kHx * 16];
+__global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
+                                                        const uint16_t* __restrict__ W,
+                                                        const uint8_t* __restrict__ mask,
+                                                        const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
+                                                        float* __restrict__ out_f32, int tiles_z, size_t bstride,
+                                                        int nb, int ipb) {
+  __shared__ __align__(16) uint16_t halo[kHz * kHy * kHx * 16];
   __shared__ __align__(16) uint16_t wl[27 * 16 * 16];
-  __shared__ float tile[16][16][17];
+  __shared__ float tile[kCz][16][17];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * kCx, y0 = blockIdx.y * kCy;
   const int it0 = blockIdx.z * ipb;
@@ -141,76 +144,88 @@ __global__ __launch_bounds__(1024) void k_conv3(GridArgs g, const uint16_t* __re
     }
   }
   const int lane = tid & 63;
-  const int wv = tid >> 6;
-  const int row = lane % 16;  // voxel within the wave's 16 (A row / D col index)
+  const int wv = tid >> 6;    // z-plane of the tile
+  const int row = lane % 16;  // voxel within a row of 16 (A row / D col index)
   const int kg = lane / 16;   // k group: channels 4 kg .. 4 kg + 3
-  const int ly = wv % kCy, lz = wv / kCy;
   uint4 reg[kHaloPerThread];
   if (it0 < it1) {
-    const int b = it0 / tiles_z, zt = it0 - b * tiles_z;
-    halo_fetch(g, in + (size_t)b * bstride, x0, y0, zt * kCz, reg);
-    halo_store(halo[0], reg);
+    const int b = it0 / tiles_z;
+    halo_fetch(g, in + (size_t)b * bstride, x0, y0, (it0 - b * tiles_z) * kCz, reg);
   }
-  __syncthreads();
   for (int it = it0; it < it1; ++it) {
-    const int buf = (it - it0) & 1;
-    const int b = it / tiles_z, zt = it - b * tiles_z;
-    const int z0 = zt * kCz;
+    const int b = it / tiles_z;
+    const int z0 = (it - b * tiles_z) * kCz;
     const size_t off = (size_t)b * bstride;
+    // Every wave is done with the previous item's halo (and the weights are
+    // staged) before the halo is overwritten; then it is complete.
+    __syncthreads();
+    halo_store(halo, reg);
+    __syncthreads();
     // The next item's halo is in flight while this item's MFMAs run.
     if (it + 1 < it1) {
-      const int bn = (it + 1) / tiles_z, ztn = (it + 1) - bn * tiles_z;
-      halo_fetch(g, in + (size_t)bn * bstride, x0, y0, ztn * kCz, reg);
+      const int bn = (it + 1) / tiles_z;
+      halo_fetch(g, in + (size_t)bn * bstride, x0, y0, (it + 1 - bn * tiles_z) * kCz, reg);
     }
-    v4f acc = {0.f, 0.f, 0.f, 0.f};
-    // Unrolled by planes of 9 taps: fewer operands held at once (two
-    // 1024-thread blocks fit a CU only at <= 64 VGPRs).
-#pragma unroll 9
-    for (int t = 0; t < 27; ++t) {
-      const int dz = t / 9, dy = (t / 3) % 3, dx = t % 3;  // 0..2 in halo coordinates
-      const int hv = ((lz + dz) * kHy + (ly + dy)) * kHx + (row + dx);
-      const v4s a = *reinterpret_cast<const v4s*>(&halo[buf][hv * 16 + 4 * kg]);
-      const v4s bw = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
-    }
-    // acc[i] = D[voxel 4 kg + i][co = row]; transpose through this wave's own
-    // slice of tile (LDS accesses of one wave complete in order, and no other
-    // wave touches the slice) so that each lane stores 4 consecutive
-    // channels of one voxel.
+    v4f acc[kCy];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
-    __builtin_amdgcn_wave_barrier();
-    const int vx = x0 + row, vy = y0 + ly, vz = z0 + lz;
-    if (vx < g.nx && vy < g.ny && vz < g.nz) {
-      const size_t v = ((size_t)vz * g.ny + vy) * g.nx + vx;
-      const bool live = mask[v] != 0;
-      uint16_t h[4];
-      float f[4];
+    for (int r = 0; r < kCy; ++r) acc[r] = v4f{0.f, 0.f, 0.f, 0.f};
+    // dz is not unrolled: one plane's operands in flight at a time keeps the
+    // kernel within the registers of 4 waves per SIMD.
+#pragma unroll 1
+    for (int dz = 0; dz < 3; ++dz) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        float val = tile[wv][row][4 * kg + i];
-        if (ACT == 1) val = val > 0.f ? val : 0.f;
-        if (gate) val = bf2f(gate[off + v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
-        if (!live) val = 0.f;
-        f[i] = val;
-        h[i] = f2bf(val);
-      }
-      if (out) {
-        uint2 raw;
-        raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
-        raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
-        *reinterpret_cast<uint2*>(out + off + v * 16 + 4 * kg) = raw;
-      }
-      if (out_f32) {
+      for (int dx = 0; dx < 3; ++dx) {
+        v4s a[kHy];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) out_f32[off + v * 16 + 4 * kg + i] = f[i];
+        for (int hy = 0; hy < kHy; ++hy) {
+          const int hv = ((wv + dz) * kHy + hy) * kHx + (row + dx);
+          a[hy] = *reinterpret_cast<const v4s*>(&halo[hv * 16 + 4 * kg]);
+        }
+#pragma unroll
+        for (int dy = 0; dy < 3; ++dy) {
+          const int t = dz * 9 + dy * 3 + dx;
+          const v4s bw = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
+#pragma unroll
+          for (int r = 0; r < kCy; ++r) acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r + dy], bw, acc[r], 0, 0, 0);
+        }
       }
     }
-    __builtin_amdgcn_wave_barrier();
-    // The other buffer was last read by the previous item, which every wave
-    // finished before the barrier that ended it.
-    if (it + 1 < it1) halo_store(halo[buf ^ 1], reg);
-    __syncthreads();
+    // acc[r][i] = D[voxel 4 kg + i][co = row] of row r; transpose each row
+    // through this wave's own slice of tile (LDS accesses of one wave
+    // complete in order, and no other wave touches the slice) so that each
+    // lane stores 4 consecutive channels of one voxel.
+    const int vx = x0 + row, vz = z0 + wv;
+#pragma unroll
+    for (int r = 0; r < kCy; ++r) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[r][i];
+      const int vy = y0 + r;
+      if (vx < g.nx && vy < g.ny && vz < g.nz) {
+        const size_t v = ((size_t)vz * g.ny + vy) * g.nx + vx;
+        const bool live = mask[v] != 0;
+        uint16_t h[4];
+        float f[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float val = tile[wv][row][4 * kg + i];
+          if (ACT == 1) val = val > 0.f ? val : 0.f;
+          if (gate) val = bf2f(gate[off + v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
+          if (!live) val = 0.f;
+          f[i] = val;
+          h[i] = f2bf(val);
+        }
+        if (out) {
+          uint2 raw;
+          raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+          raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+          *reinterpret_cast<uint2*>(out + off + v * 16 + 4 * kg) = raw;
+        }
+        if (out_f32) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) out_f32[off + v * 16 + 4 * kg + i] = f[i];
+        }
+      }
+    }
   }
 }
 
@@ -221,8 +236,8 @@ static dim3 conv_grid(int nx, int ny, int nz, int nb = 1, int ipb = 1) {
 static int conv_tiles_z(int nz) { return (nz + kCz - 1) / kCz; }
 
 // Work items (grid of the batch, z-tile) per block: about kConvBlocks blocks
-// in all (two fit a CU), each running its items in turn with the next
-// item's halo loaded during the current one's MFMAs. g_conv_ipb > 0
+// in all (two fit a CU by LDS), each running its items in turn with the
+// next item's halo loaded during the current one's MFMAs. g_conv_ipb > 0
 // (cc_cnn_set_config) fixes it instead.
 constexpr int kConvBlocks = 1024;
 static int g_conv_ipb = 0;
@@ -421,9 +436,9 @@ int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, u
   const size_t bs = (size_t)net.nx * net.ny * net.nz * 16;
   uint16_t* acts[4] = {act0, act1, act2, act3};
   for (int l = 0; l < 3; ++l)
-    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
                        acts[l + 1], nullptr, tz, bs, nb, ipb);
-  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
                      out_f32, tz, bs, nb, ipb);
   return (int)hipGetLastError();
 }
@@ -459,9 +474,9 @@ int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* peri
   const int ipb = conv_ipb(nx, ny, nz, 1);
   const dim3 blocks = conv_grid(nx, ny, nz, 1, ipb);
   if (relu)
-    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
+    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(kConvThreads), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
   else
-    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(1024), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
   return (int)hipGetLastError();
 }
 
