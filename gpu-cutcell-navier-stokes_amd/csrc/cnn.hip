@@ -70,11 +70,12 @@ struct GridArgs {
 // w the 4 rows (y0 .. y0 + 3) of 16 x-consecutive voxels in plane z0 + w,
 // one MFMA accumulator per row. The block stages the tile's 18 x 6 x 10
 // input halo (all 16 channels, zero or wrapped outside the grid) and the
-// layer's weights, transposed to [tap][co][ci], in LDS. For each (dz, dx)
-// a wave reads the A operands (4 channels of one voxel per lane, 8-byte LDS
-// reads) of the 6 halo rows its 4 output rows touch once, and each of the 3
-// dy taps' B operands once, for 12 MFMAs: 9 LDS reads per 12 MFMAs instead
-// of 2 per MFMA.
+// layer's weights in LDS, both split by channel group: halo[kg][voxel][4]
+// and wl[tap][kg][co][4], so that the 16 lanes of one k group read 16
+// consecutive 8-byte operands (bank-conflict free; a [voxel][16] layout
+// puts those lanes 32 bytes apart, a 4-way conflict). For each (dz, dx) a
+// wave reads the A operands of the 6 halo rows its 4 output rows touch once,
+// and each of the 3 dy taps' B operands once, for 12 MFMAs.
 constexpr int kCx = 16, kCy = 4, kCz = 8;
 constexpr int kHx = kCx + 2, kHy = kCy + 2, kHz = kCz + 2;
 constexpr int kConvThreads = 64 * kCz;  // one wave per z-plane of the tile
@@ -88,6 +89,10 @@ constexpr int kConvThreads = 64 * kCz;  // one wave per z-plane of the tile
 // run.
 constexpr int kHaloChunks = kHz * kHy * kHx * 2;  // 16-byte chunks of a halo
 constexpr int kHaloPerThread = (kHaloChunks + kConvThreads - 1) / kConvThreads;
+// Elements of one channel-group plane of the halo: 4 channels of every halo
+// voxel, plus 16 bytes so that the planes a 16-byte chunk is split over
+// start in different banks.
+constexpr int kHaloPlane = kHz * kHy * kHx * 4 + 8;
 
 __device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict__ inb, int x0, int y0, int z0,
                                   uint4 (&reg)[kHaloPerThread]) {
@@ -108,11 +113,17 @@ __device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict_
   }
 }
 
+// Chunk i (channels 8 h .. 8 h + 7 of halo voxel hv, h = i & 1) goes to the
+// channel-group planes 2 h and 2 h + 1.
 __device__ inline void halo_store(uint16_t* halo, const uint4 (&reg)[kHaloPerThread]) {
 #pragma unroll
   for (int k = 0; k < kHaloPerThread; ++k) {
     const int i = threadIdx.x + k * kConvThreads;
-    if (i < kHaloChunks) *reinterpret_cast<uint4*>(&halo[(i >> 1) * 16 + 8 * (i & 1)]) = reg[k];
+    if (i < kHaloChunks) {
+      const int hv = i >> 1, h = i & 1;
+      *reinterpret_cast<uint2*>(&halo[(2 * h) * kHaloPlane + hv * 4]) = make_uint2(reg[k].x, reg[k].y);
+      *reinterpret_cast<uint2*>(&halo[(2 * h + 1) * kHaloPlane + hv * 4]) = make_uint2(reg[k].z, reg[k].w);
+    }
   }
 }
 
@@ -123,15 +134,17 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
                                                         const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
                                                         float* __restrict__ out_f32, int tiles_z, size_t bstride,
                                                         int nb, int ipb) {
-  __shared__ __align__(16) uint16_t halo[kHz * kHy * kHx * 16];
+  __shared__ __align__(16) uint16_t halo[4 * kHaloPlane];
   __shared__ __align__(16) uint16_t wl[27 * 16 * 16];
-  __shared__ float tile[kCz][16][17];
+  // Row stride 20 floats: the transposing writes (4-byte) and reads (16-byte)
+  // of a wave are both bank-conflict free.
+  __shared__ __align__(16) float tile[kCz][16][20];
   const int tid = threadIdx.x;
   const int x0 = blockIdx.x * kCx, y0 = blockIdx.y * kCy;
   const int it0 = blockIdx.z * ipb;
   const int it1 = min(nb * tiles_z, it0 + ipb);
-  // Weights W[t][ci][co] -> wl[t][co][ci]: 16-byte loads of 8 output
-  // channels, scattered into LDS.
+  // Weights W[t][ci][co] -> wl[t][ci / 4][co][ci % 4]: 16-byte loads of 8
+  // output channels, scattered into LDS.
   for (int i = tid; i < 27 * 16 * 2; i += blockDim.x) {
     const int half = i & 1, tc = i >> 1;  // tc = t * 16 + ci
     const int t = tc / 16, ci = tc % 16;
@@ -140,7 +153,7 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int co = 8 * half + k;
-      wl[(t * 16 + co) * 16 + ci] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
+      wl[((t * 4 + ci / 4) * 16 + co) * 4 + ci % 4] = (uint16_t)(w4[k >> 1] >> (16 * (k & 1)));
     }
   }
   const int lane = tid & 63;
@@ -179,12 +192,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
 #pragma unroll
         for (int hy = 0; hy < kHy; ++hy) {
           const int hv = ((wv + dz) * kHy + hy) * kHx + (row + dx);
-          a[hy] = *reinterpret_cast<const v4s*>(&halo[hv * 16 + 4 * kg]);
+          a[hy] = *reinterpret_cast<const v4s*>(&halo[kg * kHaloPlane + hv * 4]);
         }
 #pragma unroll
         for (int dy = 0; dy < 3; ++dy) {
           const int t = dz * 9 + dy * 3 + dx;
-          const v4s bw = *reinterpret_cast<const v4s*>(&wl[(t * 16 + row) * 16 + 4 * kg]);
+          const v4s bw = *reinterpret_cast<const v4s*>(&wl[((t * 4 + kg) * 16 + row) * 4]);
 #pragma unroll
           for (int r = 0; r < kCy; ++r) acc[r] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[r + dy], bw, acc[r], 0, 0, 0);
         }
@@ -205,9 +218,11 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
         const bool live = mask[v] != 0;
         uint16_t h[4];
         float f[4];
+        const float4 tv = *reinterpret_cast<const float4*>(&tile[wv][row][4 * kg]);
+        const float tf[4] = {tv.x, tv.y, tv.z, tv.w};
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          float val = tile[wv][row][4 * kg + i];
+          float val = tf[i];
           if (ACT == 1) val = val > 0.f ? val : 0.f;
           if (gate) val = bf2f(gate[off + v * 16 + 4 * kg + i]) > 0.f ? val : 0.f;
           if (!live) val = 0.f;
