@@ -94,7 +94,8 @@ constexpr int kHaloPerThread = (kHaloChunks + kConvThreads - 1) / kConvThreads;
 // start in different banks.
 constexpr int kHaloPlane = kHz * kHy * kHx * 4 + 8;
 
-__device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict__ inb, int x0, int y0, int z0,
+__device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict__ inb,
+                                  const float* __restrict__ inb_f32, int x0, int y0, int z0,
                                   uint4 (&reg)[kHaloPerThread]) {
 #pragma unroll
   for (int k = 0; k < kHaloPerThread; ++k) {
@@ -106,8 +107,14 @@ __device__ inline void halo_fetch(const GridArgs& g, const uint16_t* __restrict_
       const int gx = nb_index(x0 - 1 + hx, g.nx, g.px);
       const int gy = nb_index(y0 - 1 + hy, g.ny, g.py);
       const int gz = nb_index(z0 - 1 + hz, g.nz, g.pz);
-      if (gx >= 0 && gy >= 0 && gz >= 0)
-        v = *reinterpret_cast<const uint4*>(inb + (((size_t)gz * g.ny + gy) * g.nx + gx) * 16 + 8 * half);
+      if (gx >= 0 && gy >= 0 && gz >= 0) {
+        const size_t gv = ((size_t)gz * g.ny + gy) * g.nx + gx;
+        if (inb_f32) {
+          if (half == 0) v.x = f2bf(inb_f32[gv]);
+        } else {
+          v = *reinterpret_cast<const uint4*>(inb + gv * 16 + 8 * half);
+        }
+      }
     }
     reg[k] = v;
   }
@@ -127,13 +134,18 @@ __device__ inline void halo_store(uint16_t* halo, const uint4 (&reg)[kHaloPerThr
   }
 }
 
+// in_f32 (optional, replaces in): a single-channel float32 input grid (the
+// other 15 input channels are zero), batch stride bstride / 16. out_ch0:
+// out_f32 receives channel 0 only, as a single-channel grid (batch stride
+// bstride / 16).
 template <int ACT>
 __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16_t* __restrict__ in,
+                                                        const float* __restrict__ in_f32,
                                                         const uint16_t* __restrict__ W,
                                                         const uint8_t* __restrict__ mask,
                                                         const uint16_t* __restrict__ gate, uint16_t* __restrict__ out,
-                                                        float* __restrict__ out_f32, int tiles_z, size_t bstride,
-                                                        int nb, int ipb) {
+                                                        float* __restrict__ out_f32, int out_ch0, int tiles_z,
+                                                        size_t bstride, int nb, int ipb) {
   __shared__ __align__(16) uint16_t halo[4 * kHaloPlane];
   __shared__ __align__(16) uint16_t wl[27 * 16 * 16];
   // Row stride 20 floats: the transposing writes (4-byte) and reads (16-byte)
@@ -161,9 +173,11 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
   const int row = lane % 16;  // voxel within a row of 16 (A row / D col index)
   const int kg = lane / 16;   // k group: channels 4 kg .. 4 kg + 3
   uint4 reg[kHaloPerThread];
+  const size_t bstride1 = bstride / 16;  // batch stride of single-channel grids
   if (it0 < it1) {
     const int b = it0 / tiles_z;
-    halo_fetch(g, in + (size_t)b * bstride, x0, y0, (it0 - b * tiles_z) * kCz, reg);
+    halo_fetch(g, in_f32 ? nullptr : in + (size_t)b * bstride, in_f32 ? in_f32 + (size_t)b * bstride1 : nullptr, x0,
+               y0, (it0 - b * tiles_z) * kCz, reg);
   }
   for (int it = it0; it < it1; ++it) {
     const int b = it / tiles_z;
@@ -177,7 +191,8 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
     // The next item's halo is in flight while this item's MFMAs run.
     if (it + 1 < it1) {
       const int bn = (it + 1) / tiles_z;
-      halo_fetch(g, in + (size_t)bn * bstride, x0, y0, (it + 1 - bn * tiles_z) * kCz, reg);
+      halo_fetch(g, in_f32 ? nullptr : in + (size_t)bn * bstride, in_f32 ? in_f32 + (size_t)bn * bstride1 : nullptr,
+                 x0, y0, (it + 1 - bn * tiles_z) * kCz, reg);
     }
     v4f acc[kCy];
 #pragma unroll
@@ -236,8 +251,12 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
           *reinterpret_cast<uint2*>(out + off + v * 16 + 4 * kg) = raw;
         }
         if (out_f32) {
+          if (out_ch0) {
+            if (kg == 0) out_f32[(size_t)b * bstride1 + v] = f[0];
+          } else {
 #pragma unroll
-          for (int i = 0; i < 4; ++i) out_f32[off + v * 16 + 4 * kg + i] = f[i];
+            for (int i = 0; i < 4; ++i) out_f32[off + v * 16 + 4 * kg + i] = f[i];
+          }
         }
       }
     }
@@ -451,10 +470,34 @@ int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, u
   const size_t bs = (size_t)net.nx * net.ny * net.nz * 16;
   uint16_t* acts[4] = {act0, act1, act2, act3};
   for (int l = 0; l < 3; ++l)
-    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, acts[l], net.weights[l], net.mask, nullptr,
-                       acts[l + 1], nullptr, tz, bs, nb, ipb);
-  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act3, net.weights[3], net.mask, nullptr, nullptr,
-                     out_f32, tz, bs, nb, ipb);
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, acts[l], nullptr, net.weights[l], net.mask,
+                       nullptr, acts[l + 1], nullptr, 0, tz, bs, nb, ipb);
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act3, nullptr, net.weights[3], net.mask,
+                     nullptr, nullptr, out_f32, 0, tz, bs, nb, ipb);
+  return (int)hipGetLastError();
+}
+
+// The same four layers from a single-channel float32 input grid u (channel
+// 0 of the first layer's input; its other channels are zero) to channel 0 of
+// the last layer as a single-channel float32 grid y, on nb grids of each.
+int cnn_forward_ch0(const cc_cnn_desc& net, const float* u, uint16_t* act1, uint16_t* act2, uint16_t* act3, float* y,
+                    hipStream_t s, int nb) {
+  for (int l = 0; l < 4; ++l)
+    if (((uintptr_t)net.weights[l] & 15) != 0) return CC_E_ARG;
+  GridArgs g{net.nx, net.ny, net.nz, net.periodic[0], net.periodic[1], net.periodic[2]};
+  const int tz = conv_tiles_z(net.nz);
+  const int ipb = conv_ipb(net.nx, net.ny, net.nz, nb);
+  const dim3 blocks = conv_grid(net.nx, net.ny, net.nz, nb, ipb);
+  if (blocks.z > 65535 || blocks.y > 65535) return CC_E_ARG;
+  const size_t bs = (size_t)net.nx * net.ny * net.nz * 16;
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, nullptr, u, net.weights[0], net.mask, nullptr,
+                     act1, nullptr, 0, tz, bs, nb, ipb);
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act1, nullptr, net.weights[1], net.mask, nullptr,
+                     act2, nullptr, 0, tz, bs, nb, ipb);
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act2, nullptr, net.weights[2], net.mask, nullptr,
+                     act3, nullptr, 0, tz, bs, nb, ipb);
+  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act3, nullptr, net.weights[3], net.mask, nullptr,
+                     nullptr, y, 1, tz, bs, nb, ipb);
   return (int)hipGetLastError();
 }
 
@@ -489,9 +532,11 @@ int cc_cnn_conv3_forward(int32_t nx, int32_t ny, int32_t nz, const int32_t* peri
   const int ipb = conv_ipb(nx, ny, nz, 1);
   const dim3 blocks = conv_grid(nx, ny, nz, 1, ipb);
   if (relu)
-    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(kConvThreads), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
+    hipLaunchKernelGGL(k_conv3<1>, blocks, dim3(kConvThreads), 0, s, g, in, nullptr, weights, mask, gate, out, out_f32,
+                       0, tz, 0, 1, ipb);
   else
-    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, in, weights, mask, gate, out, out_f32, tz, 0, 1, ipb);
+    hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, in, nullptr, weights, mask, gate, out, out_f32,
+                       0, tz, 0, 1, ipb);
   return (int)hipGetLastError();
 }
 

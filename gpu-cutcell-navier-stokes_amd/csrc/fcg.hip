@@ -36,6 +36,10 @@ namespace cc {
 
 int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, uint16_t* act2, uint16_t* act3,
                        float* out_f32, hipStream_t s, int nb = 1);
+// The layers from a single-channel float32 input to channel 0 of the output
+// as a single-channel float32 grid (cnn.hip).
+int cnn_forward_ch0(const cc_cnn_desc& net, const float* u, uint16_t* act1, uint16_t* act2, uint16_t* act3, float* y,
+                    hipStream_t s, int nb = 1);
 
 // Grid transfers (mg.hip); nb grids of a batch, each bs elements after the last.
 int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s, int nb = 1,
@@ -395,8 +399,8 @@ int fcg_operator(int n, const std::function<int(const double*, double*, double*,
 struct MlBuffers {
   float* u[CC_ML_MAX_LEVELS];  // S r restricted to level l (single channel)
   float* t[CC_ML_MAX_LEVELS];  // the correction prolongated down to level l
-  float* y[CC_ML_MAX_LEVELS];  // network output [voxel][16] (levels with a net)
-  uint16_t* act[CC_ML_MAX_LEVELS][4];
+  float* y[CC_ML_MAX_LEVELS];  // channel 0 of the network output (levels with a net)
+  uint16_t* act[CC_ML_MAX_LEVELS][4];  // [1..3]: hidden activations [voxel][16]
 };
 
 static size_t ml_vox(const cc_cnn_desc& d) { return (size_t)d.nx * d.ny * d.nz; }
@@ -427,8 +431,8 @@ size_t ml_bytes(const cc_mlcnn_desc& ml) {
     add(v * sizeof(float));
     add(v * sizeof(float));
     if (ml.has_net[l]) {
-      add(v * 16 * sizeof(float));
-      for (int k = 0; k < 4; ++k) add(v * 16 * sizeof(uint16_t));
+      add(v * sizeof(float));
+      for (int k = 1; k < 4; ++k) add(v * 16 * sizeof(uint16_t));
     }
   }
   return b + 256;
@@ -448,19 +452,16 @@ static MlBuffers carve_ml(const cc_mlcnn_desc& ml, char* base) {
     b.u[l] = (float*)take(v * sizeof(float));
     b.t[l] = (float*)take(v * sizeof(float));
     if (ml.has_net[l]) {
-      b.y[l] = (float*)take(v * 16 * sizeof(float));
-      for (int k = 0; k < 4; ++k) b.act[l][k] = (uint16_t*)take(v * 16 * sizeof(uint16_t));
+      b.y[l] = (float*)take(v * sizeof(float));
+      for (int k = 1; k < 4; ++k) b.act[l][k] = (uint16_t*)take(v * 16 * sizeof(uint16_t));
     }
   }
   return b;
 }
 
-// Zero what is written only in part: u[0] off the unknowns' voxels and
-// channels 1..15 of each network's input.
+// Zero what is written only in part: u[0] off the unknowns' voxels.
 static int ml_setup(const cc_mlcnn_desc& ml, const MlBuffers& b, hipStream_t s) {
   CC_CHECK(hipMemsetAsync(b.u[0], 0, ml_vox(ml.net[0]) * sizeof(float), s));
-  for (int l = 0; l < ml.levels; ++l)
-    if (ml.has_net[l]) CC_CHECK(hipMemsetAsync(b.act[l][0], 0, ml_vox(ml.net[l]) * 16 * sizeof(uint16_t), s));
   return 0;
 }
 
@@ -481,17 +482,14 @@ static int ml_prec(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double* r
   }
   for (int l = 0; l < L; ++l) {
     if (!ml.has_net[l]) continue;
-    CC_CHECK(mg_to_ch0((long long)ml_vox(ml.net[l]), mb.u[l], 1, mb.act[l][0], s));
-    CC_CHECK(cnn_forward_layers(ml.net[l], mb.act[l][0], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s));
+    CC_CHECK(cnn_forward_ch0(ml.net[l], mb.u[l], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s));
   }
   const float* src = mb.y[L - 1];
-  int ss = 16;
   for (int l = L - 1; l >= 1; --l) {
-    CC_CHECK(mg_prolong(g[l], src, ss, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 16, mb.t[l - 1], s));
+    CC_CHECK(mg_prolong(g[l], src, 1, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 1, mb.t[l - 1], s));
     src = mb.t[l - 1];
-    ss = 1;
   }
-  hipLaunchKernelGGL(k_prec_out, dim3(g1(n)), dim3(256), 0, s, n, f.red2full, r, diag, src, ss, z, q, part, st);
+  hipLaunchKernelGGL(k_prec_out, dim3(g1(n)), dim3(256), 0, s, n, f.red2full, r, diag, src, 1, z, q, part, st);
   return (int)hipGetLastError();
 }
 
@@ -770,8 +768,8 @@ static size_t ml_bytes_k(const cc_mlcnn_desc& ml, int K) {
     add(v * sizeof(float));
     add(v * sizeof(float));
     if (ml.has_net[l]) {
-      add(v * 16 * sizeof(float));
-      for (int k = 0; k < 4; ++k) add(v * 16 * sizeof(uint16_t));
+      add(v * sizeof(float));
+      for (int k = 1; k < 4; ++k) add(v * 16 * sizeof(uint16_t));
     }
   }
   return b + 256;
@@ -791,8 +789,8 @@ static MlBuffers carve_ml_k(const cc_mlcnn_desc& ml, char* base, int K) {
     b.u[l] = (float*)take(v * sizeof(float));
     b.t[l] = (float*)take(v * sizeof(float));
     if (ml.has_net[l]) {
-      b.y[l] = (float*)take(v * 16 * sizeof(float));
-      for (int k = 0; k < 4; ++k) b.act[l][k] = (uint16_t*)take(v * 16 * sizeof(uint16_t));
+      b.y[l] = (float*)take(v * sizeof(float));
+      for (int k = 1; k < 4; ++k) b.act[l][k] = (uint16_t*)take(v * 16 * sizeof(uint16_t));
     }
   }
   return b;
@@ -814,20 +812,16 @@ static int ml_prec_k(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double*
   }
   for (int l = 0; l < L; ++l) {
     if (!ml.has_net[l]) continue;
-    CC_CHECK(mg_to_ch0((long long)vox[l], mb.u[l], 1, mb.act[l][0], s, K, vox[l], vox[l] * 16));
-    CC_CHECK(
-        cnn_forward_layers(ml.net[l], mb.act[l][0], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s, K));
+    CC_CHECK(cnn_forward_ch0(ml.net[l], mb.u[l], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s, K));
   }
   const float* src = mb.y[L - 1];
-  int ss = 16;
   for (int l = L - 1; l >= 1; --l) {
-    CC_CHECK(mg_prolong(g[l], src, ss, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 16, mb.t[l - 1], s, K,
-                        vox[l] * ss, vox[l - 1] * 16, vox[l - 1]));
+    CC_CHECK(mg_prolong(g[l], src, 1, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 1, mb.t[l - 1], s, K, vox[l],
+                        vox[l - 1], vox[l - 1]));
     src = mb.t[l - 1];
-    ss = 1;
   }
-  hipLaunchKernelGGL((k_preck_out<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, src, ss,
-                     vox[0] * ss, z, q, part);
+  hipLaunchKernelGGL((k_preck_out<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, src, 1, vox[0], z, q,
+                     part);
   return (int)hipGetLastError();
 }
 
@@ -896,12 +890,9 @@ int fcg_operator_ml_k(int n, int K, const std::function<int(const double*, doubl
   const size_t core = (fcgk_bytes(n, K) + 255) & ~size_t(255);
   FcgKBuffers b = carve_fcgk((char*)ws, n, K);
   MlBuffers mb = carve_ml_k(ml, (char*)ws + core, K);
-  // Zero what is written only in part: u[0] off the unknowns' voxels and
-  // channels 1..15 of each network's input, for all K grids.
+  // Zero what is written only in part: u[0] off the unknowns' voxels, for
+  // all K grids.
   CC_CHECK(hipMemsetAsync(mb.u[0], 0, ml_vox(ml.net[0]) * K * sizeof(float), s));
-  for (int l = 0; l < ml.levels; ++l)
-    if (ml.has_net[l])
-      CC_CHECK(hipMemsetAsync(mb.act[l][0], 0, ml_vox(ml.net[l]) * K * 16 * sizeof(uint16_t), s));
   switch (K) {
     case 1: return fcg_loop_ml_k<1>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
     case 2: return fcg_loop_ml_k<2>(n, apply, ml, mb, b, diag, bvec, x, tol, maxiter, check_every, results, s, residual);
