@@ -537,19 +537,24 @@ __device__ inline void block_cols(double v, double* out, double* red) {
     out[threadIdx.x] = (red[threadIdx.x] + red[K + threadIdx.x]) + (red[2 * K + threadIdx.x] + red[3 * K + threadIdx.x]);
 }
 
-// Per-column fixed-order sum of part[nparts][K] by one 256-thread block.
+// Per-column fixed-order sum of part[nparts][K] by one block of
+// kFinK = 1024 threads: each thread's loads are all issued before they are
+// summed (unrolled), then a fixed-order tree over the threads of a column.
+constexpr int kFinK = 1024;
 template <int K>
 __device__ inline void reduce_cols(const double* __restrict__ part, int nparts, double* out, double* smem) {
-  const int c = threadIdx.x % K, r0 = threadIdx.x / K, R = blockDim.x / K;
+  const int c = threadIdx.x % K, r0 = threadIdx.x / K;
+  constexpr int R = kFinK / K;
   double acc = 0.0;
+#pragma unroll 8
   for (int b = r0; b < nparts; b += R) acc += part[(size_t)b * K + c];
   smem[threadIdx.x] = acc;
   __syncthreads();
-  if (threadIdx.x < K) {
-    double t = 0.0;
-    for (int i = 0; i < R; ++i) t += smem[i * K + threadIdx.x];
-    out[threadIdx.x] = t;
+  for (int h = R / 2; h > 0; h >>= 1) {
+    if (r0 < h) smem[threadIdx.x] += smem[threadIdx.x + h * K];
+    __syncthreads();
   }
+  if (threadIdx.x < K) out[threadIdx.x] = smem[threadIdx.x];
   __syncthreads();
 }
 
@@ -569,8 +574,8 @@ __global__ __launch_bounds__(256) void k_fcgk_setup(int n, const double* __restr
 
 // A zero right-hand side is solved by x = 0 (converged at iteration 0).
 template <int K>
-__global__ __launch_bounds__(256) void k_fcgk_init(const double* __restrict__ part, int nparts, FcgState* st) {
-  __shared__ double smem[256];
+__global__ __launch_bounds__(kFinK) void k_fcgk_init(const double* __restrict__ part, int nparts, FcgState* st) {
+  __shared__ double smem[kFinK];
   __shared__ double s[K];
   reduce_cols<K>(part, nparts, s, smem);
   if (threadIdx.x < K) {
@@ -606,10 +611,10 @@ __global__ __launch_bounds__(256) void k_fcgk_dir(int n, double* __restrict__ p,
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_fcgk_alpha(const double* __restrict__ part_pq, int n_pq,
+__global__ __launch_bounds__(kFinK) void k_fcgk_alpha(const double* __restrict__ part_pq, int n_pq,
                                                     const double* __restrict__ part_pr, int n_pr, FcgState* st,
                                                     int iter) {
-  __shared__ double smem[256];
+  __shared__ double smem[kFinK];
   __shared__ double a[K], b[K];
   reduce_cols<K>(part_pq, n_pq, a, smem);
   reduce_cols<K>(part_pr, n_pr, b, smem);
@@ -651,9 +656,9 @@ __global__ __launch_bounds__(256) void k_fcgk_update(int n, double* __restrict__
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_fcgk_rel(const double* __restrict__ part, int nparts, FcgState* st, int iter,
+__global__ __launch_bounds__(kFinK) void k_fcgk_rel(const double* __restrict__ part, int nparts, FcgState* st, int iter,
                                                   double tol, int maxiter) {
-  __shared__ double smem[256];
+  __shared__ double smem[kFinK];
   __shared__ double s[K];
   reduce_cols<K>(part, nparts, s, smem);
   if (threadIdx.x < K) {
@@ -671,8 +676,8 @@ __global__ __launch_bounds__(256) void k_fcgk_rel(const double* __restrict__ par
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_fcgk_beta(const double* __restrict__ part_zq, int n_zq, FcgState* st) {
-  __shared__ double smem[256];
+__global__ __launch_bounds__(kFinK) void k_fcgk_beta(const double* __restrict__ part_zq, int n_zq, FcgState* st) {
+  __shared__ double smem[kFinK];
   __shared__ double s[K];
   reduce_cols<K>(part_zq, n_zq, s, smem);
   if (threadIdx.x < K) {
@@ -833,7 +838,7 @@ static int fcg_loop_ml_k(int n, const std::function<int(const double*, double*, 
                          const std::function<int(const double*, double*, double*)>& residual) {
   const int gs = gk(n, K);
   hipLaunchKernelGGL((k_fcgk_setup<K>), dim3(gs), dim3(256), 0, s, n, bvec, x, b.r, b.pa);
-  hipLaunchKernelGGL((k_fcgk_init<K>), dim3(1), dim3(256), 0, s, b.pa, gs, b.st);
+  hipLaunchKernelGGL((k_fcgk_init<K>), dim3(1), dim3(kFinK), 0, s, b.pa, gs, b.st);
   CC_CHECK(ml_prec_k<K>(ml, mb, b.r, diag, nullptr, b.z, nullptr, s));
   hipLaunchKernelGGL((k_fcgk_dir<K>), dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 1);
   CC_CHECK(hipGetLastError());
@@ -844,13 +849,13 @@ static int fcg_loop_ml_k(int n, const std::function<int(const double*, double*, 
     prof_begin(kProfSpmm, s);
     CC_CHECK(apply(b.p, b.q, b.pa, &np));
     prof_end(s);
-    hipLaunchKernelGGL((k_fcgk_alpha<K>), dim3(1), dim3(256), 0, s, b.pa, np, b.pb, gs, b.st, it);
+    hipLaunchKernelGGL((k_fcgk_alpha<K>), dim3(1), dim3(kFinK), 0, s, b.pa, np, b.pb, gs, b.st, it);
     hipLaunchKernelGGL((k_fcgk_update<K>), dim3(gs), dim3(256), 0, s, n, x, b.r, b.p, b.q, b.pc, b.st);
-    hipLaunchKernelGGL((k_fcgk_rel<K>), dim3(1), dim3(256), 0, s, b.pc, gs, b.st, it, tol, maxiter);
+    hipLaunchKernelGGL((k_fcgk_rel<K>), dim3(1), dim3(kFinK), 0, s, b.pc, gs, b.st, it, tol, maxiter);
     prof_begin(kProfLearned, s);
     CC_CHECK(ml_prec_k<K>(ml, mb, b.r, diag, b.q, b.z, b.pa, s));
     prof_end(s);
-    hipLaunchKernelGGL((k_fcgk_beta<K>), dim3(1), dim3(256), 0, s, b.pa, gs, b.st);
+    hipLaunchKernelGGL((k_fcgk_beta<K>), dim3(1), dim3(kFinK), 0, s, b.pa, gs, b.st);
     hipLaunchKernelGGL((k_fcgk_dir<K>), dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 0);
     CC_CHECK(hipGetLastError());
     if (it % check_every == 0 || it == maxiter) {
