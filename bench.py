@@ -405,10 +405,11 @@ def pressure_section(hip, torch, args):
     diag = torch.from_numpy(mf.diag).to("cuda")
     periodic = tuple(a.periodic for a in mf.grid.axes)
     mlnet = MultilevelCNN(mf.grid.nb, mf.interior, periodic=periodic, levels=args.ml_levels, seed=args.seed)
-    rep = mlnet.train(R, E, diag, steps=args.learned_steps, batch=8, lr=args.learned_lr)
+    rep = mlnet.train(R, E, diag, steps=args.c5_steps, batch=8, lr=args.c5_lr)
     out["training_multilevel"] = {"pairs": int(R.shape[0]), "pairs_s": sig(pairs_s),
                                   "grids": [list(d) for d in mlnet.dims], "net_levels": mlnet.net_levels,
-                                  "steps": rep.steps, "seconds": sig(rep.seconds), "loss_first": sig(rep.loss_first),
+                                  "steps": rep.steps, "lr": args.c5_lr, "seconds": sig(rep.seconds),
+                                  "loss_first": sig(rep.loss_first),
                                   "loss_last20": sig(float(np.mean(rep.history[-20:])))}
     del R, E
     torch.cuda.empty_cache()
@@ -500,14 +501,16 @@ def main(argv=None) -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner section")
     ap.add_argument("--learned-n", type=int, default=64, help="background elements per axis of the sphere box")
     ap.add_argument("--learned-rhs", type=int, default=200, help="training right-hand sides (10 pairs each)")
-    ap.add_argument("--learned-steps", type=int, default=1000, help="Adam steps (8 pairs each)")
-    ap.add_argument("--learned-lr", type=float, default=2e-3, help="Adam learning rate")
+    ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps (8 pairs each), sphere box")
+    ap.add_argument("--learned-lr", type=float, default=5e-3, help="Adam learning rate, sphere box")
     ap.add_argument("--learned-eval", type=int, default=16, help="right-hand sides timed per method (1-16)")
     ap.add_argument("--ml-levels", type=int, default=5, help="grids of the multilevel learned preconditioner")
     ap.add_argument("--no-pressure", action="store_true", help="skip the bump-channel pressure-projection section")
     ap.add_argument("--c5-eval", type=int, default=16,
                     help="right-hand sides of the pressure stream timed per method (1, 2, 4, 8 or 16)")
     ap.add_argument("--c5-pair-rhs", type=int, default=30, help="pressure right-hand sides that give training pairs")
+    ap.add_argument("--c5-steps", type=int, default=1000, help="Adam steps (8 pairs each), bump channel")
+    ap.add_argument("--c5-lr", type=float, default=2e-3, help="Adam learning rate, bump channel")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="before timing, copy a buffer of this many MiB once (a known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
