@@ -282,6 +282,10 @@ def build_sphere_operator(seed: int, n_elements: int, degree: int = 3):
 
 
 SNAPSHOTS = (2, 5, 10, 20, 40, 70, 100, 150, 200, 300)
+# Iteration counts at which the learned FCG's own iterates are taken for
+# fine-tuning pairs.
+REFINE_ITERS = (5, 20, 50, 100, 150, 200)
+C5_REFINE_ITERS = (5, 15, 30, 50, 70, 90)  # the bump channel converges in ~115
 
 
 def build_channel_operator():
@@ -373,7 +377,7 @@ def pressure_section(hip, torch, args):
     compatibility condition a discrete divergence satisfies), matrix-free
     Jacobi-PCG against flexible CG with the multilevel learned preconditioner
     trained on the solver's own iterates of this operator."""
-    from cutcell.learned import MultilevelCNN, generate_pairs
+    from cutcell.learned import MultilevelCNN, generate_pairs, generate_pairs_fcg
     from cutcell.matfree import DeviceMatrixFree
 
     t0 = time.perf_counter()
@@ -406,6 +410,12 @@ def pressure_section(hip, torch, args):
     periodic = tuple(a.periodic for a in mf.grid.axes)
     mlnet = MultilevelCNN(mf.grid.nb, mf.interior, periodic=periodic, levels=args.ml_levels, seed=args.seed)
     rep = mlnet.train(R, E, diag, steps=args.c5_steps, batch=8, lr=args.c5_lr)
+    if args.c5_refine_rhs > 0:
+        R2, E2 = generate_pairs_fcg(mlnet, dmf, [rhs() for _ in range(args.c5_refine_rhs)], C5_REFINE_ITERS)
+        mlnet.train(torch.cat([R, R2]), torch.cat([E, E2]), diag, steps=args.learned_refine_steps, batch=8,
+                    lr=args.learned_refine_lr)
+        log(f"pressure: fine-tuned on {R2.shape[0]} pairs from the learned FCG's own iterates")
+        del R2, E2
     out["training_multilevel"] = {"pairs": int(R.shape[0]), "pairs_s": sig(pairs_s),
                                   "grids": [list(d) for d in mlnet.dims], "net_levels": mlnet.net_levels,
                                   "steps": rep.steps, "lr": args.c5_lr, "seconds": sig(rep.seconds),
@@ -429,7 +439,7 @@ def learned_section(hip, torch, args):
     solver's own iterates, GPU training, then flexible CG with the network
     against Jacobi-PCG on the same matrix-free operator, one right-hand side
     per solve, to relative residual 1e-8 (float64 true residual judged)."""
-    from cutcell.learned import MultilevelCNN, generate_pairs, solve_learned_with_fallback
+    from cutcell.learned import MultilevelCNN, generate_pairs, generate_pairs_fcg, solve_learned_with_fallback
     from cutcell.matfree import DeviceMatrixFree
 
     t0 = time.perf_counter()
@@ -454,6 +464,21 @@ def learned_section(hip, torch, args):
     log(f"learned: network on {net.levels} grids {net.dims} (nets on {net.net_levels}) trained {rep.steps} Adam "
         f"steps x {rep.batch} pairs in {rep.seconds:.1f} s, loss {rep.loss_first:.3f} -> "
         f"{np.mean(rep.history[-20:]):.3f}")
+    refine = None
+    if args.learned_refine_rhs > 0:
+        t0 = time.perf_counter()
+        rrhs = [torch.from_numpy(rng.standard_normal(n)).to("cuda") for _ in range(args.learned_refine_rhs)]
+        R2, E2 = generate_pairs_fcg(net, dmf, rrhs, REFINE_ITERS)
+        torch.cuda.synchronize()
+        pairs2_s = time.perf_counter() - t0
+        rep2 = net.train(torch.cat([R, R2]), torch.cat([E, E2]), diag, steps=args.learned_refine_steps, batch=8,
+                         lr=args.learned_refine_lr)
+        refine = {"fcg_pairs": int(R2.shape[0]), "rhs": args.learned_refine_rhs, "iters": list(REFINE_ITERS),
+                  "pairs_s": sig(pairs2_s), "steps": rep2.steps, "lr": args.learned_refine_lr,
+                  "seconds": sig(rep2.seconds)}
+        log(f"learned: fine-tuned on {R2.shape[0]} pairs from the learned FCG's own iterates "
+            f"({pairs2_s:.1f} s) plus the Jacobi pairs, {rep2.steps} steps in {rep2.seconds:.1f} s")
+        del R2, E2
     del R, E
     torch.cuda.empty_cache()
     B = [torch.from_numpy(rng.standard_normal(n)).to("cuda") for _ in range(args.learned_eval)]
@@ -465,7 +490,8 @@ def learned_section(hip, torch, args):
                                               "seconds": sig(pairs_s), "snapshots": list(SNAPSHOTS)},
            "training": {"grids": [list(d) for d in net.dims], "net_levels": net.net_levels, "steps": rep.steps,
                         "batch": rep.batch, "lr": args.learned_lr, "seconds": sig(rep.seconds),
-                        "loss_first": sig(rep.loss_first), "loss_last20": sig(float(np.mean(rep.history[-20:])))}}
+                        "loss_first": sig(rep.loss_first), "loss_last20": sig(float(np.mean(rep.history[-20:]))),
+                        "fine_tuning": refine}}
     solvers = {
         "jacobi": lambda b: dmf.pcg(b, tol=TOL, maxiter=20000),
         "learned": lambda b: net.fcg(dmf, b, tol=TOL, maxiter=20000),
@@ -503,6 +529,10 @@ def main(argv=None) -> int:
     ap.add_argument("--learned-rhs", type=int, default=200, help="training right-hand sides (10 pairs each)")
     ap.add_argument("--learned-steps", type=int, default=3000, help="Adam steps (8 pairs each), sphere box")
     ap.add_argument("--learned-lr", type=float, default=5e-3, help="Adam learning rate, sphere box")
+    ap.add_argument("--learned-refine-rhs", type=int, default=60,
+                    help="sphere box: right-hand sides whose learned-FCG iterates give fine-tuning pairs (0: none)")
+    ap.add_argument("--learned-refine-steps", type=int, default=1500, help="fine-tuning Adam steps, sphere box")
+    ap.add_argument("--learned-refine-lr", type=float, default=1e-3, help="fine-tuning learning rate, sphere box")
     ap.add_argument("--learned-eval", type=int, default=16, help="right-hand sides timed per method (1-16)")
     ap.add_argument("--ml-levels", type=int, default=5, help="grids of the multilevel learned preconditioner")
     ap.add_argument("--no-pressure", action="store_true", help="skip the bump-channel pressure-projection section")
@@ -511,6 +541,8 @@ def main(argv=None) -> int:
     ap.add_argument("--c5-pair-rhs", type=int, default=30, help="pressure right-hand sides that give training pairs")
     ap.add_argument("--c5-steps", type=int, default=1000, help="Adam steps (8 pairs each), bump channel")
     ap.add_argument("--c5-lr", type=float, default=2e-3, help="Adam learning rate, bump channel")
+    ap.add_argument("--c5-refine-rhs", type=int, default=0,
+                    help="bump channel: right-hand sides whose learned-FCG iterates give fine-tuning pairs (0: none)")
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="before timing, copy a buffer of this many MiB once (a known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")

@@ -650,6 +650,25 @@ def generate_pairs(dmf, rhs, snapshots, tol_ref: float = 1.0e-11):
     return torch.stack(Rs), torch.stack(Es)
 
 
+def generate_pairs_fcg(net, dmf, rhs, iters, tol_ref: float = 1.0e-11):
+    """(residual, error) pairs from the learned solver's own iterates: for
+    every right-hand side b, x* from one tight Jacobi-PCG solve and x_k from
+    flexible CG with ``net`` stopped after k iterations, for each k in
+    ``iters``; r_k = b - A x_k, e_k = x* - x_k. Fine-tuning on these pairs
+    (with the Jacobi-PCG pairs) targets the modes the trained preconditioner
+    still leaves slow. Returns float32 tensors ``R, E`` of shape [S, n]."""
+    import torch
+
+    Rs, Es = [], []
+    for b in rhs:
+        xs, _ = dmf.pcg(b, tol=tol_ref, maxiter=40000)
+        for k in iters:
+            xk, _ = net.fcg(dmf, b, tol=0.0, maxiter=int(k))
+            Rs.append((b - dmf.apply(xk)).float())
+            Es.append((xs - xk).float())
+    return torch.stack(Rs), torch.stack(Es)
+
+
 def solve_learned_with_fallback(net, dmf, b, *, tol=1.0e-8, maxiter=5000):
     """Flexible CG with the network (``MaskedCNN`` or ``MultilevelCNN``);
     Jacobi-PCG when it does not converge."""

@@ -379,3 +379,30 @@ def test_ml_fcg_batched_matches_single_columns(lib):
         assert res[c].true_rel <= 1e-7
         x1 = x1.cpu().numpy()
         assert np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1) < 1e-6
+
+
+def test_fcg_pairs_are_consistent(lib):
+    """Pairs from the learned solver's own iterates: e_k = x* - x_k and
+    r_k = b - A x_k, so A e_k = r_k up to x*'s tolerance and float32
+    storage; the iterate after k iterations is the one flexible CG returns
+    with maxiter = k."""
+    from cutcell.cut3d import SphereObstacle
+    from cutcell.grid3d import Axis, Grid3D
+    from cutcell.learned import MultilevelCNN, generate_pairs_fcg
+    from cutcell.matfree import DeviceMatrixFree, build_matrix_free
+
+    grid = Grid3D(tuple(Axis(10, 3) for _ in range(3)))
+    mf = build_matrix_free(grid, SphereObstacle.random(2))
+    dmf = DeviceMatrixFree(mf)
+    net = MultilevelCNN(grid.nb, mf.interior, levels=3, seed=2)
+    b = _dev(np.random.default_rng(4).standard_normal(mf.n))
+    R, E = generate_pairs_fcg(net, dmf, [b], (3, 12))
+    assert R.shape == E.shape == (2, mf.n)
+    for i, k in enumerate((3, 12)):
+        r = R[i].double()
+        Ae = dmf.apply(E[i].double())
+        assert (torch.linalg.norm(Ae - r) / torch.linalg.norm(r)).item() < 1e-3
+        xk, res = net.fcg(dmf, b, tol=0.0, maxiter=k)
+        assert res.iterations == k
+        np.testing.assert_allclose(r.cpu().numpy(), (b - dmf.apply(xk)).cpu().numpy(), rtol=0,
+                                   atol=1e-6 * float(torch.linalg.norm(r)))
