@@ -333,8 +333,10 @@ __global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, con
   if (lane < K) u[(size_t)c * n_full + d.small_dof[sm]] = s;
 }
 
-// One wave per shell cell and all K columns: W[s][c][l] = (C_s - A_s) u_s
-// (cut) or -A_s u_s (empty), columns handled CK at a time in registers.
+// One wave per shell cell and all K columns: W[s][l][c] = (C_s - A_s) u_s
+// (cut) or -A_s u_s (empty), columns handled CK at a time in registers. The
+// column index is fastest, so the touch pass reads the K columns of one
+// entry as one contiguous segment.
 template <int P1, int K>
 __global__ __launch_bounds__(256) void k_mfk_shell(cc_sumfact_desc d, const double* __restrict__ u,
                                                    double* __restrict__ Wout) {
@@ -415,7 +417,7 @@ __global__ __launch_bounds__(256) void k_mfk_shell(cc_sumfact_desc d, const doub
         }
       }
 #pragma unroll
-      for (int c = 0; c < CK; ++c) Wout[((size_t)s * K + c0 + c) * PP + l] = w[c];
+      for (int c = 0; c < CK; ++c) Wout[((size_t)s * PP + l) * K + c0 + c] = w[c];
     }
   }
 }
@@ -434,9 +436,9 @@ __global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const doub
   for (int t = wave; t < d.n_touch; t += nwaves) {
     double s = 0.0;
     for (int i = d.touch_ptr[t] + el; i < d.touch_ptr[t + 1]; i += EL) {
-      const int idx = d.touch_idx[i];
-      const int sc = idx / PP, l = idx - sc * PP;
-      s += Win[((size_t)sc * K + c) * PP + l];
+      // Win[cell][local dof][K]: the K columns of one entry are one
+      // contiguous 8 K-byte segment.
+      s += Win[(size_t)d.touch_idx[i] * K + c];
     }
     for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
     if (lane < K) v[(size_t)c * n_full + d.touch_dof[t]] += s;
