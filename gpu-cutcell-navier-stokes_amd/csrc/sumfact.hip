@@ -27,8 +27,6 @@
 
 namespace cc {
 
-constexpr int kTile = 8;
-
 __device__ inline int wrap_or_invalid(int i, int n, int periodic) {
   if (periodic) {
     i %= n;
@@ -62,92 +60,149 @@ __global__ __launch_bounds__(256) void k_mf_gather_small(cc_sumfact_desc d, cons
   if (lane == 0) u[d.small_dof[sm]] = s;
 }
 
-// blockIdx.z = column * nzt + z tile: column c reads u + c n_full and writes
-// v + c n_full (one column: nzt = gridDim.z).
+// Tensor-product part on the background grid, v = (Kx My Mz + Mx Ky Mz +
+// Mx My Kz + c Mx My Mz) u with the banded 1D forms (x pass a = Mx u,
+// b = (Kx + c Mx) u; y pass c = My b + Ky a, d = My a; z pass v = Mz c +
+// Kz d), streamed along z: a block owns a 16 x 16 (x, y) column of the grid
+// and a run of zc output planes. For each input
+// plane it stages the plane's 22 x 22 halo in LDS (loaded into registers
+// one plane ahead), runs the x and y passes for that plane, and keeps the
+// y-pass results of the last 2p + 1 planes in an LDS ring, from which the z
+// pass emits output plane z - p. Every input value is staged once per
+// (x, y) tile and chunk instead of once per 8^3 tile with a (8 + 2p)^3 halo.
+// blockIdx.z = column * nzc + chunk (column c of u_all / v_all starts
+// c * n_full in).
+constexpr int kSx = 16, kSy = 16;
+constexpr int kMaxZc = 32;  // output planes per block at most
+
 template <int P1>
-__global__ __launch_bounds__(256) void k_mf_tensor(cc_sumfact_desc d, const double* __restrict__ u_all,
-                                                   double* __restrict__ v_all, int nzt) {
+__global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const double* __restrict__ u_all,
+                                                     double* __restrict__ v_all, int nzc, int zc) {
   constexpr int p = P1 - 1;
-  const size_t col_off = (size_t)(blockIdx.z / nzt) * ((size_t)d.nb[0] * d.nb[1] * d.nb[2]);
+  constexpr int W = 2 * p + 1;
+  constexpr int HX = kSx + 2 * p, HY = kSy + 2 * p;
+  constexpr int HALO = HX * HY;
+  constexpr int PER = (HALO + 255) / 256;
+  __shared__ double su[HY][HX];
+  __shared__ double sa[HY][kSx];
+  __shared__ double sb[HY][kSx];
+  __shared__ double rc[W][kSy][kSx];
+  __shared__ double rd[W][kSy][kSx];
+  const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
+  const size_t plane = (size_t)nbx * nby;
+  const size_t col_off = (size_t)(blockIdx.z / nzc) * plane * nbz;
   const double* __restrict__ u = u_all + col_off;
   double* __restrict__ v = v_all + col_off;
-  constexpr int W = 2 * p + 1;
-  constexpr int T = kTile;
-  constexpr int R = T + 2 * p;
-  __shared__ double su[R * R * R];
-  __shared__ double sa[R * R * T];
-  __shared__ double sb[R * R * T];
-  __shared__ double sc[R * T * T];
-  __shared__ double sd[R * T * T];
-  const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
-  const int x0 = blockIdx.x * T, y0 = blockIdx.y * T, z0 = (blockIdx.z % nzt) * T;
+  const int zs = (blockIdx.z % nzc) * zc;
+  const int ze = min(nbz, zs + zc);
+  const int x0 = blockIdx.x * kSx, y0 = blockIdx.y * kSy;
   const int tid = threadIdx.x;
-  const double c = d.mass_coef;
-  for (int i = tid; i < R * R * R; i += blockDim.x) {
-    const int lz = i / (R * R), ly = (i / R) % R, lx = i % R;
-    const int gx = wrap_or_invalid(x0 - p + lx, nbx, d.periodic[0]);
-    const int gy = wrap_or_invalid(y0 - p + ly, nby, d.periodic[1]);
-    const int gz = wrap_or_invalid(z0 - p + lz, nbz, d.periodic[2]);
-    su[i] = (gx >= 0 && gy >= 0 && gz >= 0) ? u[((size_t)gz * nby + gy) * nbx + gx] : 0.0;
+  const int tx = tid % kSx, ty = tid / kSx;
+  const double cm = d.mass_coef;
+  // This thread's halo elements: in-plane offset, or -1 outside the grid.
+  long long hoff[PER];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int e = tid + 256 * k;
+    hoff[k] = -1;
+    if (e < HALO) {
+      const int gx = wrap_or_invalid(x0 - p + e % HX, nbx, d.periodic[0]);
+      const int gy = wrap_or_invalid(y0 - p + e / HX, nby, d.periodic[1]);
+      if (gx >= 0 && gy >= 0) hoff[k] = (long long)gy * nbx + gx;
+    }
   }
-  __syncthreads();
-  // x pass: rows (lz, ly) of the halo region, outputs tx.
-  const double* __restrict__ bmx = d.band_m[0];
-  const double* __restrict__ bkx = d.band_k[0];
-  for (int i = tid; i < R * R * T; i += blockDim.x) {
-    const int lz = i / (R * T), ly = (i / T) % R, tx = i % T;
-    const int gx = x0 + tx;
-    double a = 0.0, b = 0.0;
-    if (gx < nbx) {
-      const double* uu = su + (lz * R + ly) * R + tx;
+  auto fetch = [&](int zi, double (&val)[PER]) {
+    const int gz = wrap_or_invalid(zi, nbz, d.periodic[2]);
+#pragma unroll
+    for (int k = 0; k < PER; ++k) val[k] = (gz >= 0 && hoff[k] >= 0) ? u[(size_t)gz * plane + hoff[k]] : 0.0;
+  };
+  const int gxo = x0 + tx, gyo = y0 + ty;
+  const int z_first = zs - p;
+  // Band rows held for the whole z run: this thread's x row (both of its
+  // x-pass items have x = x0 + tx) and y row in registers, the z rows of the
+  // block's output planes in LDS.
+  double mx[W], kx[W], my[W], ky[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) {
+    mx[k] = gxo < nbx ? d.band_m[0][gxo * W + k] : 0.0;
+    kx[k] = gxo < nbx ? d.band_k[0][gxo * W + k] + cm * d.band_m[0][gxo * W + k] : 0.0;
+    my[k] = gyo < nby ? d.band_m[1][gyo * W + k] : 0.0;
+    ky[k] = gyo < nby ? d.band_k[1][gyo * W + k] : 0.0;
+  }
+  __shared__ double mz[kMaxZc][W], kz[kMaxZc][W];
+  for (int i = tid; i < (ze - zs) * W; i += 256) {
+    mz[i / W][i % W] = d.band_m[2][(size_t)zs * W + i];
+    kz[i / W][i % W] = d.band_k[2][(size_t)zs * W + i];
+  }
+  double val[PER];
+  fetch(z_first, val);
+  for (int zi = z_first; zi < ze + p; ++zi) {
+    // Every thread finished the previous plane's x pass (it ended with the
+    // barrier after it), so su may be rewritten.
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + 256 * k;
+      if (e < HALO) su[e / HX][e % HX] = val[k];
+    }
+    if (zi + 1 < ze + p) fetch(zi + 1, val);
+    __syncthreads();
+    // x pass: a = Mx u, b = (Kx + c Mx) u on the HY rows of the halo.
+    for (int i = tid; i < HY * kSx; i += 256) {
+      const int ly = i / kSx, lx = i % kSx;  // lx == tx
+      double a = 0.0, b = 0.0;
 #pragma unroll
       for (int k = 0; k < W; ++k) {
-        const double m = bmx[gx * W + k];
-        const double kk = bkx[gx * W + k];
-        a += m * uu[k];
-        b += (kk + c * m) * uu[k];
+        const double uu = su[ly][lx + k];
+        a += mx[k] * uu;
+        b += kx[k] * uu;
       }
+      sa[ly][lx] = a;
+      sb[ly][lx] = b;
     }
-    sa[i] = a;
-    sb[i] = b;
-  }
-  __syncthreads();
-  // y pass: c = My b + Ky a, d = My a on (lz, ty, tx).
-  const double* __restrict__ bmy = d.band_m[1];
-  const double* __restrict__ bky = d.band_k[1];
-  for (int i = tid; i < R * T * T; i += blockDim.x) {
-    const int lz = i / (T * T), ty = (i / T) % T, tx = i % T;
-    const int gy = y0 + ty;
-    double cc_ = 0.0, dd = 0.0;
-    if (gy < nby) {
+    __syncthreads();
+    // y pass into the ring: c = My b + Ky a, d = My a.
+    const int slot = (zi - z_first) % W;
+    {
+      double cc_ = 0.0, dd = 0.0;
 #pragma unroll
       for (int k = 0; k < W; ++k) {
-        const double m = bmy[gy * W + k];
-        const double kk = bky[gy * W + k];
-        const int li = (lz * R + ty + k) * T + tx;
-        cc_ += m * sb[li] + kk * sa[li];
-        dd += m * sa[li];
+        cc_ += my[k] * sb[ty + k][tx] + ky[k] * sa[ty + k][tx];
+        dd += my[k] * sa[ty + k][tx];
       }
+      rc[slot][ty][tx] = cc_;
+      rd[slot][ty][tx] = dd;
     }
-    sc[i] = cc_;
-    sd[i] = dd;
-  }
-  __syncthreads();
-  // z pass: v = Mz c + Kz d.
-  const double* __restrict__ bmz = d.band_m[2];
-  const double* __restrict__ bkz = d.band_k[2];
-  for (int i = tid; i < T * T * T; i += blockDim.x) {
-    const int tz = i / (T * T), ty = (i / T) % T, tx = i % T;
-    const int gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
-    if (gx >= nbx || gy >= nby || gz >= nbz) continue;
-    double out = 0.0;
+    __syncthreads();
+    // z pass: output plane zo = zi - p from ring planes zo - p .. zo + p.
+    const int zo = zi - p;
+    if (zo >= zs && gxo < nbx && gyo < nby) {
+      double out = 0.0;
 #pragma unroll
-    for (int k = 0; k < W; ++k) {
-      const int li = ((tz + k) * T + ty) * T + tx;
-      out += bmz[gz * W + k] * sc[li] + bkz[gz * W + k] * sd[li];
+      for (int k = 0; k < W; ++k) {
+        const int sl = (zo - p + k - z_first) % W;
+        out += mz[zo - zs][k] * rc[sl][ty][tx] + kz[zo - zs][k] * rd[sl][ty][tx];
+      }
+      v[(size_t)zo * plane + (size_t)gyo * nbx + gxo] = out;
     }
-    v[((size_t)gz * nby + gy) * nbx + gx] = out;
   }
+}
+
+// z-chunk length: enough blocks (about 2048) with the shortest chunk
+// (4 planes, 2p + 4 input planes each) for small grids.
+static int tensor_zc(const cc_sumfact_desc& d, int K) {
+  const long long xy = (long long)((d.nb[0] + kSx - 1) / kSx) * ((d.nb[1] + kSy - 1) / kSy);
+  long long zc = (long long)d.nb[2] * xy * K / 2048;
+  if (zc < 4) zc = 4;
+  if (zc > kMaxZc) zc = kMaxZc;
+  return (int)zc;
+}
+
+template <int P1>
+static void tensor_launch(const cc_sumfact_desc& d, const double* u, double* v, int K, hipStream_t s) {
+  const int zc = tensor_zc(d, K);
+  const int nzc = (d.nb[2] + zc - 1) / zc;
+  const dim3 grid((d.nb[0] + kSx - 1) / kSx, (d.nb[1] + kSy - 1) / kSy, nzc * K);
+  hipLaunchKernelGGL((k_mf_tensor_z<P1>), grid, dim3(256), 0, s, d, u, v, nzc, zc);
 }
 
 // One wave per shell cell: W_s = C_s u_s - A_s u_s (or -A_s u_s).
@@ -570,8 +625,7 @@ static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   hipLaunchKernelGGL(k_mf_gather, dim3(grid1d(n_full)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0) hipLaunchKernelGGL(k_mf_gather_small, dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
-  dim3 tiles((d.nb[0] + kTile - 1) / kTile, (d.nb[1] + kTile - 1) / kTile, (d.nb[2] + kTile - 1) / kTile);
-  hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v, (int)tiles.z);
+  tensor_launch<P1>(d, b.u, b.v, 1, s);
   if (d.n_corr > 0) {
     hipLaunchKernelGGL((k_mf_shell<P1>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
     hipLaunchKernelGGL(k_mf_touch, dim3(grid_items(d.n_touch, 16)), dim3(256), 0, s, d, b.W, b.v);
@@ -648,9 +702,7 @@ static int mfk_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const doubl
   hipLaunchKernelGGL((k_mfk_gather<K>), dim3(grid1d(n_full * K)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0)
     hipLaunchKernelGGL((k_mfk_gather_small<K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
-  const int nzt = (d.nb[2] + kTile - 1) / kTile;
-  dim3 tiles((d.nb[0] + kTile - 1) / kTile, (d.nb[1] + kTile - 1) / kTile, nzt * K);
-  hipLaunchKernelGGL((k_mf_tensor<P1>), tiles, dim3(256), 0, s, d, b.u, b.v, nzt);
+  tensor_launch<P1>(d, b.u, b.v, K, s);
   if (d.n_corr > 0) {
     hipLaunchKernelGGL((k_mfk_shell<P1, K>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
     hipLaunchKernelGGL((k_mfk_touch<K>), dim3(grid_items(d.n_touch, 4)), dim3(256), 0, s, d, b.W, b.v, PP);
