@@ -205,71 +205,6 @@ static void tensor_launch(const cc_sumfact_desc& d, const double* u, double* v, 
   hipLaunchKernelGGL((k_mf_tensor_z<P1>), grid, dim3(256), 0, s, d, u, v, nzc, zc);
 }
 
-// One wave per shell cell: W_s = C_s u_s - A_s u_s (or -A_s u_s).
-template <int P1>
-__global__ __launch_bounds__(256) void k_mf_shell(cc_sumfact_desc d, const double* __restrict__ u,
-                                                  double* __restrict__ Wout) {
-  constexpr int PP = P1 * P1 * P1;
-  constexpr int WAVES = 4;
-  __shared__ double su[WAVES][PP];
-  __shared__ double sf[WAVES][6][P1 * P1];
-  const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
-  const int s = blockIdx.x * WAVES + wv;
-  const bool live = s < d.n_corr;
-  const int ex = live ? d.corr_cell[3 * s + 0] : 0;
-  const int ey = live ? d.corr_cell[3 * s + 1] : 0;
-  const int ez = live ? d.corr_cell[3 * s + 2] : 0;
-  if (live) {
-    for (int l = lane; l < PP; l += 64) {
-      const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
-      const int gx = wrap_or_invalid(ex + lx, d.nb[0], d.periodic[0]);
-      const int gy = wrap_or_invalid(ey + ly, d.nb[1], d.periodic[1]);
-      const int gz = wrap_or_invalid(ez + lz, d.nb[2], d.periodic[2]);
-      su[wv][l] = u[((size_t)gz * d.nb[1] + gy) * d.nb[0] + gx];
-    }
-    for (int l = lane; l < P1 * P1; l += 64) {
-      sf[wv][0][l] = d.elem_m[0][(size_t)ex * P1 * P1 + l];
-      sf[wv][1][l] = d.elem_k[0][(size_t)ex * P1 * P1 + l];
-      sf[wv][2][l] = d.elem_m[1][(size_t)ey * P1 * P1 + l];
-      sf[wv][3][l] = d.elem_k[1][(size_t)ey * P1 * P1 + l];
-      sf[wv][4][l] = d.elem_m[2][(size_t)ez * P1 * P1 + l];
-      sf[wv][5][l] = d.elem_k[2][(size_t)ez * P1 * P1 + l];
-    }
-  }
-  __syncthreads();
-  if (!live) return;
-  const int blk = d.corr_block[s];
-  const double c = d.mass_coef;
-  for (int l = lane; l < PP; l += 64) {
-    const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
-    double acc = 0.0;
-    // Full tensor block row l: sum_j (Kz My Mx + Mz Ky Mx + Mz My (Kx + c Mx)) u_j.
-    for (int jz = 0; jz < P1; ++jz) {
-      const double mz = sf[wv][4][lz * P1 + jz], kz = sf[wv][5][lz * P1 + jz];
-      for (int jy = 0; jy < P1; ++jy) {
-        const double my = sf[wv][2][ly * P1 + jy], ky = sf[wv][3][ly * P1 + jy];
-        double sx_m = 0.0, sx_k = 0.0;
-#pragma unroll
-        for (int jx = 0; jx < P1; ++jx) {
-          const double uj = su[wv][(jz * P1 + jy) * P1 + jx];
-          sx_m += sf[wv][0][lx * P1 + jx] * uj;
-          sx_k += sf[wv][1][lx * P1 + jx] * uj;
-        }
-        acc += (kz * my + mz * ky + c * mz * my) * sx_m + mz * my * sx_k;
-      }
-    }
-    double w = -acc;
-    if (blk >= 0) {
-      const double* __restrict__ Cb = d.blocks + (size_t)blk * PP * PP;
-      double cu = 0.0;
-      for (int j = 0; j < PP; ++j) cu += Cb[(size_t)j * PP + l] * su[wv][j];  // symmetric: column l = row l
-      w += cu;
-    }
-    Wout[(size_t)s * PP + l] = w;
-  }
-}
-
 // v[g] += sum of the shell-cell outputs W that belong to dof g; a 16-lane
 // team per touched dof (a dof lies in up to (p+1)^3 shell cells).
 __global__ __launch_bounds__(256) void k_mf_touch(cc_sumfact_desc d, const double* __restrict__ Win,
@@ -388,92 +323,163 @@ __global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, con
   if (lane < K) u[(size_t)c * n_full + d.small_dof[sm]] = s;
 }
 
-// One wave per shell cell and all K columns: W[s][l][c] = (C_s - A_s) u_s
-// (cut) or -A_s u_s (empty), columns handled CK at a time in registers. The
-// column index is fastest, so the touch pass reads the K columns of one
-// entry as one contiguous segment.
+// Shell corrections W[s][l][c] for all K columns: one workgroup per shell
+// cell, CK columns per wave (K / CK waves, so a workgroup's waves always take
+// the same branch and meet the same barriers). Lane l holds cell dof
+// (lx, ly, lz).
+//  - Cut cell: W = D_s u_s with the stored D_s = C_s - A_s. Row l of the
+//    symmetric block is its column l, so each step reads one 512-byte row
+//    segment; a group of rows sits in registers while the next group's
+//    loads are in flight. The waves of one cell read the same rows
+//    at the same time, so the block crosses HBM once.
+//  - Empty cell: W = -A_s u_s with A_s = Kx My Mz + Mx Ky Mz +
+//    Mx My (Kz + c Mz), by sum factorisation: one 1D contraction per axis
+//    (x: mx u, kx u; y: my, ky; z: mz, kz + c mz), 4 P1^4 multiply-adds per
+//    column instead of the dense block's 2 P1^6. The contraction along an
+//    axis reads, per step, the P1^2 values at one coordinate of that axis;
+//    each stage's input is stored with that axis slowest, so those values
+//    are contiguous in LDS and the read is free of bank conflicts (the P1
+//    lanes that share one value get it by broadcast).
+// The column index of W is fastest, so the touch pass reads the K columns
+// of one entry as one contiguous segment.
 template <int P1, int K>
-__global__ __launch_bounds__(256) void k_mfk_shell(cc_sumfact_desc d, const double* __restrict__ u,
-                                                   double* __restrict__ Wout) {
-  constexpr int PP = P1 * P1 * P1;
-  constexpr int WAVES = 4;
+__global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfact_desc d,
+                                                                      const double* __restrict__ u,
+                                                                      double* __restrict__ Wout) {
+  constexpr int QQ = P1 * P1;
+  constexpr int PP = QQ * P1;
   constexpr int CK = K < 8 ? K : 8;
-  __shared__ double su[WAVES][K][PP];
-  __shared__ double sf[WAVES][6][P1 * P1];
+  constexpr int WPC = K / CK;
+  static_assert(PP <= 64 && WPC * CK == K, "one lane per cell dof, whole column groups");
+  // Per wave: [0, CK) u of its columns, x slowest ((x*P1 + z)*P1 + y), later
+  // the first stage value of each column; [CK, 2 CK) the second.
+  __shared__ double su[WPC][2 * CK][PP];
+  __shared__ double sf[6][QQ];  // mx kx my ky mz kz of the cell
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int s = blockIdx.x * WAVES + wv;
-  const bool live = s < d.n_corr;
+  const int c0 = wv * CK;
+  const int s = blockIdx.x;
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
-  const int ex = live ? d.corr_cell[3 * s + 0] : 0;
-  const int ey = live ? d.corr_cell[3 * s + 1] : 0;
-  const int ez = live ? d.corr_cell[3 * s + 2] : 0;
-  if (live) {
-    for (int l = lane; l < PP; l += 64) {
-      const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
-      const int gx = wrap_or_invalid(ex + lx, d.nb[0], d.periodic[0]);
-      const int gy = wrap_or_invalid(ey + ly, d.nb[1], d.periodic[1]);
-      const int gz = wrap_or_invalid(ez + lz, d.nb[2], d.periodic[2]);
-      const size_t g = ((size_t)gz * d.nb[1] + gy) * d.nb[0] + gx;
+  const int ex = d.corr_cell[3 * s + 0];
+  const int ey = d.corr_cell[3 * s + 1];
+  const int ez = d.corr_cell[3 * s + 2];
+  if (lane < PP) {
+    const int lz = lane / QQ, ly = (lane / P1) % P1, lx = lane % P1;
+    const int gx = wrap_or_invalid(ex + lx, d.nb[0], d.periodic[0]);
+    const int gy = wrap_or_invalid(ey + ly, d.nb[1], d.periodic[1]);
+    const int gz = wrap_or_invalid(ez + lz, d.nb[2], d.periodic[2]);
+    const size_t g = ((size_t)gz * d.nb[1] + gy) * d.nb[0] + gx;
+    const int q = (lx * P1 + lz) * P1 + ly;
 #pragma unroll
-      for (int c = 0; c < K; ++c) su[wv][c][l] = u[(size_t)c * n_full + g];
-    }
-    for (int l = lane; l < P1 * P1; l += 64) {
-      sf[wv][0][l] = d.elem_m[0][(size_t)ex * P1 * P1 + l];
-      sf[wv][1][l] = d.elem_k[0][(size_t)ex * P1 * P1 + l];
-      sf[wv][2][l] = d.elem_m[1][(size_t)ey * P1 * P1 + l];
-      sf[wv][3][l] = d.elem_k[1][(size_t)ey * P1 * P1 + l];
-      sf[wv][4][l] = d.elem_m[2][(size_t)ez * P1 * P1 + l];
-      sf[wv][5][l] = d.elem_k[2][(size_t)ez * P1 * P1 + l];
+    for (int c = 0; c < CK; ++c) su[wv][c][q] = u[(size_t)(c0 + c) * n_full + g];
+  }
+  if (wv == 0) {
+    for (int i = lane; i < QQ; i += 64) {
+      sf[0][i] = d.elem_m[0][(size_t)ex * QQ + i];
+      sf[1][i] = d.elem_k[0][(size_t)ex * QQ + i];
+      sf[2][i] = d.elem_m[1][(size_t)ey * QQ + i];
+      sf[3][i] = d.elem_k[1][(size_t)ey * QQ + i];
+      sf[4][i] = d.elem_m[2][(size_t)ez * QQ + i];
+      sf[5][i] = d.elem_k[2][(size_t)ez * QQ + i];
     }
   }
   __syncthreads();
-  if (!live) return;
+  const bool act = lane < PP;
+  const int l = act ? lane : 0;
+  const int lz = l / QQ, ly = (l / P1) % P1, lx = l % P1;
+  double w[CK];
+#pragma unroll
+  for (int c = 0; c < CK; ++c) w[c] = 0.0;
   const int blk = d.corr_block[s];
-  const double cm = d.mass_coef;
-  for (int l = lane; l < PP; l += 64) {
-    const int lz = l / (P1 * P1), ly = (l / P1) % P1, lx = l % P1;
-    for (int c0 = 0; c0 < K; c0 += CK) {
-      double acc[CK];
+  if (blk >= 0) {
+    // Rows in groups of G, the next group's loads in flight meanwhile.
+    constexpr int G = PP % 8 == 0 ? 8 : QQ;
+    const double* __restrict__ Db = d.blocks + (size_t)blk * PP * PP + l;
+    double cur[G], nxt[G];
 #pragma unroll
-      for (int c = 0; c < CK; ++c) acc[c] = 0.0;
-      for (int jz = 0; jz < P1; ++jz) {
-        const double mz = sf[wv][4][lz * P1 + jz], kz = sf[wv][5][lz * P1 + jz];
-        for (int jy = 0; jy < P1; ++jy) {
-          const double my = sf[wv][2][ly * P1 + jy], ky = sf[wv][3][ly * P1 + jy];
-          const double cyz = kz * my + mz * ky + cm * mz * my, myz = mz * my;
-          double sm[CK], sk[CK];
+    for (int r = 0; r < G; ++r) cur[r] = Db[(size_t)r * PP];
+#pragma unroll 1
+    for (int j0 = 0; j0 < PP; j0 += G) {
+      if (j0 + G < PP) {
 #pragma unroll
-          for (int c = 0; c < CK; ++c) sm[c] = sk[c] = 0.0;
-#pragma unroll
-          for (int jx = 0; jx < P1; ++jx) {
-            const double mx = sf[wv][0][lx * P1 + jx], kx = sf[wv][1][lx * P1 + jx];
-            const int j = (jz * P1 + jy) * P1 + jx;
-#pragma unroll
-            for (int c = 0; c < CK; ++c) {
-              const double uj = su[wv][c0 + c][j];
-              sm[c] += mx * uj;
-              sk[c] += kx * uj;
-            }
-          }
-#pragma unroll
-          for (int c = 0; c < CK; ++c) acc[c] += cyz * sm[c] + myz * sk[c];
-        }
-      }
-      double w[CK];
-#pragma unroll
-      for (int c = 0; c < CK; ++c) w[c] = -acc[c];
-      if (blk >= 0) {
-        const double* __restrict__ Cb = d.blocks + (size_t)blk * PP * PP;
-        for (int j = 0; j < PP; ++j) {
-          const double cb = Cb[(size_t)j * PP + l];  // symmetric: column l = row l
-#pragma unroll
-          for (int c = 0; c < CK; ++c) w[c] += cb * su[wv][c0 + c][j];
-        }
+        for (int r = 0; r < G; ++r) nxt[r] = Db[(size_t)(j0 + G + r) * PP];
       }
 #pragma unroll
-      for (int c = 0; c < CK; ++c) Wout[((size_t)s * PP + l) * K + c0 + c] = w[c];
+      for (int r = 0; r < G; ++r) {
+        const int j = j0 + r;  // (jz, jy, jx) = (j / QQ, j / P1 % P1, j % P1)
+        const int q = ((j % P1) * P1 + j / QQ) * P1 + (j / P1) % P1;
+#pragma unroll
+        for (int c = 0; c < CK; ++c) w[c] += cur[r] * su[wv][c][q];
+      }
+#pragma unroll
+      for (int r = 0; r < G; ++r) cur[r] = nxt[r];
     }
+  } else {
+    const double cm = d.mass_coef;
+    // x: a = mx u, b = kx u, stored y slowest.
+    double a[CK], b[CK], am[CK];
+#pragma unroll
+    for (int c = 0; c < CK; ++c) a[c] = b[c] = 0.0;
+#pragma unroll
+    for (int jx = 0; jx < P1; ++jx) {
+      const double mx = sf[0][lx * P1 + jx], kx = sf[1][lx * P1 + jx];
+      const int q = (jx * P1 + lz) * P1 + ly;
+#pragma unroll
+      for (int c = 0; c < CK; ++c) {
+        const double uj = su[wv][c][q];
+        a[c] += mx * uj;
+        b[c] += kx * uj;
+      }
+    }
+    __syncthreads();
+    if (act) {
+      const int q = (ly * P1 + lz) * P1 + lx;
+#pragma unroll
+      for (int c = 0; c < CK; ++c) {
+        su[wv][c][q] = a[c];
+        su[wv][CK + c][q] = b[c];
+      }
+    }
+    __syncthreads();
+    // y: ky a + my b (summed, both meet mz) and my a; stored z slowest.
+#pragma unroll
+    for (int c = 0; c < CK; ++c) a[c] = b[c] = am[c] = 0.0;
+#pragma unroll
+    for (int jy = 0; jy < P1; ++jy) {
+      const double my = sf[2][ly * P1 + jy], ky = sf[3][ly * P1 + jy];
+      const int q = (jy * P1 + lz) * P1 + lx;
+#pragma unroll
+      for (int c = 0; c < CK; ++c) {
+        const double va = su[wv][c][q], vb = su[wv][CK + c][q];
+        am[c] += my * va;
+        a[c] += ky * va;
+        b[c] += my * vb;
+      }
+    }
+    __syncthreads();
+    if (act) {
+      const int q = (lz * P1 + ly) * P1 + lx;
+#pragma unroll
+      for (int c = 0; c < CK; ++c) {
+        su[wv][c][q] = a[c] + b[c];
+        su[wv][CK + c][q] = am[c];
+      }
+    }
+    __syncthreads();
+    // z: -(mz (ky mx + my kx) u + (kz + c mz) my mx u).
+#pragma unroll
+    for (int jz = 0; jz < P1; ++jz) {
+      const double mz = sf[4][lz * P1 + jz];
+      const double kzc = __fma_rn(cm, mz, sf[5][lz * P1 + jz]);
+      const int q = (jz * P1 + ly) * P1 + lx;
+      // Explicit roundings, so every K gives each column the same result.
+#pragma unroll
+      for (int c = 0; c < CK; ++c) w[c] -= __fma_rn(mz, su[wv][c][q], __dmul_rn(kzc, su[wv][CK + c][q]));
+    }
+  }
+  if (act) {
+#pragma unroll
+    for (int c = 0; c < CK; ++c) Wout[((size_t)s * PP + l) * K + c0 + c] = w[c];
   }
 }
 
@@ -627,7 +633,7 @@ static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double
   if (d.n_small > 0) hipLaunchKernelGGL(k_mf_gather_small, dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
   tensor_launch<P1>(d, b.u, b.v, 1, s);
   if (d.n_corr > 0) {
-    hipLaunchKernelGGL((k_mf_shell<P1>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
+    hipLaunchKernelGGL((k_mfk_shell<P1, 1>), dim3(d.n_corr), dim3(64), 0, s, d, b.u, b.W);
     hipLaunchKernelGGL(k_mf_touch, dim3(grid_items(d.n_touch, 16)), dim3(256), 0, s, d, b.W, b.v);
   }
   if (d.n_et_rows > 0)
@@ -704,7 +710,7 @@ static int mfk_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const doubl
     hipLaunchKernelGGL((k_mfk_gather_small<K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
   tensor_launch<P1>(d, b.u, b.v, K, s);
   if (d.n_corr > 0) {
-    hipLaunchKernelGGL((k_mfk_shell<P1, K>), dim3((d.n_corr + 3) / 4), dim3(256), 0, s, d, b.u, b.W);
+    hipLaunchKernelGGL((k_mfk_shell<P1, K>), dim3(d.n_corr), dim3(64 * (K < 8 ? 1 : K / 8)), 0, s, d, b.u, b.W);
     hipLaunchKernelGGL((k_mfk_touch<K>), dim3(grid_items(d.n_touch, 4)), dim3(256), 0, s, d, b.W, b.v, PP);
   }
   if (d.n_et_rows > 0)

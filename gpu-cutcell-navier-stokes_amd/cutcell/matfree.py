@@ -186,6 +186,15 @@ def shell_blocks(mf: MatrixFreeSystem) -> np.ndarray:
     return D
 
 
+def cut_corrections(mf: MatrixFreeSystem) -> np.ndarray:
+    """``C_e - A_e`` for each cut block, in block order: what a cut cell adds
+    to the full-grid tensor apply (the device descriptor's ``blocks``)."""
+    D = mf.blocks.copy()
+    cutm = mf.corr_block >= 0
+    D[mf.corr_block[cutm]] -= _full_cell_blocks(mf.grid, mf.corr_cells[cutm], mf.elem_m, mf.elem_k, mf.mass_coef)
+    return D
+
+
 def jacobi_diagonal(mf: MatrixFreeSystem) -> np.ndarray:
     """Exact ``diag(E^T A E)``.
 
@@ -343,7 +352,7 @@ class DeviceMatrixFree:
         d.n_corr = mf.corr_cells.shape[0]
         d.corr_cell = t(mf.corr_cells, np.int32)
         d.corr_block = t(mf.corr_block, np.int32)
-        d.blocks = t(mf.blocks if mf.blocks.size else np.zeros(1), np.float64)
+        d.blocks = t(cut_corrections(mf) if mf.blocks.size else np.zeros(1), np.float64)
         d.n_touch = mf.touch_dof.size
         d.touch_dof = t(mf.touch_dof, np.int32)
         d.touch_ptr = t(mf.touch_ptr, np.int32)

@@ -234,7 +234,9 @@ int cc_pcg_batched(int32_t nsys, int32_t n_max, const int32_t* sys_n, const int6
  *  - corr_cell[n_corr][3] (ex, ey, ez): the shell of cells that are not
  *    fully fluid but touch an active dof; corr_block[s] >= 0 indexes
  *    blocks[][P][P] (P = (degree+1)^3, local order (z, y, x)), the cut cell's
- *    Gram block of K + c M, and -1 marks an empty cell;
+ *    Gram block of K + c M minus the full cell's tensor block (C_e - A_e,
+ *    the correction the cell adds to the full-grid tensor apply), and -1
+ *    marks an empty cell (correction -A_e, applied by sum factorisation);
  *  - touch_dof/touch_ptr/touch_idx: CSR over the dofs that receive shell
  *    corrections, entries index the per-cell outputs (s * P + local);
  *  - red2full[n_red]: interior dof of each unknown; full2red/full2small:
@@ -296,8 +298,8 @@ int cc_pcg_sumfact(const cc_sumfact_desc* d, const double* b, double* x, double 
                    double* snapshots, cc_stream_t stream);
 
 /* The same operator for k in {1, 2, 4, 8, 16} right-hand sides at once (X,
- * Y, B row-major [n_red][k]); the cut blocks are read once per apply for all
- * k columns. cc_pcg_sumfact_k runs Jacobi-PCG on the k columns (semantics of
+ * Y, B row-major [n_red][k]); each cut block is streamed from memory once per
+ * apply for all k columns. cc_pcg_sumfact_k runs Jacobi-PCG on the k columns (semantics of
  * cc_pcg_jacobi_f64 per column; no history or snapshots); synchronises;
  * results[k] host memory. (No reference counterpart: the reference solves
  * one right-hand side per call, code/fast_solver_backends.py:194-289.) */
