@@ -466,19 +466,27 @@ def learned_section(hip, torch, args):
         f"{np.mean(rep.history[-20:]):.3f}")
     refine = None
     if args.learned_refine_rhs > 0:
-        t0 = time.perf_counter()
-        rrhs = [torch.from_numpy(rng.standard_normal(n)).to("cuda") for _ in range(args.learned_refine_rhs)]
-        R2, E2 = generate_pairs_fcg(net, dmf, rrhs, REFINE_ITERS)
-        torch.cuda.synchronize()
-        pairs2_s = time.perf_counter() - t0
-        rep2 = net.train(torch.cat([R, R2]), torch.cat([E, E2]), diag, steps=args.learned_refine_steps, batch=8,
-                         lr=args.learned_refine_lr)
-        refine = {"fcg_pairs": int(R2.shape[0]), "rhs": args.learned_refine_rhs, "iters": list(REFINE_ITERS),
-                  "pairs_s": sig(pairs2_s), "steps": rep2.steps, "lr": args.learned_refine_lr,
-                  "seconds": sig(rep2.seconds)}
-        log(f"learned: fine-tuned on {R2.shape[0]} pairs from the learned FCG's own iterates "
-            f"({pairs2_s:.1f} s) plus the Jacobi pairs, {rep2.steps} steps in {rep2.seconds:.1f} s")
-        del R2, E2
+        # Each round takes pairs from the current network's own FCG iterates
+        # and fine-tunes on every pair so far.
+        refine = {"rounds": [], "rhs": args.learned_refine_rhs, "iters": list(REFINE_ITERS),
+                  "steps": args.learned_refine_steps, "lr": args.learned_refine_lr}
+        Rs, Es = [R], [E]
+        for rnd in range(args.learned_refine_rounds):
+            t0 = time.perf_counter()
+            rrhs = [torch.from_numpy(rng.standard_normal(n)).to("cuda") for _ in range(args.learned_refine_rhs)]
+            R2, E2 = generate_pairs_fcg(net, dmf, rrhs, REFINE_ITERS)
+            torch.cuda.synchronize()
+            pairs2_s = time.perf_counter() - t0
+            Rs.append(R2)
+            Es.append(E2)
+            rep2 = net.train(torch.cat(Rs), torch.cat(Es), diag, steps=args.learned_refine_steps, batch=8,
+                             lr=args.learned_refine_lr)
+            refine["rounds"].append({"fcg_pairs": int(R2.shape[0]), "pairs_s": sig(pairs2_s),
+                                     "seconds": sig(rep2.seconds)})
+            log(f"learned: fine-tuning round {rnd + 1}: {R2.shape[0]} pairs from the learned FCG's own iterates "
+                f"({pairs2_s:.1f} s) plus all earlier pairs, {rep2.steps} steps in {rep2.seconds:.1f} s")
+            del rrhs, R2, E2
+        del Rs, Es
     del R, E
     torch.cuda.empty_cache()
     B = [torch.from_numpy(rng.standard_normal(n)).to("cuda") for _ in range(args.learned_eval)]
@@ -533,6 +541,8 @@ def main(argv=None) -> int:
                     help="sphere box: right-hand sides whose learned-FCG iterates give fine-tuning pairs (0: none)")
     ap.add_argument("--learned-refine-steps", type=int, default=1500, help="fine-tuning Adam steps, sphere box")
     ap.add_argument("--learned-refine-lr", type=float, default=1e-3, help="fine-tuning learning rate, sphere box")
+    ap.add_argument("--learned-refine-rounds", type=int, default=2,
+                    help="sphere box: fine-tuning rounds, each on pairs from the current network's iterates")
     ap.add_argument("--learned-eval", type=int, default=16, help="right-hand sides timed per method (1-16)")
     ap.add_argument("--ml-levels", type=int, default=5, help="grids of the multilevel learned preconditioner")
     ap.add_argument("--no-pressure", action="store_true", help="skip the bump-channel pressure-projection section")
