@@ -327,10 +327,13 @@ __global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, con
 // cell, CK columns per wave (K / CK waves, so a workgroup's waves always take
 // the same branch and meet the same barriers). Lane l holds cell dof
 // (lx, ly, lz).
-//  - Cut cell: W = D_s u_s with the stored D_s = C_s - A_s. Row l of the
-//    symmetric block is its column l, so each step reads one 512-byte row
-//    segment; a group of rows sits in registers while the next group's
-//    loads are in flight. The waves of one cell read the same rows
+//  - Cut cell: W = D_s u_s with the stored D_s = C_s - A_s. For K = 16 the
+//    two waves each take 32 rows of W on the f64 MFMA (16x16x4): the A
+//    operand is read from HBM (row l of the symmetric block is its column l),
+//    the B operand is one u value per lane from LDS, so each byte of the
+//    block and of u is read once. Otherwise each step reads one 512-byte row
+//    segment against the wave's columns (a group of rows in registers, the
+//    next group's loads in flight); the waves of one cell read the same rows
 //    at the same time, so the block crosses HBM once.
 //  - Empty cell: W = -A_s u_s with A_s = Kx My Mz + Mx Ky Mz +
 //    Mx My (Kz + c Mz), by sum factorisation: one 1D contraction per axis
@@ -342,6 +345,21 @@ __global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, con
 //    lanes that share one value get it by broadcast).
 // The column index of W is fastest, so the touch pass reads the K columns
 // of one entry as one contiguous segment.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+// LDS layout of the shell kernel's per-wave values: rows of 2 CK columns,
+// each row P1 slabs of P1^2 dofs in stage order. Slab and row padding put
+// the 64 lanes of an MFMA operand read (16 columns x 4 slabs) on every bank
+// pair exactly four times; a stage read stays 16 contiguous doubles.
+template <int P1, int CK>
+struct ShellLds {
+  static constexpr int QQ = P1 * P1;
+  static constexpr int SLAB = QQ + (P1 == 4 ? 4 : 0);
+  static constexpr int ROW = P1 * SLAB + 1;
+  static constexpr int WST = 2 * CK * ROW + 8;
+  static __device__ __forceinline__ int at(int h, int r, int q) { return h * WST + r * ROW + (q / QQ) * SLAB + q % QQ; }
+};
+
 template <int P1, int K>
 __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfact_desc d,
                                                                       const double* __restrict__ u,
@@ -351,9 +369,10 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
   constexpr int CK = K < 8 ? K : 8;
   constexpr int WPC = K / CK;
   static_assert(PP <= 64 && WPC * CK == K, "one lane per cell dof, whole column groups");
-  // Per wave: [0, CK) u of its columns, x slowest ((x*P1 + z)*P1 + y), later
-  // the first stage value of each column; [CK, 2 CK) the second.
-  __shared__ double su[WPC][2 * CK][PP];
+  // Per wave: rows [0, CK) u of its columns, x slowest ((x*P1 + z)*P1 + y),
+  // later the first stage value of each column; [CK, 2 CK) the second.
+  using L = ShellLds<P1, CK>;
+  __shared__ double su[WPC * L::WST];
   __shared__ double sf[6][QQ];  // mx kx my ky mz kz of the cell
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -371,7 +390,7 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
     const size_t g = ((size_t)gz * d.nb[1] + gy) * d.nb[0] + gx;
     const int q = (lx * P1 + lz) * P1 + ly;
 #pragma unroll
-    for (int c = 0; c < CK; ++c) su[wv][c][q] = u[(size_t)(c0 + c) * n_full + g];
+    for (int c = 0; c < CK; ++c) su[L::at(wv, c, q)] = u[(size_t)(c0 + c) * n_full + g];
   }
   if (wv == 0) {
     for (int i = lane; i < QQ; i += 64) {
@@ -391,6 +410,39 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
 #pragma unroll
   for (int c = 0; c < CK; ++c) w[c] = 0.0;
   const int blk = d.corr_block[s];
+  if constexpr (K == 16 && P1 == 4) {
+    if (blk >= 0) {
+      // W rows [32 wv, 32 wv + 32) = two 16-row tiles. Step t covers block
+      // rows j = 4t + kk (kk = lane / 16): A[i][kk] = D[16 m + i][j] is read
+      // as D[j][16 m + i]; B[kk][c] = u_j of column c = lane % 16, which sits
+      // in wave c / 8, row c % 8, at stage-order position 16 kk + t.
+      const int i16 = lane & 15, kk = lane >> 4;
+      const double* __restrict__ Db = d.blocks + (size_t)blk * PP * PP + (size_t)kk * PP + 32 * wv + i16;
+      double a0[16], a1[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        a0[t] = Db[(size_t)(4 * t) * PP];
+        a1[t] = Db[(size_t)(4 * t) * PP + 16];
+      }
+      v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+      const int bbase = L::at(i16 >> 3, i16 & 7, 16 * kk);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const double bv = su[bbase + t];
+        acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[t], bv, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[t], bv, acc1, 0, 0, 0);
+      }
+      // f64 16x16 tile layout (unlike the f32 ones): register r of lane
+      // (i16, kk) holds row kk + 4 r of column i16.
+      const size_t row0 = (size_t)s * PP + 32 * wv + kk;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Wout[(row0 + 4 * r) * K + i16] = acc0[r];
+        Wout[(row0 + 16 + 4 * r) * K + i16] = acc1[r];
+      }
+      return;
+    }
+  }
   if (blk >= 0) {
     // Rows in groups of G, the next group's loads in flight meanwhile.
     constexpr int G = PP % 8 == 0 ? 8 : QQ;
@@ -409,7 +461,7 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
         const int j = j0 + r;  // (jz, jy, jx) = (j / QQ, j / P1 % P1, j % P1)
         const int q = ((j % P1) * P1 + j / QQ) * P1 + (j / P1) % P1;
 #pragma unroll
-        for (int c = 0; c < CK; ++c) w[c] += cur[r] * su[wv][c][q];
+        for (int c = 0; c < CK; ++c) w[c] += cur[r] * su[L::at(wv, c, q)];
       }
 #pragma unroll
       for (int r = 0; r < G; ++r) cur[r] = nxt[r];
@@ -426,7 +478,7 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       const int q = (jx * P1 + lz) * P1 + ly;
 #pragma unroll
       for (int c = 0; c < CK; ++c) {
-        const double uj = su[wv][c][q];
+        const double uj = su[L::at(wv, c, q)];
         a[c] += mx * uj;
         b[c] += kx * uj;
       }
@@ -436,8 +488,8 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       const int q = (ly * P1 + lz) * P1 + lx;
 #pragma unroll
       for (int c = 0; c < CK; ++c) {
-        su[wv][c][q] = a[c];
-        su[wv][CK + c][q] = b[c];
+        su[L::at(wv, c, q)] = a[c];
+        su[L::at(wv, CK + c, q)] = b[c];
       }
     }
     __syncthreads();
@@ -450,7 +502,7 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       const int q = (jy * P1 + lz) * P1 + lx;
 #pragma unroll
       for (int c = 0; c < CK; ++c) {
-        const double va = su[wv][c][q], vb = su[wv][CK + c][q];
+        const double va = su[L::at(wv, c, q)], vb = su[L::at(wv, CK + c, q)];
         am[c] += my * va;
         a[c] += ky * va;
         b[c] += my * vb;
@@ -461,8 +513,8 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       const int q = (lz * P1 + ly) * P1 + lx;
 #pragma unroll
       for (int c = 0; c < CK; ++c) {
-        su[wv][c][q] = a[c] + b[c];
-        su[wv][CK + c][q] = am[c];
+        su[L::at(wv, c, q)] = a[c] + b[c];
+        su[L::at(wv, CK + c, q)] = am[c];
       }
     }
     __syncthreads();
@@ -474,7 +526,7 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       const int q = (jz * P1 + ly) * P1 + lx;
       // Explicit roundings, so every K gives each column the same result.
 #pragma unroll
-      for (int c = 0; c < CK; ++c) w[c] -= __fma_rn(mz, su[wv][c][q], __dmul_rn(kzc, su[wv][CK + c][q]));
+      for (int c = 0; c < CK; ++c) w[c] -= __fma_rn(mz, su[L::at(wv, c, q)], __dmul_rn(kzc, su[L::at(wv, CK + c, q)]));
     }
   }
   if (act) {
