@@ -520,6 +520,30 @@ def learned_section(hip, torch, args):
     return out
 
 
+def operator_probe(hip, torch, args, reps: int = 30):
+    """Back-to-back applies of one matrix-free operator: 16 columns, then one."""
+    from cutcell.matfree import DeviceMatrixFree
+
+    mf = build_sphere_operator(args.seed, args.learned_n) if args.operator_probe == "sphere" \
+        else build_channel_operator()
+    dmf = DeviceMatrixFree(mf)
+    rng = np.random.default_rng(args.seed)
+    X = torch.from_numpy(rng.standard_normal((mf.n, 16))).to("cuda").contiguous()
+    x1 = X[:, 0].contiguous()
+    out = {"operator": args.operator_probe, "n": mf.n, "n_shell_cells": int(mf.corr_cells.shape[0]),
+           "n_cut_cells": int(mf.blocks.shape[0]), "reps": reps}
+    for name, f in (("apply_k16_us", lambda: dmf.apply_k(X)), ("apply_k1_us", lambda: dmf.apply(x1))):
+        for _ in range(3):
+            f()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            f()
+        torch.cuda.synchronize()
+        out[name] = sig((time.perf_counter() - t0) / reps * 1e6)
+    return out
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -556,6 +580,9 @@ def main(argv=None) -> int:
     ap.add_argument("--copy-probe-mb", type=float, default=0.0,
                     help="before timing, copy a buffer of this many MiB once (a known-size kernel that calibrates "
                          "the FETCH_SIZE / WRITE_SIZE counters of a profiler run)")
+    ap.add_argument("--operator-probe", choices=("sphere", "channel"), default=None,
+                    help="only time the matrix-free operator of the sphere box or the bump channel (16-column and "
+                         "single-column applies, for a kernel trace) and print one JSON line")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -571,6 +598,9 @@ def main(argv=None) -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 2
     hip.load_library()
+    if args.operator_probe:
+        print(json.dumps(operator_probe(hip, torch, args)))
+        return 0
 
     log("building the 3D planar-cut system (p=3, ~81k unknowns) ...")
     t0 = time.perf_counter()
