@@ -48,6 +48,8 @@ namespace cc {
 
 struct Profiler {
   bool on = false;
+  uint32_t classes = 0;  // bit c: class c is timed
+  bool open = false;     // a timed class's first mark is down (pairs never nest)
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<std::pair<int, size_t>> pending;
@@ -69,14 +71,16 @@ static void prof_mark(hipStream_t s) {
 }
 
 void prof_begin(int cls, hipStream_t s) {
-  if (!g_prof.on) return;
+  if (!g_prof.on || !(g_prof.classes >> cls & 1u)) return;
   g_prof.pending.push_back({cls, g_prof.used});
   prof_mark(s);
+  g_prof.open = true;
 }
 
 void prof_end(hipStream_t s) {
-  if (!g_prof.on) return;
+  if (!g_prof.on || !g_prof.open) return;
   prof_mark(s);
+  g_prof.open = false;
 }
 
 // Call only after the stream has been synchronised.
@@ -1422,12 +1426,16 @@ int cc_device_count(int* count) {
   return (int)hipGetDeviceCount(count);
 }
 
-int cc_profile_enable(int32_t on) {
-  g_prof.on = on != 0;
+int cc_profile_enable_classes(int32_t mask) {
+  g_prof.classes = (uint32_t)mask & ((1u << kProfClasses) - 1u);
+  g_prof.on = g_prof.classes != 0;
+  g_prof.open = false;
   g_prof.pending.clear();
   g_prof.used = 0;
   return 0;
 }
+
+int cc_profile_enable(int32_t on) { return cc_profile_enable_classes(on ? -1 : 0); }
 
 int cc_profile_reset(void) {
   for (int i = 0; i < kProfClasses; ++i) {

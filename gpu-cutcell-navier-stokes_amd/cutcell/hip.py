@@ -57,6 +57,7 @@ _SIGNATURES = {
     "cc_version": ([], ctypes.c_int),
     "cc_device_count": ([_P], ctypes.c_int),
     "cc_profile_enable": ([_I32], ctypes.c_int),
+    "cc_profile_enable_classes": ([_I32], ctypes.c_int),
     "cc_profile_reset": ([], ctypes.c_int),
     "cc_profile_read": ([_I32, _P, _P], ctypes.c_int),
     "cc_profile_collect": ([], ctypes.c_int),
@@ -323,9 +324,18 @@ def residual(A: DeviceCSR, B, X) -> np.ndarray:
 PROFILE_CLASSES = {"spmm": 0, "update": 1, "direction": 2, "batched": 3, "learned": 4}
 
 
-def profile_enable(on: bool = True) -> None:
+def profile_enable(on=True) -> None:
+    """Event timing of the solver kernels: True (every class), False (off) or
+    an iterable of class names (only those: each timed launch adds its two
+    events' dispatch to the stream)."""
     lib = load_library()
-    lib.cc_profile_enable(1 if on else 0)
+    if isinstance(on, bool):
+        lib.cc_profile_enable(1 if on else 0)
+    else:
+        mask = 0
+        for name in on:
+            mask |= 1 << PROFILE_CLASSES[name]
+        lib.cc_profile_enable_classes(mask)
     lib.cc_profile_reset()
 
 

@@ -69,8 +69,13 @@ int cc_device_count(int* count);
  * stream; cc_profile_read returns the summed milliseconds and launch count of
  * a class: 0 = SpMV/SpMM (or matrix-free apply) of the CG loops, 1 = CG
  * update, 2 = CG direction, 3 = batched persistent solve, 4 = learned
- * preconditioner apply. Process-wide and not thread-safe. */
+ * preconditioner apply. Process-wide and not thread-safe. Each timed launch
+ * adds the dispatch of its two events to the stream, so cc_profile_enable_classes
+ * times only the classes whose bit is set in mask (bit c = class c; 0 = off):
+ * the benchmark brackets only the product it reports. cc_profile_enable(on)
+ * times every class (on != 0) or none. */
 int cc_profile_enable(int32_t on);
+int cc_profile_enable_classes(int32_t mask);
 int cc_profile_reset(void);
 int cc_profile_read(int32_t which, double* total_ms, int64_t* count);
 /* Folds the events of launches made outside the solvers' own polling (the
