@@ -548,11 +548,23 @@ __global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const doub
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   for (int t = wave; t < d.n_touch; t += nwaves) {
     double s = 0.0;
-    for (int i = d.touch_ptr[t] + el; i < d.touch_ptr[t + 1]; i += EL) {
+    const int iend = d.touch_ptr[t + 1];
+    int i = d.touch_ptr[t] + el;
+    // Four entries' indices, then their values, in flight at once; the sum
+    // keeps the one-entry-at-a-time order.
+    for (; i + 3 * EL < iend; i += 4 * EL) {
+      const int a0 = d.touch_idx[i], a1 = d.touch_idx[i + EL];
+      const int a2 = d.touch_idx[i + 2 * EL], a3 = d.touch_idx[i + 3 * EL];
       // Win[cell][local dof][K]: the K columns of one entry are one
       // contiguous 8 K-byte segment.
-      s += Win[(size_t)d.touch_idx[i] * K + c];
+      const double w0 = Win[(size_t)a0 * K + c], w1 = Win[(size_t)a1 * K + c];
+      const double w2 = Win[(size_t)a2 * K + c], w3 = Win[(size_t)a3 * K + c];
+      s += w0;
+      s += w1;
+      s += w2;
+      s += w3;
     }
+    for (; i < iend; i += EL) s += Win[(size_t)d.touch_idx[i] * K + c];
     for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
     if (lane < K) v[(size_t)c * n_full + d.touch_dof[t]] += s;
   }
