@@ -317,8 +317,22 @@ __global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, con
   if (sm >= d.n_small) return;
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   double s = 0.0;
-  for (int t = d.small_ptr[sm] + el; t < d.small_ptr[sm + 1]; t += EL)
-    s += d.small_w[t] * x[(size_t)d.small_col[t] * K + c];
+  const int tend = d.small_ptr[sm + 1];
+  int t = d.small_ptr[sm] + el;
+  // Four entries' loads in flight at once, summed in entry order.
+  for (; t + 3 * EL < tend; t += 4 * EL) {
+    int col[4];
+    double w[4], xv[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) col[q] = d.small_col[t + q * EL];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) w[q] = d.small_w[t + q * EL];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xv[q] = x[(size_t)col[q] * K + c];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s += w[q] * xv[q];
+  }
+  for (; t < tend; t += EL) s += d.small_w[t] * x[(size_t)d.small_col[t] * K + c];
   for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
   if (lane < K) u[(size_t)c * n_full + d.small_dof[sm]] = s;
 }
@@ -583,8 +597,24 @@ __global__ __launch_bounds__(256) void k_mfk_et_extra(cc_sumfact_desc d, const d
   for (int i = wave; i < d.n_et_rows; i += nwaves) {
     const int j = d.et_rows[i];
     double s = 0.0;
-    for (int t = d.et_ptr[j] + el; t < d.et_ptr[j + 1]; t += EL)
-      s += d.et_w[t] * v[(size_t)c * n_full + d.small_dof[d.et_small[t]]];
+    const int tend = d.et_ptr[j + 1];
+    int t = d.et_ptr[j] + el;
+    // Four entries' index chains in flight at once, summed in entry order.
+    for (; t + 3 * EL < tend; t += 4 * EL) {
+      int b[4];
+      double w[4], vv[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = d.et_small[t + q * EL];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) w[q] = d.et_w[t + q * EL];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) b[q] = d.small_dof[b[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) vv[q] = v[(size_t)c * n_full + b[q]];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) s += w[q] * vv[q];
+    }
+    for (; t < tend; t += EL) s += d.et_w[t] * v[(size_t)c * n_full + d.small_dof[d.et_small[t]]];
     for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
     if (lane < K) extra[(size_t)j * K + c] = s;
   }
