@@ -293,16 +293,39 @@ __global__ __launch_bounds__(256) void k_mf_residual(int n, const double* __rest
 // columns. Shell outputs are [n_corr][K][P], extra is [n_red][K].
 // ---------------------------------------------------------------------------
 
+// u[c][g] = x[full2red[g]][c] (0 off the interior), through an LDS tile of
+// 1024 / K dofs x K columns: the reads are whole K-column rows of x, the
+// writes runs of consecutive dofs of one column.
 template <int K>
 __global__ __launch_bounds__(256) void k_mfk_gather(cc_sumfact_desc d, const double* __restrict__ x,
                                                     double* __restrict__ u) {
+  constexpr int TG = 1024 / K;
+  __shared__ double tile[K][TG + 1];
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
-  const size_t total = n_full * K;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t g = e / K;
-    const int c = (int)(e % K);
-    const int r = d.full2red[g];
-    u[(size_t)c * n_full + g] = r >= 0 ? x[(size_t)r * K + c] : 0.0;
+  const size_t ntiles = (n_full + TG - 1) / TG;
+  for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+    const size_t g0 = tb * TG;
+    __syncthreads();  // the previous tile's column runs are written
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = threadIdx.x + 256 * q;
+      const int gi = k / K, c = k % K;
+      const size_t g = g0 + gi;
+      double val = 0.0;
+      if (g < n_full) {
+        const int r = d.full2red[g];
+        if (r >= 0) val = x[(size_t)r * K + c];
+      }
+      tile[c][gi] = val;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = threadIdx.x + 256 * q;
+      const int c = k / TG, gi = k % TG;
+      const size_t g = g0 + gi;
+      if (g < n_full) u[(size_t)c * n_full + g] = tile[c][gi];
+    }
   }
 }
 
@@ -625,17 +648,39 @@ template <int K>
 __global__ __launch_bounds__(256) void k_mfk_scatter(cc_sumfact_desc d, const double* __restrict__ v,
                                                      const double* __restrict__ extra, double* __restrict__ y,
                                                      const double* __restrict__ x, double* __restrict__ partial) {
+  // Through an LDS tile of 1024 / K unknowns x K columns: the reads of v are
+  // runs of one column's grid, the writes of y whole K-column rows.
+  constexpr int TJ = 1024 / K;
+  __shared__ double tile[TJ][K + 1];
   __shared__ double red[4 * K];
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
-  const size_t total = (size_t)d.n_red * K;
-  const int c = threadIdx.x % K;  // fixed per thread: the stride is a multiple of K
+  const size_t n_red = (size_t)d.n_red;
+  const size_t ntiles = (n_red + TJ - 1) / TJ;
+  const int c = threadIdx.x % K;  // the column of every element this thread writes
   double dot = 0.0;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const int j = (int)(e / K);
-    double sv = v[(size_t)c * n_full + d.red2full[j]];
-    if (d.et_ptr[j + 1] > d.et_ptr[j]) sv += extra[e];
-    y[e] = sv;
-    if (partial) dot += x[e] * sv;
+  for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+    const size_t j0 = tb * TJ;
+    __syncthreads();  // the previous tile's rows are written
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = threadIdx.x + 256 * q;
+      const int cc = k / TJ, ji = k % TJ;
+      const size_t j = j0 + ji;
+      if (j < n_red) tile[ji][cc] = v[(size_t)cc * n_full + d.red2full[j]];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int ji = (threadIdx.x + 256 * q) / K;
+      const size_t j = j0 + ji;
+      if (j < n_red) {
+        const size_t e = j * K + c;
+        double sv = tile[ji][c];
+        if (d.et_ptr[j + 1] > d.et_ptr[j]) sv += extra[e];
+        y[e] = sv;
+        if (partial) dot += x[e] * sv;
+      }
+    }
   }
   if (partial) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
