@@ -844,7 +844,8 @@ static int mfk_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const doubl
                        int* nparts, hipStream_t s) {
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   constexpr int PP = P1 * P1 * P1;
-  hipLaunchKernelGGL((k_mfk_gather<K>), dim3(grid1d(n_full * K)), dim3(256), 0, s, d, x, b.u);
+  // One block per 1024-element tile: no partials, so no cap on the grid.
+  hipLaunchKernelGGL((k_mfk_gather<K>), dim3(grid_items(n_full * K, 1024)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0)
     hipLaunchKernelGGL((k_mfk_gather_small<K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
   tensor_launch<P1>(d, b.u, b.v, K, s);
