@@ -694,11 +694,29 @@ template <int K>
 __global__ __launch_bounds__(256) void k_preck_in(int n, const int* __restrict__ red2full, const double* __restrict__ r,
                                                   const double* __restrict__ diag, float* __restrict__ u0,
                                                   size_t vox) {
-  const size_t total = (size_t)n * K;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t j = e / K;
-    const int c = (int)(e % K);
-    u0[(size_t)c * vox + red2full[j]] = (float)(sym_scale(diag[j]) * r[e]);
+  // Through an LDS tile of 1024 / K unknowns x K columns: the reads of r are
+  // whole K-column rows, the writes runs of one column's grid.
+  constexpr int TJ = 1024 / K;
+  __shared__ float tile[K][TJ + 1];
+  const size_t ntiles = ((size_t)n + TJ - 1) / TJ;
+  for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+    const size_t j0 = tb * TJ;
+    __syncthreads();  // the previous tile's runs are written
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = threadIdx.x + 256 * q;
+      const int ji = k / K, c = k % K;
+      const size_t j = j0 + ji;
+      if (j < (size_t)n) tile[c][ji] = (float)(sym_scale(diag[j]) * r[j * K + c]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = threadIdx.x + 256 * q;
+      const int c = k / TJ, ji = k % TJ;
+      const size_t j = j0 + ji;
+      if (j < (size_t)n) u0[(size_t)c * vox + red2full[j]] = tile[c][ji];
+    }
   }
 }
 
@@ -809,7 +827,9 @@ static int ml_prec_k(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double*
   const int L = ml.levels;
   size_t vox[CC_ML_MAX_LEVELS];
   for (int l = 0; l < L; ++l) vox[l] = ml_vox(ml.net[l]);
-  hipLaunchKernelGGL((k_preck_in<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, mb.u[0], vox[0]);
+  // One block per 1024-element tile (no partials).
+  hipLaunchKernelGGL((k_preck_in<K>), dim3((unsigned)(((size_t)n * K + 1023) / 1024)), dim3(256), 0, s, n, f.red2full,
+                     r, diag, mb.u[0], vox[0]);
   MgDims g[CC_ML_MAX_LEVELS];
   for (int l = 1; l < L; ++l) {
     mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
