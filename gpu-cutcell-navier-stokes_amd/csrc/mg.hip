@@ -115,15 +115,33 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgDims g, const float* __res
     const int fx = (int)(v % nfx), fy = (int)((v / nfx) % nfy), fz = (int)(v / ((long long)nfx * nfy));
     int cx[2], cy[2], cz[2];
     float wx[2], wy[2], wz[2];
-    const int nx = pro_src(fx, g.a[0], cx, wx);
-    const int ny = pro_src(fy, g.a[1], cy, wy);
-    const int nz = pro_src(fz, g.a[2], cz, wz);
+    // A one-source axis repeats its source at weight 0, so the 8 loads are
+    // independent and issued together; adding 0 * c leaves every sum as the
+    // taps-only loop computes it.
+    if (pro_src(fx, g.a[0], cx, wx) == 1) {
+      cx[1] = cx[0];
+      wx[1] = 0.0f;
+    }
+    if (pro_src(fy, g.a[1], cy, wy) == 1) {
+      cy[1] = cy[0];
+      wy[1] = 0.0f;
+    }
+    if (pro_src(fz, g.a[2], cz, wz) == 1) {
+      cz[1] = cz[0];
+      wz[1] = 0.0f;
+    }
+    float cv[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = t >> 2, j = (t >> 1) & 1, i = t & 1;
+      cv[t] = coarse[(((size_t)cz[k] * g.a[1].nc + cy[j]) * g.a[0].nc + cx[i]) * cs];
+    }
     float acc = add ? add[(size_t)v * as] : 0.0f;
-    for (int k = 0; k < nz; ++k)
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i)
-          acc += (wz[k] * wy[j] * wx[i]) *
-                 coarse[(((size_t)cz[k] * g.a[1].nc + cy[j]) * g.a[0].nc + cx[i]) * cs];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int k = t >> 2, j = (t >> 1) & 1, i = t & 1;
+      acc += (wz[k] * wy[j] * wx[i]) * cv[t];
+    }
     fine[v] = acc;
   }
 }
