@@ -79,23 +79,37 @@ __global__ __launch_bounds__(256) void k_mg_restrict(MgDims g, const float* __re
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
        v += (long long)gridDim.x * blockDim.x) {
     const int cx = (int)(v % ncx), cy = (int)((v / ncx) % ncy), cz = (int)(v / ((long long)ncx * ncy));
-    float acc = 0.0f;
-    for (int dz = -1; dz <= 1; ++dz) {
-      int fz;
-      float wz;
-      if (!rest_tap(cz, dz, g.a[2], &fz, &wz)) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        int fy;
-        float wy;
-        if (!rest_tap(cy, dy, g.a[1], &fy, &wy)) continue;
-        const size_t rowoff = ((size_t)fz * g.a[1].nf + fy) * g.a[0].nf;
+    // Missing taps read the (always present) centre point at weight 0, so
+    // every row's loads are independent and issued together; adding 0 * f
+    // leaves the sum as the taps-only loop computes it.
+    int fx[3], fy[3], fz[3];
+    float wx[3], wy[3], wz[3];
 #pragma unroll
-        for (int dx = -1; dx <= 1; ++dx) {
-          int fx;
-          float wx;
-          if (!rest_tap(cx, dx, g.a[0], &fx, &wx)) continue;
-          acc += (wz * wy * wx) * fine[(rowoff + fx) * fs];
-        }
+    for (int d = 0; d < 3; ++d) {
+      if (!rest_tap(cx, d - 1, g.a[0], &fx[d], &wx[d])) {
+        fx[d] = 2 * cx;
+        wx[d] = 0.0f;
+      }
+      if (!rest_tap(cy, d - 1, g.a[1], &fy[d], &wy[d])) {
+        fy[d] = 2 * cy;
+        wy[d] = 0.0f;
+      }
+      if (!rest_tap(cz, d - 1, g.a[2], &fz[d], &wz[d])) {
+        fz[d] = 2 * cz;
+        wz[d] = 0.0f;
+      }
+    }
+    float acc = 0.0f;
+#pragma unroll
+    for (int kz = 0; kz < 3; ++kz) {
+#pragma unroll
+      for (int ky = 0; ky < 3; ++ky) {
+        const size_t rowoff = ((size_t)fz[kz] * g.a[1].nf + fy[ky]) * g.a[0].nf;
+        float fv[3];
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) fv[kx] = fine[(rowoff + fx[kx]) * fs];
+#pragma unroll
+        for (int kx = 0; kx < 3; ++kx) acc += (wz[kz] * wy[ky] * wx[kx]) * fv[kx];
       }
     }
     coarse[(size_t)v * cs] = acc;
