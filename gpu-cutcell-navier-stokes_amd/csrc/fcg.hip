@@ -727,16 +727,37 @@ __global__ __launch_bounds__(256) void k_preck_out(int n, const int* __restrict_
                                                    const double* __restrict__ diag, const float* __restrict__ y,
                                                    int ys, size_t ybs, double* __restrict__ z,
                                                    const double* __restrict__ q, double* __restrict__ part) {
+  // Through an LDS tile of 1024 / K unknowns x K columns: the reads of y are
+  // runs of one column's grid, the rows of r, q and z whole K-column rows.
+  constexpr int TJ = 1024 / K;
+  __shared__ float tile[TJ][K + 1];
   __shared__ double red[4 * K];
-  const int c = threadIdx.x % K;
+  const int c = threadIdx.x % K;  // the column of every element this thread writes
+  const size_t ntiles = ((size_t)n + TJ - 1) / TJ;
   double zq = 0.0;
-  const size_t total = (size_t)n * K;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t j = e / K;
-    const double d = diag[j];
-    const double zz = jacobi_part(r[e], d) + sym_scale(d) * (double)y[(size_t)c * ybs + (size_t)red2full[j] * ys];
-    z[e] = zz;
-    if (q) zq += zz * q[e];
+  for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+    const size_t j0 = tb * TJ;
+    __syncthreads();  // the previous tile's rows are done
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int k = threadIdx.x + 256 * qq;
+      const int cc = k / TJ, ji = k % TJ;
+      const size_t j = j0 + ji;
+      if (j < (size_t)n) tile[ji][cc] = y[(size_t)cc * ybs + (size_t)red2full[j] * ys];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int ji = (threadIdx.x + 256 * qq) / K;
+      const size_t j = j0 + ji;
+      if (j < (size_t)n) {
+        const size_t e = j * K + c;
+        const double d = diag[j];
+        const double zz = jacobi_part(r[e], d) + sym_scale(d) * (double)tile[ji][c];
+        z[e] = zz;
+        if (q) zq += zz * q[e];
+      }
+    }
   }
   if (part) block_cols<K>(zq, part + (size_t)blockIdx.x * K, red);
 }
