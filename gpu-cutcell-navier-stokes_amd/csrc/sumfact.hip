@@ -90,12 +90,22 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const do
   __shared__ double rd[W][kSy][kSx];
   const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
   const size_t plane = (size_t)nbx * nby;
-  const size_t col_off = (size_t)(blockIdx.z / nzc) * plane * nbz;
+  // Blocks go to the 8 XCDs round-robin by linear id; each XCD is given one
+  // contiguous range of the (x, y, z-run, column) tile space instead, so
+  // that adjacent columns, whose halos overlap, share that XCD's L2.
+  const unsigned gx = gridDim.x, gy = gridDim.y;
+  const unsigned total = gx * gy * gridDim.z;
+  const unsigned lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  constexpr unsigned kX = 8;  // XCDs of the part
+  const unsigned xq = total / kX, xr = total % kX, xcd = lin % kX;
+  const unsigned tile = xcd * xq + min(xcd, xr) + lin / kX;
+  const int bx = (int)(tile % gx), by = (int)((tile / gx) % gy), bz = (int)(tile / (gx * gy));
+  const size_t col_off = (size_t)(bz / nzc) * plane * nbz;
   const double* __restrict__ u = u_all + col_off;
   double* __restrict__ v = v_all + col_off;
-  const int zs = (blockIdx.z % nzc) * zc;
+  const int zs = (bz % nzc) * zc;
   const int ze = min(nbz, zs + zc);
-  const int x0 = blockIdx.x * kSx, y0 = blockIdx.y * kSy;
+  const int x0 = bx * kSx, y0 = by * kSy;
   const int tid = threadIdx.x;
   const int tx = tid % kSx, ty = tid / kSx;
   const double cm = d.mass_coef;
