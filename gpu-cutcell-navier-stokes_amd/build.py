@@ -29,16 +29,12 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found (ROCm is required to build the HIP library)")
 
 
-def build(verbose: bool = False) -> Path:
-    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
-    tmp = OUT.with_suffix(".so.tmp")
-    cmd = [
-        hipcc(),
+def _flags() -> list[str]:
+    return [
         f"--offload-arch={ARCH}",
         "-O3",
         "-std=c++17",
         "-fPIC",
-        "-shared",
         "-Xclang",
         "-target-feature",
         "-Xclang",
@@ -46,17 +42,45 @@ def build(verbose: bool = False) -> Path:
         "-Wno-unused-command-line-argument",
         f"-I{INCLUDE}",
         f"-I{CSRC}",
-        *[str(s) for s in srcs],
-        "-o",
-        str(tmp),
     ]
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every source to an object in parallel (one hipcc per file),
+    then link the shared library. Objects go to a scratch folder beside the
+    sources that is removed afterwards."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    srcs = [CSRC / s for s in SOURCES if (CSRC / s).exists()]
+    objdir = PKG / "_build"
+    objdir.mkdir(exist_ok=True)
+    cc = hipcc()
+
+    def compile_one(src: Path):
+        obj = objdir / (src.stem + ".o")
+        cmd = [cc, *_flags(), "-c", str(src), "-o", str(obj)]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        return obj, subprocess.run(cmd, capture_output=True, text=True)
+
+    jobs = jobs or min(len(srcs), os.cpu_count() or 1, 16)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(compile_one, srcs))
+    failed = [r for _, r in results if r.returncode != 0]
+    if failed:
+        for r in failed:
+            sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed on {len(failed)} source(s)")
+    tmp = OUT.with_suffix(".so.tmp")
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *[str(o) for o, _ in results], "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed with status {res.returncode}")
+        raise RuntimeError(f"hipcc link failed with status {res.returncode}")
     os.replace(tmp, OUT)
+    shutil.rmtree(objdir, ignore_errors=True)
     return OUT
 
 

@@ -136,6 +136,22 @@ __global__ __launch_bounds__(NT) void k_batched_pcg(BatchArgs a) {
     if (w >= a.nsys) return;
     const int sid = a.order[w];
     const int n = a.sys_n[sid];
+    if (n < 0 || n > RPT * NT) {
+      // The caller's n_max was below this system's size: the launch shape
+      // holds only RPT * NT rows per system in registers and LDS. Report
+      // the system as a bad argument and leave its solution untouched.
+      if (threadIdx.x == 0) {
+        cc_pcg_result res;
+        res.iterations = 0;
+        res.status = CC_E_ARG;
+        res.outer = 0;
+        res.inner_total = 0;
+        res.recursive_rel = 0.0;
+        res.true_rel = 0.0;
+        a.results[sid] = res;
+      }
+      continue;
+    }
     const int* ip = a.indptr + a.sys_ip_off[sid];
     const int* ix = a.indices + a.sys_nnz_off[sid];
     const double* va = a.vals + a.sys_nnz_off[sid];

@@ -213,7 +213,10 @@ int cc_pcg_jacobi_mixed_rb(int32_t n, int32_t k, const int32_t* indptr, const in
  * column indices (local) and values at sys_nnz_off[s], and its right-hand side
  * and solution at sys_vec_off[s]. All per-system arrays and order (the
  * dispatch order, longest first) are device memory. n_max <= 8192 (the
- * workgroup keeps p and Ap of its system in LDS). precision 0 = float64,
+ * workgroup keeps p and Ap of its system in LDS) and must be at least
+ * max(sys_n): the launch shape is chosen from n_max, and a system with more
+ * rows than that shape holds gets status CC_E_ARG (-1) in its result, with
+ * its solution left untouched. precision 0 = float64,
  * 1 = float32 inner / float64 outer (vals32 required). block_threads in
  * {256, 512, 1024} and team_lanes in {4, 8, 16, 32, 64} come from the
  * autotuner; grid <= 0 means one workgroup per system. counter is one device

@@ -17,3 +17,15 @@ for p in (str(PKG), str(ROOT)):
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an AMD Instinct GPU (MI355X, gfx950)")
+
+
+def pytest_terminal_summary(terminalreporter):
+    """Distribution of the stop gaps that parity.check_trajectory accepted."""
+    try:
+        import parity
+    except ImportError:
+        return
+    if parity.GAPS:
+        hist = [parity.GAPS.count(g) for g in range(max(parity.GAPS) + 1)]
+        terminalreporter.write_line(f"parity stop gaps (count of |gap| = 0, 1, ...): {hist}, "
+                                    f"largest {max(parity.GAPS)} of {len(parity.GAPS)} trajectories")

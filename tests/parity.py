@@ -18,13 +18,18 @@ import numpy as np
 
 MAX_ITER_GAP = 5  # 4 seen: batched 8-lane-team sums vs the oracle, a p=5 oblique system
 ENVELOPE_FACTOR = 2.0
+REFERENCE_ITER_GAP = 3  # provisional: rounding alone moves the stop by 3 (permuted oracle runs)
 STOP_SLACK = ENVELOPE_FACTOR
+
+
+GAPS: list = []  # every stop gap checked in this session (printed at the end)
 
 
 def check_trajectory(h_a, it_a, h_b, it_b, tol):
     """Assert two residual histories (index = iteration) describe the same solve."""
     h_a = np.asarray(h_a, dtype=float)
     h_b = np.asarray(h_b, dtype=float)
+    GAPS.append(abs(it_a - it_b))
     assert abs(it_a - it_b) <= MAX_ITER_GAP, (it_a, it_b)
     m = min(it_a, it_b)
     e_a = np.minimum.accumulate(h_a[: m + 1])

@@ -91,3 +91,32 @@ def test_gpu_backend_without_gpu_reports_unavailable():
     assert x is None and not rep.success and rep.message.startswith("HIP backend unavailable")
     x, rep = solve_spd(A, np.ones(3), backend="auto", tol=1e-10, maxiter=20)
     assert rep.backend == "cpu" and rep.success and rep.message.startswith("GPU skipped/fell back")
+
+
+def test_cpu_gmres_follows_scipy():
+    """solve_cpu_gmres (code/fast_solver_backends.py:135-191): restarted SciPy
+    GMRES with the Jacobi preconditioner, iterations counted by the pr_norm
+    callback, the reference's success rule; on a nonsymmetric
+    convection-diffusion system its count and solution equal a direct SciPy
+    call's, and it reports failure when maxiter is too small."""
+    import numpy as np
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from fast_solver_backends import solve_cpu_gmres
+
+    n = 300
+    A = sp.diags([-1.3 * np.ones(n - 1), 2.5 * np.ones(n), -0.7 * np.ones(n - 1)], [-1, 0, 1], format="csr")
+    b = np.random.default_rng(0).standard_normal(n)
+    x, rep = solve_cpu_gmres(A, b, tol=1e-10, restart=20)
+    count = [0]
+    inv = 1.0 / A.diagonal()
+    M = spla.LinearOperator(A.shape, matvec=lambda r: inv * r)
+    xs, info = spla.gmres(A, b, rtol=1e-10, atol=0.0, restart=20, maxiter=5000, M=M,
+                          callback=lambda _: count.__setitem__(0, count[0] + 1), callback_type="pr_norm")
+    assert info == 0 and rep.success and rep.method == "scipy.sparse.linalg.gmres"
+    assert rep.iterations == count[0]
+    np.testing.assert_array_equal(x, xs)
+    assert rep.relative_residual <= 1e-9
+    _, rep2 = solve_cpu_gmres(A, b, tol=1e-12, restart=5, maxiter=1)
+    assert not rep2.success and rep2.message.startswith("not converged")

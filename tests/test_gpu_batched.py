@@ -88,3 +88,27 @@ def test_batched_edge_cases():
     np.testing.assert_allclose(out.X[2], [0.5, 1.0 / 3.0], rtol=1e-15)
     with pytest.raises(ValueError):
         BatchedSystems([sp.eye(9000, format="csr")])
+
+
+def test_batched_rejects_a_system_beyond_n_max():
+    """A direct caller passing n_max below a system's size: the launch shape
+    is chosen from n_max (256 threads, one row each), the 300-row system gets
+    status CC_E_ARG (-1) and an untouched solution, the others still solve."""
+    import scipy.sparse as sp
+    import torch
+
+    from cutcell.batched import BatchedSystems
+
+    mats = [sp.diags(np.full(300, 2.0), format="csr"), sp.diags([2.0, 4.0], format="csr")]
+    bs = BatchedSystems(mats)
+    bs.n_max = 100  # what a careless direct ABI caller might pass
+    B = bs.concat([np.ones(300), np.ones(2)])
+    X = torch.full_like(B, 7.0)
+    bs.solve_device(B, X, tol=1e-12, block_threads=256, team_lanes=8)
+    torch.cuda.synchronize()
+    r_big, r_small = bs.read_results()
+    assert r_big.status == -1 and r_big.iterations == 0
+    assert r_small.status == 1
+    Xs = bs.split(X)
+    assert np.all(Xs[0] == 7.0)
+    np.testing.assert_allclose(Xs[1], [0.5, 0.25], rtol=1e-15)

@@ -283,3 +283,120 @@ def test_flip_is_the_adjoint_layer():
     a, _ = cnn_ref.conv3(x, W, one)
     b, _ = cnn_ref.conv3(y, cnn_ref.flip(W), one)
     assert abs(np.sum(a * y) - np.sum(x * b)) <= 1e-10 * np.sum(np.abs(a * y))
+
+
+def test_prolongation_reproduces_linear_fields_and_keeps_constants():
+    """prolong_1d puts fine point f at coordinate f and coarse point c at 2c.
+    Linear interpolation then reproduces a linear field at every fine point
+    that has a coarse point on each side; on a clamped axis of even length the
+    last fine point copies the last coarse point; on a periodic axis the last
+    fine point averages the last coarse point and coarse point 0 taken at
+    coordinate n (the wrap). Every row sums to one."""
+    from oracle import cnn_ref
+
+    for n in (5, 8, 9, 12):
+        P = cnn_ref.prolong_1d(n, False)
+        nc = P.shape[1]
+        assert nc == (n + 1) // 2
+        np.testing.assert_array_equal(P.sum(axis=1), np.ones(n))
+        f = P @ (2.0 * np.arange(nc))
+        last = n if n % 2 else n - 1  # fine points with coarse points on both sides
+        np.testing.assert_array_equal(f[:last], np.arange(last, dtype=float))
+        if n % 2 == 0:
+            assert f[-1] == 2.0 * (nc - 1)
+    for n in (6, 8, 12):
+        P = cnn_ref.prolong_1d(n, True)
+        nc = P.shape[1]
+        assert nc == n // 2
+        np.testing.assert_array_equal(P.sum(axis=1), np.ones(n))
+        f = P @ (2.0 * np.arange(nc))
+        np.testing.assert_array_equal(f[: n - 1], np.arange(n - 1, dtype=float))
+        wrapped = 2.0 * np.arange(nc)
+        wrapped[0] = n
+        assert (P @ wrapped)[n - 1] == n - 1
+
+
+def test_prolongation_3d_is_the_tensor_product_in_zyx_order():
+    """prolong_3d on grids raveled [z, y, x] with dims (nx, ny, nz): a field
+    linear in x, y and z with distinct slopes, sampled on the coarse grid at
+    (2i, 2j, 2k), comes back exactly at every fine point (odd lengths, clamped),
+    so a transposed axis order fails; periodic axes keep constants."""
+    from oracle import cnn_ref
+
+    dims = (5, 7, 9)
+    P = cnn_ref.prolong_3d(dims, (False, False, False))
+    nc = [(n + 1) // 2 for n in dims]
+    Zc, Yc, Xc = np.meshgrid(*(2.0 * np.arange(m) for m in nc[::-1]), indexing="ij")
+    Zf, Yf, Xf = np.meshgrid(*(np.arange(m, dtype=float) for m in dims[::-1]), indexing="ij")
+    fc = 1.0 + 2.0 * Xc + 3.0 * Yc + 5.0 * Zc
+    ff = 1.0 + 2.0 * Xf + 3.0 * Yf + 5.0 * Zf
+    np.testing.assert_array_equal(P @ fc.ravel(), ff.ravel())
+    Pp = cnn_ref.prolong_3d((8, 5, 6), (True, False, True))
+    np.testing.assert_array_equal(Pp.sum(axis=1), np.ones(8 * 5 * 6))
+
+
+def test_periodic_shift_and_conv_land_at_the_wrapped_index():
+    """shift(a, dz, dy, dx)[z, y, x] = a[z + dz, y + dy, x + dx]: a delta at
+    the origin lands at the hand-computed wrapped index on a periodic axis and
+    vanishes on a clamped one; a one-tap conv on the same delta does the same
+    (periodic tuples are (x, y, z))."""
+    from oracle import cnn_ref
+
+    a = np.zeros((4, 5, 6))
+    a[0, 0, 0] = 1.0
+    cases = [((1, 0, 0), (False, False, True), (3, 0, 0)),
+             ((0, 0, 1), (True, False, False), (0, 0, 5)),
+             ((0, -1, 0), (False, True, False), (0, 1, 0)),
+             ((-1, 1, -1), (True, True, True), (1, 4, 1))]
+    for (dz, dy, dx), per, where in cases:
+        out = cnn_ref.shift(a, dz, dy, dx, per)
+        expect = np.zeros_like(a)
+        expect[where] = 1.0
+        np.testing.assert_array_equal(out, expect)
+        # the same shift on a clamped grid moves the delta off the grid
+        if (dz, dy, dx) != (0, -1, 0):
+            assert not cnn_ref.shift(a, dz, dy, dx, (False, False, False)).any()
+        x = np.zeros((4, 5, 6, 16))
+        x[0, 0, 0, 3] = 1.0
+        W = np.zeros((27, 16, 16))
+        W[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1), 3, 7] = 2.5
+        y, _ = cnn_ref.conv3(x, W, np.ones((4, 5, 6)), periodic=per)
+        expect = np.zeros((4, 5, 6, 16))
+        expect[where + (7,)] = 2.5
+        np.testing.assert_array_equal(y, expect)
+
+
+def test_multilevel_network_reduces_to_its_parts():
+    """ml_network with one level equals sym_network; with all weights zero it
+    is D^-1 r; with coarse levels it stays symmetric with M - D^-1 positive
+    semidefinite (clamped and periodic grids)."""
+    from oracle import cnn_ref
+
+    rng = np.random.default_rng(4)
+    dims = (6, 5, 4)  # (nx, ny, nz)
+    shape = dims[::-1]
+    mask = (rng.random(shape) > 0.15).astype(float)
+    d = (0.5 + rng.random(shape)) * mask
+    W0 = 0.3 * rng.standard_normal((27, 16, 16))
+    W0[:, 1:, :] = 0.0
+    W1 = 0.2 * rng.standard_normal((27, 16, 16))
+    r = rng.standard_normal(shape) * mask
+    z1 = cnn_ref.ml_network(r, d, [(dims, W0, W1, mask)])
+    z2, _ = cnn_ref.sym_network(r, d, W0, W1, mask)
+    np.testing.assert_allclose(z1, z2, rtol=0, atol=1e-13 * np.abs(z2).max())
+    zero = np.zeros((27, 16, 16))
+    dinv = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    for per in ((False, False, False), (True, False, False)):
+        dims_c = tuple(cnn_ref.coarse_len(n, p) for n, p in zip(dims, per))
+        mask_c = np.ones(dims_c[::-1])
+        z0 = cnn_ref.ml_network(r, d, [(dims, zero, zero, mask), (dims_c, zero, zero, mask_c)], per)
+        np.testing.assert_allclose(z0, dinv * r, rtol=0, atol=1e-15)
+        levels = [(dims, W0, W1, mask), (dims_c, 0.5 * W0, W1, mask_c)]
+        a = rng.standard_normal(shape) * mask
+        b = rng.standard_normal(shape) * mask
+        Ma = cnn_ref.ml_network(a, d, levels, per)
+        Mb = cnn_ref.ml_network(b, d, levels, per)
+        assert abs(np.sum(Ma * b) - np.sum(a * Mb)) <= 1e-10 * np.sum(np.abs(Ma * b))
+        assert np.sum(Ma * a) >= np.sum(dinv * a * a) * (1.0 - 1e-12)
+        # the coarse level changes the result (it is not silently dropped)
+        assert np.abs(Ma - cnn_ref.ml_network(a, d, levels[:1], per)).max() > 1e-6
