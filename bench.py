@@ -22,8 +22,9 @@ float64 true residual <= 1e-8.
 Also reported: the dominant kernel's roofline (algorithmic bytes per SpMM
 launch over its average duration measured with HIP events on the launch
 stream during the timed region), the per-method error against a tight
-float64 solve, and a CPU baseline (the oracle's scalar C port of the
-reference loop, one core, one right-hand side).
+float64 solve, and the reference's CPU solver (SciPy CG with the Jacobi
+preconditioner, as ``solve_cpu_cg``) timed on the host: one solve alone on
+one core, and the step's 16 right-hand sides over one process per core.
 
 Usage: ``python bench.py [--gpus 1] [--steps K] [--warmup W]``.
 """
@@ -209,32 +210,96 @@ def spmv_bandwidth(hip, torch, Ad, reps: int = 20):
     return out
 
 
-def cpu_baseline(system, b: np.ndarray, budget_s: float = 30.0):
-    """Oracle scalar C port of the reference loop, one core, one right-hand side."""
-    import oracle
+_CPU_JOB = {}
 
+
+def _cpu_solve(j: int):
+    """One reference CPU solve (``solve_cpu_cg``: SciPy ``cg``, rtol 1e-8,
+    atol 0, Jacobi ``LinearOperator``; ``code/fast_solver_backends.py:90-132``)
+    of job ``j`` in a worker whose BLAS runs one thread."""
+    from threadpoolctl import threadpool_limits
+
+    from fast_solver_backends import solve_cpu_cg
+
+    A, b = _CPU_JOB["systems"](j)
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        _, rep = solve_cpu_cg(A, b, tol=TOL, maxiter=20000)
+        dt = time.perf_counter() - t0
+    return dt, rep.iterations, rep.success
+
+
+def cpu_host() -> dict:
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_reference(systems, n_jobs: int, max_workers: int = 16) -> dict:
+    """The reference's CPU solver timed on this host, as the reference runs
+    it (``code/fast_solver_workbench.py:41-49, 201-217``): one solve per
+    process with BLAS threads at 1, first alone on one core (job 0), then
+    every job over ``min(nproc, max_workers)`` forked worker processes
+    (solves/s = jobs / wall time). ``systems(j)`` returns job j's
+    ``(A, b)``. Run before the GPU is initialised (the workers are forks)."""
+    from concurrent.futures import ProcessPoolExecutor
+    import multiprocessing as mp
+
+    _CPU_JOB["systems"] = systems
+    t1, it1, ok1 = _cpu_solve(0)
+    workers = max(1, min(os.cpu_count() or 1, max_workers, n_jobs))
     t0 = time.perf_counter()
-    x, it, st, rel = oracle.pcg_gpu_loop(system.A, b, tol=TOL, maxiter=20000)
-    dt = time.perf_counter() - t0
-    return dict(value=1.0 / dt, unit="solves/s", cores=1, kind="port",
-                sample=f"1 of the {NRHS} right-hand sides, full system, float64 Jacobi-PCG to 1e-8 "
-                       f"({it} iterations, {dt:.2f} s, oracle/pcg_ref.c scalar loop)",
-                iterations=it, converged=st == oracle.CONVERGED)
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("fork")) as ex:
+        res = list(ex.map(_cpu_solve, range(n_jobs)))
+    wall = time.perf_counter() - t0
+    its = [r[1] for r in res]
+    return {
+        "single_core": {"solves_per_s": sig(1.0 / t1), "s_per_solve": sig(t1), "iterations": it1, "converged": ok1},
+        "all_core": {"solves_per_s": sig(n_jobs / wall), "wall_s": sig(wall), "processes": workers,
+                     "jobs": n_jobs, "iterations_range": [min(its), max(its)],
+                     "converged": all(r[2] for r in res)},
+    }
 
 
-def batched_section(hip, torch, steps: int, warmup: int, seed: int = 0):
+def cpu_baseline(system, B: np.ndarray) -> dict:
+    """C2 on the host: SciPy CG on each of the 16 right-hand sides, one per
+    process (BLAS at one thread), plus one solve alone on one core."""
+    A = system.A.tocsr()
+    ref = cpu_reference(lambda j: (A, np.ascontiguousarray(B[:, j])), B.shape[1])
+    host = cpu_host()
+    allc = ref["all_core"]
+    return dict(value=allc["solves_per_s"], unit="solves/s", cores=allc["processes"], kind="port",
+                sample=f"the {B.shape[1]} right-hand sides of one step on the full system, one SciPy CG solve "
+                       f"per process, {allc['processes']} processes with one BLAS thread each "
+                       f"({allc['wall_s']} s wall); single core: one of them alone "
+                       f"({ref['single_core']['s_per_solve']} s)",
+                solver="fast_solver_backends.solve_cpu_cg = scipy.sparse.linalg.cg(rtol=1e-8, atol=0, "
+                       "M=Jacobi LinearOperator), the reference's CPU solver semantics",
+                single_core=ref["single_core"], all_core=allc, **host)
+
+
+def batched_section(hip, torch, steps: int, warmup: int, seed: int = 0, systems=None, cpu_ref=None):
     """Batched uneven 2D systems in one persistent launch (reported beside the headline)."""
     from cutcell.batched import BatchedSystems
     from cutcell.workloads import c3_batch
 
     t0 = time.perf_counter()
-    systems = c3_batch(seed=seed)
+    if systems is None:
+        systems = c3_batch(seed=seed)
     build_s = time.perf_counter() - t0
     bs = BatchedSystems([s.A for s in systems])
     B = bs.concat([s.b for s in systems])
     X = torch.empty_like(B)
     out = {"systems": bs.nsys, "total_unknowns": bs.total_n, "total_nnz": bs.total_nnz, "n_max": bs.n_max,
            "build_s": sig(build_s)}
+    if cpu_ref is not None:
+        out["cpu_reference"] = cpu_ref
     for method in ("fp64", "mixed"):
         cfg = bs.solve_device(B, X, tol=TOL, maxiter=20000, precision=method)  # tunes on first use
         for _ in range(warmup):
@@ -599,6 +664,35 @@ def main(argv=None) -> int:
               "independent, and multi-GPU sharding is not part of this build (--gpus 1 only).", file=sys.stderr)
         return 2
 
+    # CPU legs first: their worker processes are forks, made before the GPU
+    # is initialised in this process.
+    cpu, c3_systems, c3_cpu, c3_build_s = None, None, None, None
+    if not args.operator_probe:
+        log("building the 3D planar-cut system (p=3, ~81k unknowns) ...")
+        t0 = time.perf_counter()
+        system = build_workload(args.seed, args.target)
+        log(f"system: n={system.A.shape[0]} nnz={system.A.nnz} N={system.meta['n_elements']} "
+            f"({time.perf_counter() - t0:.1f} s)")
+        rng = np.random.default_rng(1000 + args.seed)
+        B = rng.standard_normal((system.A.shape[0], NRHS))
+        if not args.no_cpu_baseline:
+            log("CPU baseline: SciPy CG (the reference's CPU solver), one process per right-hand side ...")
+            cpu = cpu_baseline(system, B)
+            log(f"cpu: {cpu['value']} solves/s over {cpu['cores']} processes; single core "
+                f"{cpu['single_core']['solves_per_s']} solves/s ({cpu['single_core']['iterations']} it), "
+                f"{cpu['cpu_model']}, nproc {cpu['nproc']}")
+        if not args.no_batched:
+            from cutcell.workloads import c3_batch
+
+            t0 = time.perf_counter()
+            c3_systems = c3_batch(seed=args.seed)
+            c3_build_s = time.perf_counter() - t0
+            if not args.no_cpu_baseline:
+                log(f"CPU reference over the {len(c3_systems)} batched 2D systems ...")
+                c3_cpu = cpu_reference(lambda j: (c3_systems[j].A, c3_systems[j].b), len(c3_systems))
+                c3_cpu.update(cpu_host())
+                log(f"cpu (batched 2D): {c3_cpu['all_core']}")
+
     import torch
 
     from cutcell import hip
@@ -611,14 +705,8 @@ def main(argv=None) -> int:
         print(json.dumps(operator_probe(hip, torch, args)))
         return 0
 
-    log("building the 3D planar-cut system (p=3, ~81k unknowns) ...")
-    t0 = time.perf_counter()
-    system = build_workload(args.seed, args.target)
     A = system.A
     n, nnz = A.shape[0], A.nnz
-    log(f"system: n={n} nnz={nnz} N={system.meta['n_elements']} ({time.perf_counter() - t0:.1f} s)")
-    rng = np.random.default_rng(1000 + args.seed)
-    B = rng.standard_normal((n, NRHS))
 
     Ad = hip.DeviceCSR.from_scipy(A, with_f32=True)
     Bd = torch.from_numpy(B).to("cuda")
@@ -682,7 +770,8 @@ def main(argv=None) -> int:
                     bytes_per_launch=traffic_alg, avg_us=round(head["spmm_avg_us"], 2),
                     avg_us_back_to_back=round(head["spmm_back_to_back_us"], 2),
                     csr_bytes_per_launch=csr_alg,
-                    csr_equivalent_GBps=round(csr_alg / (head["spmm_avg_us"] * 1e-6) / 1e9, 1))
+                    csr_equivalent_GBps=round(csr_alg / (head["spmm_avg_us"] * 1e-6) / 1e9, 1),
+                    frac_b_spmm=round(csr_alg / (head["spmm_avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
 
     log("single-vector SpMV bandwidth on the workload matrix ...")
     spmv = spmv_bandwidth(hip, torch, Ad)
@@ -691,7 +780,8 @@ def main(argv=None) -> int:
     batched = None
     if not args.no_batched:
         log("batched section: 241 uneven 2D systems in one persistent launch ...")
-        batched = batched_section(hip, torch, max(1, args.steps), args.warmup)
+        batched = batched_section(hip, torch, max(1, args.steps), args.warmup, systems=c3_systems, cpu_ref=c3_cpu)
+        batched["build_s"] = sig(c3_build_s)
 
     learned = None
     if not args.no_learned:
@@ -702,12 +792,6 @@ def main(argv=None) -> int:
     if not args.no_pressure:
         log("pressure-projection section: bump channel, matrix-free E^T K E ...")
         pressure = pressure_section(hip, torch, args)
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("CPU baseline (oracle C port, 1 core, 1 RHS) ...")
-        cpu = cpu_baseline(system, B[:, 0])
-        log(f"cpu: {cpu['value']:.4f} solves/s ({cpu['iterations']} it)")
 
     out = {
         "metric": "PCG solves/s to 1e-8 rel. residual, learned vs Jacobi; SpMV HBM TB/s",

@@ -16,9 +16,16 @@ can sit well above the threshold; 1.8x was seen for batched solves of the
 
 import numpy as np
 
-MAX_ITER_GAP = 5  # 4 seen: batched 8-lane-team sums vs the oracle, a p=5 oblique system
+# Measured on the GPU suite: the largest stop gap between a HIP solve and the
+# oracle (or between two HIP solves of one system that sum in different
+# orders) is 3, over 77 trajectories; 33 of them stop on the same iteration.
+# Permuting one stress system in the FP64 oracle alone moves its stop by 3.
+MAX_ITER_GAP = 3
 ENVELOPE_FACTOR = 2.0
-REFERENCE_ITER_GAP = 3  # provisional: rounding alone moves the stop by 3 (permuted oracle runs)
+# Against the reference's published counts: at most 3 on the 25 stress rows
+# (19 within one) and 2 on the bell systems N = 20..44 (rebuilt to 0.1% of
+# the published nnz).
+REFERENCE_ITER_GAP = 3
 STOP_SLACK = ENVELOPE_FACTOR
 
 
@@ -42,3 +49,9 @@ def check_trajectory(h_a, it_a, h_b, it_b, tol):
     elif it_b < it_a:
         assert h_a[it_b] <= STOP_SLACK * tol, (it_a, it_b, h_a[it_b])
     return abs(it_a - it_b)
+
+
+def check_stop_gap(it_a, it_b):
+    """Record and bound the stop gap of two solves of one system."""
+    GAPS.append(abs(it_a - it_b))
+    assert abs(it_a - it_b) <= MAX_ITER_GAP, (it_a, it_b)
