@@ -33,6 +33,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <vector>
 
 #include "../../include/cutcell_hip.h"
@@ -1395,6 +1396,42 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Fixed device scratch for cc_pcg_jacobi_f64_noalloc.
+//
+// One file-scope array per GPU (the code object's global lives on every
+// device the library runs on, until the process ends). Host code never uses
+// the array's name as an address: at a device's first call the address is
+// read once with hipGetSymbolAddress, with that device current, and kept per
+// device. The solver writes every word of the workspace it reads before it
+// reads it (k_cg_setup, k_diag and the producer kernels), so no clearing is
+// needed between calls; calls that overlap on one GPU share the array and
+// race.
+// ---------------------------------------------------------------------------
+
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kScratchDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static void* g_scratch_ptr[kScratchDevices];
+static int g_scratch_status[kScratchDevices];
+static std::once_flag g_scratch_once[kScratchDevices];
+
+static int scratch_for_current_device(void** out) {
+  int dev = -1;
+  CC_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return CC_E_ARG;
+  std::call_once(g_scratch_once[dev], [dev] {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_scratch_status[dev] = (int)e;
+    g_scratch_ptr[dev] = e == hipSuccess ? p : nullptr;
+  });
+  if (g_scratch_status[dev] != 0) return g_scratch_status[dev];
+  if (!g_scratch_ptr[dev]) return CC_E_NODEVICE;
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 // Hooks used by the batched solver (batched.hip).
 void prof_begin_batched(hipStream_t s) { prof_begin(kProfBatched, s); }
 void prof_end_batched(hipStream_t s) { prof_end(s); }
@@ -1598,6 +1635,24 @@ int cc_pcg_jacobi_mixed_rb(int32_t n, int32_t k, const int32_t* indptr, const in
   // The inner solve never reads the CSR float32 values when rb is given.
   CC_DISPATCH_K(k, (pcg_mixed<KK>(n, indptr, indices, vals, nullptr, B, X, tol, maxiter, inner_tol, max_outer,
                                   preconditioner, check_every, workspace, workspace_bytes, results, s, rb)));
+}
+
+
+size_t cc_pcg_noalloc_max_bytes(void) { return kScratchBytes; }
+
+int cc_pcg_jacobi_f64_noalloc(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices,
+                              const double* vals, const double* B, double* X, double tol, int32_t maxiter,
+                              int32_t preconditioner, int32_t check_every, cc_pcg_result* results, double* history,
+                              cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !B || !X || !results || maxiter < 0) return CC_E_ARG;
+  if (k != 1 && k != 2 && k != 4 && k != 8 && k != 16) return CC_E_ARG;
+  const size_t need = cc_pcg_workspace_bytes(n, k, 0);
+  if (need == 0 || need > kScratchBytes) return CC_E_TOO_LARGE;
+  void* ws = nullptr;
+  CC_CHECK(scratch_for_current_device(&ws));
+  hipStream_t s = (hipStream_t)stream;
+  CC_DISPATCH_K(k, (pcg_f64<KK>(n, indptr, indices, vals, B, X, tol, maxiter, preconditioner, check_every, ws,
+                                kScratchBytes, results, history, s)));
 }
 
 }  // extern "C"

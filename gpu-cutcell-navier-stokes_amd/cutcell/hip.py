@@ -81,6 +81,8 @@ _SIGNATURES = {
     "cc_residual_f64": ([_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _SZ, _P], ctypes.c_int),
     "cc_pcg_workspace_bytes": ([_I32, _I32, _I32], _SZ),
     "cc_pcg_jacobi_f64": ([_I32, _I32, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int),
+    "cc_pcg_noalloc_max_bytes": ([], _SZ),
+    "cc_pcg_jacobi_f64_noalloc": ([_I32, _I32, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _P, _P], ctypes.c_int),
     "cc_pcg_jacobi_mixed": (
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P],
         ctypes.c_int,
@@ -151,7 +153,13 @@ def require_gpu():
     return load_library()
 
 
+CC_E_TOO_LARGE = -4
+
+
 def _check(status: int, what: str) -> None:
+    if status == CC_E_TOO_LARGE:
+        raise HipError(f"{what}: the problem needs more device scratch than the library's fixed array holds "
+                       f"(status {status})")
     if status != 0:
         raise HipError(f"{what} failed with status {status}")
 

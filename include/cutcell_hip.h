@@ -38,7 +38,8 @@ enum {
   CC_OK = 0,
   CC_E_ARG = -1,       /* bad size / null pointer / unsupported k */
   CC_E_WORKSPACE = -2, /* workspace too small */
-  CC_E_NODEVICE = -3
+  CC_E_NODEVICE = -3,
+  CC_E_TOO_LARGE = -4  /* the problem needs more than a fixed scratch holds */
 };
 
 /* Per-column solve status (the `message` of the reference report). */
@@ -151,6 +152,23 @@ int cc_pcg_jacobi_mixed(int32_t n, int32_t k, const int32_t* indptr, const int32
                         const float* vals32, const double* B, double* X, double tol, int32_t maxiter,
                         double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
                         void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream);
+
+/* cc_pcg_jacobi_f64 without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). No counterpart in the reference. Same
+ * arguments, kernels and results as cc_pcg_jacobi_f64; the workspace is a
+ * fixed device array of cc_pcg_noalloc_max_bytes() (64 MiB) that the library
+ * holds on every GPU it runs on until the process ends. Problems whose
+ * cc_pcg_workspace_bytes(n, k, 0) exceeds it return CC_E_TOO_LARGE
+ * (C2's n = 81,195 fits for every k up to 16). The entry point calls no
+ * hipMalloc/hipFree. Every call on one GPU shares that GPU's array: calls
+ * that overlap on one GPU (several streams or host threads) race. The stream
+ * must belong to the current device. */
+size_t cc_pcg_noalloc_max_bytes(void);
+int cc_pcg_jacobi_f64_noalloc(int32_t n, int32_t k, const int32_t* indptr, const int32_t* indices,
+                              const double* vals, const double* B, double* X, double tol, int32_t maxiter,
+                              int32_t preconditioner, int32_t check_every, cc_pcg_result* results, double* history,
+                              cc_stream_t stream);
 
 /* Row-block form of a CSR matrix (no reference counterpart: a derived layout
  * for products with several right-hand sides, built once per matrix). Rows

@@ -93,6 +93,23 @@ def test_gpu_backend_without_gpu_reports_unavailable():
     assert rep.backend == "cpu" and rep.success and rep.message.startswith("GPU skipped/fell back")
 
 
+def test_noalloc_entry_rejects_oversize_before_touching_the_gpu():
+    """cc_pcg_jacobi_f64_noalloc checks the problem against its fixed 64 MiB
+    scratch before any HIP call (the pointers are never dereferenced)."""
+    import ctypes
+
+    from cutcell import hip
+
+    lib = hip.load_library()
+    assert lib.cc_pcg_noalloc_max_bytes() == 64 << 20
+    dummy = ctypes.c_double(0.0)
+    p = ctypes.addressof(dummy)
+    res = (hip.PcgResult * 16)()
+    st = lib.cc_pcg_jacobi_f64_noalloc(200_000, 16, p, p, p, p, p, 1e-8, 10, 1, 16, ctypes.addressof(res), None, None)
+    assert st == hip.CC_E_TOO_LARGE
+    assert lib.cc_pcg_jacobi_f64_noalloc(10, 3, p, p, p, p, p, 1e-8, 10, 1, 16, ctypes.addressof(res), None, None) == -1
+
+
 def test_cpu_gmres_follows_scipy():
     """solve_cpu_gmres (code/fast_solver_backends.py:135-191): restarted SciPy
     GMRES with the Jacobi preconditioner, iterations counted by the pr_norm
