@@ -81,20 +81,94 @@ def _cache_dir() -> Path:
 
 
 def build_workload(seed: int, target: int, degree: int = 3):
-    """Build (or load from the per-machine cache) the planar-cut system."""
-    from cutcell.cut3d import planar_cut_system
+    """Build (or load from the per-machine cache) the planar-cut system and
+    the factors of its E-fused form (one assembly serves both)."""
+    from cutcell.cut3d import (PlanarCut, System3D, assemble_levelset_3d, planar_grid_for_target,
+                               polynomial_preserving_extension_3d, restrict_spd)
+    from cutcell.ekte import ekte_parts
 
-    key = hashlib.sha1(f"planar3d-v1-{seed}-{target}-{degree}".encode()).hexdigest()[:16]
+    key = hashlib.sha1(f"planar3d-v2-{seed}-{target}-{degree}".encode()).hexdigest()[:16]
     path = _cache_dir() / f"planar3d_{key}.pkl"
     if path.exists():
         with path.open("rb") as f:
             return pickle.load(f)
     t0 = time.perf_counter()
-    system = planar_cut_system(seed=seed, degree=degree, target_unknowns=target)
+    cut = PlanarCut.random(seed)
+    N = planar_grid_for_target(cut, degree, target)
+    asm = assemble_levelset_3d(cut, N, degree)
+    agg = polynomial_preserving_extension_3d(asm)
+    t_asm = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    A = (restrict_spd(asm.K, agg.E) + restrict_spd(asm.H, agg.E)).tocsr()
+    A.sort_indices()
+    t_restrict = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    parts = ekte_parts(asm.K + asm.H, agg.E, agg.interior, agg.small_support)
+    t_parts = time.perf_counter() - t1
+    meta = dict(n_elements=N, degree=degree, n_full=asm.n_full, n_active=int(asm.active.size),
+                n_reduced=int(A.shape[0]), nnz=int(A.nnz), n_cut_cells=asm.n_cut_cells,
+                min_cut_fraction=asm.min_cut_fraction, polynomial_residual=agg.polynomial_residual,
+                kind="planar3d", seed=seed, normal=cut.normal.tolist(), offset=cut.offset,
+                assembly_and_extension_s=t_asm, restrict_s=t_restrict, ekte_parts_s=t_parts)
+    system = System3D(A, agg.E, agg.interior, agg.small_support, agg.active, meta)
     system.meta["build_s"] = time.perf_counter() - t0
+    out = (system, parts)
     with path.open("wb") as f:
-        pickle.dump(system, f, protocol=pickle.HIGHEST_PROTOCOL)
-    return system
+        pickle.dump(out, f, protocol=pickle.HIGHEST_PROTOCOL)
+    return out
+
+
+def ekte_section(hip, torch, system, parts, B, reps: int = 20):
+    """The E-fused operator on the cut-box system, with the product on B in
+    CSR and in row-block form: float64 Jacobi-PCG on the 16 right-hand sides
+    (solves/s, iterations) with the apply inside the loop timed by HIP events
+    on the launch stream, its algorithmic bytes (B over the active dofs, E_s
+    and T once, vectors) per apply, and the standalone apply (which also
+    copies x in and y out) on 1 and 16 columns."""
+    from cutcell.ekte import DeviceEkte
+
+    out = {"n": parts.n_red, "n_small": parts.n_small, "n_touch": int(parts.t_rows.size),
+           "nnz_B_active": parts.B.nnz, "nnz_Es": parts.Es.nnz, "nnz_formed": int(system.A.nnz),
+           "host_restrict_s": sig(system.meta.get("restrict_s")), "host_parts_s": sig(system.meta.get("ekte_parts_s"))}
+    for form in ("csr", "rb"):
+        dev = DeviceEkte(parts, row_blocks=form == "rb")
+        o = {}
+        for k in (1, 16):
+            X = torch.randn(parts.n_red, k, dtype=torch.float64, device="cuda")
+            Y = torch.empty_like(X)
+            for _ in range(3):
+                dev.apply(X, Y)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                dev.apply(X, Y)
+            e1.record()
+            torch.cuda.synchronize()
+            o[f"standalone_apply_k{k}_us"] = sig(1e3 * e0.elapsed_time(e1) / reps)
+        dev.pcg(B, tol=TOL, maxiter=20000)  # warm-up (and autotuned shapes)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        X, res = dev.pcg(B, tol=TOL, maxiter=20000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        hip.profile_enable(["spmm"])
+        dev.pcg(B, tol=TOL, maxiter=20000)
+        ms, cnt = hip.profile_read("spmm")
+        hip.profile_enable(False)
+        us = 1e3 * ms / max(cnt, 1)
+        nbytes = parts.algorithmic_bytes(B.shape[1])
+        its = [r.iterations for r in res]
+        o["pcg_fp64_k16"] = {"solves_per_s": sig(B.shape[1] / dt), "ms_per_step": sig(dt * 1e3),
+                             "converged": all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res),
+                             "iterations_range": [min(its), max(its)],
+                             "true_rel_max": sig(max(r.true_rel for r in res)),
+                             "apply_in_loop_us": sig(us), "apply_bytes": nbytes,
+                             "apply_GBps": sig(nbytes / (us * 1e-6) / 1e9),
+                             "apply_frac_of_8TBps": sig(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS)}
+        out[form] = o
+        del dev
+    return out
 
 
 def spmm_bytes(n: int, nnz: int, k: int, value_bytes: int, vec_bytes: int) -> int:
@@ -629,6 +703,7 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-error-check", action="store_true")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched uneven-systems section")
+    ap.add_argument("--no-ekte", action="store_true", help="skip the E-fused operator section")
     ap.add_argument("--no-tune", action="store_true", help="use the default SpMM launch shape")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-preconditioner section")
     ap.add_argument("--learned-n", type=int, default=64, help="background elements per axis of the sphere box")
@@ -670,7 +745,7 @@ def main(argv=None) -> int:
     if not args.operator_probe:
         log("building the 3D planar-cut system (p=3, ~81k unknowns) ...")
         t0 = time.perf_counter()
-        system = build_workload(args.seed, args.target)
+        system, ekte_parts = build_workload(args.seed, args.target)
         log(f"system: n={system.A.shape[0]} nnz={system.A.nnz} N={system.meta['n_elements']} "
             f"({time.perf_counter() - t0:.1f} s)")
         rng = np.random.default_rng(1000 + args.seed)
@@ -773,6 +848,12 @@ def main(argv=None) -> int:
                     csr_equivalent_GBps=round(csr_alg / (head["spmm_avg_us"] * 1e-6) / 1e9, 1),
                     frac_b_spmm=round(csr_alg / (head["spmm_avg_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4))
 
+    ekte = None
+    if not args.no_ekte:
+        log("E-fused operator (no formed E^T A E) on the cut-box system ...")
+        ekte = ekte_section(hip, torch, system, ekte_parts, Bd)
+        log(f"ekte: {ekte}")
+
     log("single-vector SpMV bandwidth on the workload matrix ...")
     spmv = spmv_bandwidth(hip, torch, Ad)
     log(f"spmv: {spmv}")
@@ -823,6 +904,7 @@ def main(argv=None) -> int:
         },
         "roofline": roofline,
         "spmv_k1": spmv,
+        "ekte_fused": ekte,
         "cpu_baseline": cpu,
         "batched_uneven_2d": batched,
         "learned_sphere_3d": learned,

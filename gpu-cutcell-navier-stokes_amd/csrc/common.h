@@ -85,11 +85,16 @@ size_t cg_operator_bytes(size_t n);
 // The same for k in {1, 2, 4, 8, 16} right-hand sides (row-major [n][k]);
 // apply writes per-block, per-column partials [nparts][k]; residual writes k
 // relative residuals to host memory.
+// tail: rows after P and Q ([n + tail][k]) that the operator may use.
 int cg_operator_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& apply,
                       const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
                       int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
-                      const std::function<int(const double*, double*, double*)>& residual);
-size_t cg_operator_multi_bytes(size_t n, int k);
+                      const std::function<int(const double*, double*, double*)>& residual, size_t tail = 0);
+size_t cg_operator_multi_bytes(size_t n, int k, size_t tail = 0);
+// Y = A X (CSR, float64, k in {1, 2, 4, 8, 16}) with the autotuned launch
+// shape; partial[block][c] gets per-block X.Y sums when not null (pcg.hip).
+int csr_spmm_f64(int n, int k, double avg_row, const int* indptr, const int* indices, const double* vals,
+                 const double* X, double* Y, double* partial, hipStream_t s, int* gs);
 }  // namespace cc
 
 struct cc_cnn_desc;

@@ -1084,11 +1084,13 @@ struct CgBuffers {
   ColState* st;
 };
 
+// tail: extra rows after P and Q (an operator may keep per-apply data
+// there, as the E-fused product keeps its small dofs' values).
 template <typename VT>
-static size_t cg_bytes(size_t n, int k) {
+static size_t cg_bytes(size_t n, int k, size_t tail = 0) {
   size_t b = 0;
   auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
-  for (int i = 0; i < 5; ++i) add(n * k * sizeof(VT));
+  for (int i = 0; i < 5; ++i) add((i >= 3 ? n + tail : n) * k * sizeof(VT));
   add(n * sizeof(double));
   add((size_t)kMaxParts * 2 * k * sizeof(double));
   add((size_t)kMaxParts * 2 * k * sizeof(double));
@@ -1097,14 +1099,14 @@ static size_t cg_bytes(size_t n, int k) {
 }
 
 template <typename VT>
-static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external) {
+static CgBuffers<VT> carve_cg(Carve& cv, size_t n, int k, VT* X_external, size_t tail = 0) {
   CgBuffers<VT> b;
   b.X = cv.take<VT>(n * k);
   if (X_external) b.X = X_external;
   b.R = cv.take<VT>(n * k);
   b.Z = cv.take<VT>(n * k);
-  b.P = cv.take<VT>(n * k);
-  b.Q = cv.take<VT>(n * k);
+  b.P = cv.take<VT>((n + tail) * k);
+  b.Q = cv.take<VT>((n + tail) * k);
   b.diag = cv.take<double>(n);
   b.pa = cv.take<double>((size_t)kMaxParts * 2 * k);
   b.pb = cv.take<double>((size_t)kMaxParts * 2 * k);
@@ -1228,9 +1230,9 @@ template <int K>
 static int cg_operator_k(int n, const std::function<int(const double*, double*, double*, int*)>& apply,
                          const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
                          int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
-                         const std::function<int(const double*, double*, double*)>& residual) {
+                         const std::function<int(const double*, double*, double*)>& residual, size_t tail) {
   Carve cv{(char*)ws, 0, wsb};
-  CgBuffers<double> b = carve_cg<double>(cv, n, K, X);
+  CgBuffers<double> b = carve_cg<double>(cv, n, K, X, tail);
   if (!cv.ok()) return CC_E_WORKSPACE;
   if (precond && diag_src)
     CC_CHECK(hipMemcpyAsync(b.diag, diag_src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -1254,23 +1256,44 @@ static int cg_operator_k(int n, const std::function<int(const double*, double*, 
 int cg_operator_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& apply,
                       const double* diag_src, const double* B, double* X, double tol, int maxiter, int precond,
                       int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
-                      const std::function<int(const double*, double*, double*)>& residual) {
+                      const std::function<int(const double*, double*, double*)>& residual, size_t tail) {
+#define CC_OPK(KK)                                                                                            \
+  case KK:                                                                                                    \
+    return cg_operator_k<KK>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s, \
+                             residual, tail);
   switch (k) {
-    case 1: return cg_operator_k<1>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
-                                    residual);
-    case 2: return cg_operator_k<2>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
-                                    residual);
-    case 4: return cg_operator_k<4>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
-                                    residual);
-    case 8: return cg_operator_k<8>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results, s,
-                                    residual);
-    case 16: return cg_operator_k<16>(n, apply, diag_src, B, X, tol, maxiter, precond, check_every, ws, wsb, results,
-                                      s, residual);
+    CC_OPK(1)
+    CC_OPK(2)
+    CC_OPK(4)
+    CC_OPK(8)
+    CC_OPK(16)
     default: return CC_E_ARG;
   }
+#undef CC_OPK
 }
 
-size_t cg_operator_multi_bytes(size_t n, int k) { return cg_bytes<double>(n, k); }
+size_t cg_operator_multi_bytes(size_t n, int k, size_t tail) { return cg_bytes<double>(n, k, tail); }
+
+// Y = A X over n rows for k columns with the autotuned CSR launch shape
+// (float64); partial[block][c] receives the per-block sums of X.Y when not
+// null, *gs the number of blocks. avg_row (the mean row length) picks the
+// default shape when the autotuner set none.
+int csr_spmm_f64(int n, int k, double avg_row, const int* indptr, const int* indices, const double* vals,
+                 const double* X, double* Y, double* partial, hipStream_t s, int* gs) {
+#define CC_SPK(KK)                                                                                                 \
+  case KK:                                                                                                         \
+    return spmm_launch<double, double, KK, 0>(n, avg_row, indptr, indices, vals, X, Y, nullptr, partial, nullptr, s, \
+                                              gs);
+  switch (k) {
+    CC_SPK(1)
+    CC_SPK(2)
+    CC_SPK(4)
+    CC_SPK(8)
+    CC_SPK(16)
+    default: return CC_E_ARG;
+  }
+#undef CC_SPK
+}
 
 // True relative residuals ||B - A X|| / ||B|| (||B|| = 0 -> 1) of the float64
 // block X, written to rel[K] (host). The per-block partial sums are combined

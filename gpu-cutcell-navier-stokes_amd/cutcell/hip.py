@@ -87,6 +87,9 @@ _SIGNATURES = {
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P],
         ctypes.c_int,
     ),
+    "cc_ekte_workspace_bytes": ([_P, _I32, _I32], _SZ),
+    "cc_ekte_apply": ([_P, _I32, _P, _P, _P, _SZ, _P], ctypes.c_int),
+    "cc_pcg_ekte": ([_P, _I32, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
     "cc_rb_build_count": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_build_fill": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),

@@ -222,6 +222,56 @@ int cc_pcg_jacobi_mixed_rb(int32_t n, int32_t k, const int32_t* indptr, const in
                            double inner_tol, int32_t max_outer, int32_t preconditioner, int32_t check_every,
                            void* workspace, size_t workspace_bytes, cc_pcg_result* results, cc_stream_t stream);
 
+/* The aggregated operator E^T B E applied without forming it (replaces the
+ * triple product restrict_spd, code/bell_3d_core.py:550-552, and the cuSPARSE
+ * product A_gpu @ p by it, code/fast_solver_backends.py:250). B is the
+ * background operator (K + H, symmetrised) over the active dofs, its rows and
+ * columns renumbered: index j < n_red is the interior dof of reduced unknown
+ * j, index n_red + s is small dof s. E_s (n_small rows) holds the small dofs'
+ * extension weights over the reduced unknowns; T (n_touch rows) is E_s^T
+ * restricted to the reduced unknowns it reaches (t_rows ascending, each row's
+ * small dofs t_small with weights t_vals). One apply is three launches on a
+ * combined vector [x ; u_s] of n_red + n_small rows: the small dofs' gather
+ * u_s = E_s x, the product [y ; v_s] = B [x ; u_s] (the autotuned CSR
+ * product, or the row-block one when b_rb is set), and y[t_rows] += T v_s
+ * summed in a fixed order. diag (may be NULL) is the Jacobi diagonal of
+ * E^T B E. All arrays are device memory. */
+typedef struct cc_ekte_desc {
+  int32_t n_red;
+  int32_t n_small;
+  int32_t n_touch;
+  int32_t pad;
+  int64_t nnz_b;            /* nonzeros of B (picks the default product shape) */
+  const int32_t* b_indptr;  /* [n_red + n_small + 1] */
+  const int32_t* b_indices; /* combined numbering */
+  const double* b_vals;
+  const int32_t* e_indptr;  /* [n_small + 1] */
+  const int32_t* e_indices; /* reduced unknowns */
+  const double* e_vals;
+  const int32_t* t_rows;    /* [n_touch] reduced unknowns */
+  const int32_t* t_indptr;  /* [n_touch + 1] */
+  const int32_t* t_small;   /* small dof indices */
+  const double* t_vals;
+  const struct cc_rb_desc* b_rb; /* row-block form of B (float64 values), or NULL for the CSR product */
+  const double* diag;       /* [n_red] or NULL */
+} cc_ekte_desc;
+
+/* Workspace of one apply (solve = 0) or of cc_pcg_ekte (solve = 1) on k
+ * right-hand sides; 0 on a bad descriptor or k. */
+size_t cc_ekte_workspace_bytes(const cc_ekte_desc* d, int32_t k, int32_t solve);
+/* Y = E^T B E X for k in {1, 2, 4, 8, 16} columns (row-major [n_red][k]);
+ * X is copied into and Y out of the workspace's combined vectors. */
+int cc_ekte_apply(const cc_ekte_desc* d, int32_t k, const double* X, double* Y, void* workspace,
+                  size_t workspace_bytes, cc_stream_t stream);
+/* Jacobi-PCG (float64, the reference GPU loop's recurrence and stopping
+ * rules, code/fast_solver_backends.py:233-274) on the fused operator for k
+ * right-hand sides; results[k] host memory; synchronises the stream. Its
+ * direction and product vectors carry the n_small tail rows, so the products
+ * inside the loop copy nothing. */
+int cc_pcg_ekte(const cc_ekte_desc* d, int32_t k, const double* B, double* X, double tol, int32_t maxiter,
+                int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                cc_pcg_result* results, cc_stream_t stream);
+
 /* Batched persistent solve of nsys independent systems of uneven size in one
  * launch (replaces a loop of solve_spd calls over independent cases, the
  * reference's case-level parallelism in code/fast_solver_workbench.py:201-217,
