@@ -46,6 +46,19 @@ class PcgResult(ctypes.Structure):
     ]
 
 
+class GmresResult(ctypes.Structure):
+    """``cc_gmres_result`` of the C header."""
+
+    _fields_ = [
+        ("iterations", ctypes.c_int32),
+        ("info", ctypes.c_int32),
+        ("restarts", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("residual_norm", ctypes.c_double),
+        ("b_norm", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 _P = ctypes.c_void_p
@@ -90,6 +103,8 @@ _SIGNATURES = {
     "cc_ekte_workspace_bytes": ([_P, _I32, _I32], _SZ),
     "cc_ekte_apply": ([_P, _I32, _P, _P, _P, _SZ, _P], ctypes.c_int),
     "cc_pcg_ekte": ([_P, _I32, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
+    "cc_gmres_workspace_bytes": ([_I32, _I32], _SZ),
+    "cc_gmres_jacobi_f64": ([_I32, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
     "cc_rb_build_count": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_build_fill": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),
@@ -458,3 +473,24 @@ def pcg(
     if history:
         return out, list(res), (hist.cpu().numpy() if hist is not None else None)
     return out, list(res)
+
+
+def gmres(A: DeviceCSR, b, *, tol: float = 1.0e-8, restart: int = 80, maxiter: int = 5000,
+          preconditioner: str = "jacobi", workspace: Workspace | None = None):
+    """Restarted GMRES (float64, one right-hand side) on the device; returns
+    ``(x, GmresResult)``. ``maxiter`` counts restarts, as SciPy's does."""
+    lib = require_gpu()
+    torch = _torch()
+    bd = b.reshape(-1).to(torch.float64).contiguous()
+    if bd.numel() != A.n:
+        raise ValueError("b has the wrong length")
+    x = torch.empty_like(bd)
+    rs = int(min(restart if restart > 0 else 20, A.n))
+    nbytes = int(lib.cc_gmres_workspace_bytes(A.n, rs))
+    ws = (workspace or _default_ws).get(nbytes, A.vals.device)
+    res = GmresResult()
+    prec = {"none": 0, "jacobi": 1}[preconditioner]
+    _check(lib.cc_gmres_jacobi_f64(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(bd), _ptr(x),
+                                   float(tol), rs, int(maxiter), prec, _ptr(ws), nbytes, ctypes.addressof(res),
+                                   _stream()), "gmres")
+    return x, res

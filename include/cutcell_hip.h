@@ -170,6 +170,30 @@ int cc_pcg_jacobi_f64_noalloc(int32_t n, int32_t k, const int32_t* indptr, const
                               int32_t preconditioner, int32_t check_every, cc_pcg_result* results, double* history,
                               cc_stream_t stream);
 
+/* Restarted GMRES with the Jacobi preconditioner (CPU rule: zero diagonal
+ * -> 0) on a CSR matrix, float64, one right-hand side, x0 = 0: the GMRES leg
+ * of the solver interface (replaces the SciPy gmres call of solve_cpu_gmres,
+ * code/fast_solver_backends.py:135-191, with its semantics: left
+ * preconditioning, modified Gram-Schmidt, Givens rotations as LAPACK
+ * dlartg, the inner stop on the preconditioned residual with the adaptive
+ * tolerance, the outer stop on ||b - A x|| <= tol ||b||, maxiter counting
+ * restarts (0 -> 10 n), restart (0 -> 20) capped at n). The vector work runs
+ * on the device; the Hessenberg problem on the host, which reads one column
+ * per inner step (so the call synchronises the stream each step). */
+typedef struct cc_gmres_result {
+  int32_t iterations;    /* inner steps, as SciPy's pr_norm callback counts them */
+  int32_t info;          /* 0 converged, else maxiter */
+  int32_t restarts;      /* outer cycles run */
+  int32_t pad;
+  double residual_norm;  /* ||b - A x|| at exit (float64, device) */
+  double b_norm;         /* ||b|| */
+} cc_gmres_result;
+size_t cc_gmres_workspace_bytes(int32_t n, int32_t restart);
+int cc_gmres_jacobi_f64(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                        const double* b, double* x, double tol, int32_t restart, int32_t maxiter,
+                        int32_t preconditioner, void* workspace, size_t workspace_bytes, cc_gmres_result* result,
+                        cc_stream_t stream);
+
 /* Row-block form of a CSR matrix (no reference counterpart: a derived layout
  * for products with several right-hand sides, built once per matrix). Rows
  * are taken in groups of CC_RB_ROWS consecutive rows; group g stores the
