@@ -374,6 +374,10 @@ def batched_section(hip, torch, steps: int, warmup: int, seed: int = 0, systems=
            "build_s": sig(build_s)}
     if cpu_ref is not None:
         out["cpu_reference"] = cpu_ref
+    Xref = torch.empty_like(B)
+    bs.solve_device(B, Xref, tol=1e-13, maxiter=40000, precision="fp64")
+    torch.cuda.synchronize()
+    ref_parts = bs.split(Xref)
     for method in ("fp64", "mixed"):
         cfg = bs.solve_device(B, X, tol=TOL, maxiter=20000, precision=method)  # tunes on first use
         for _ in range(warmup):
@@ -402,6 +406,8 @@ def batched_section(hip, torch, steps: int, warmup: int, seed: int = 0, systems=
             "iterations_range": [int(iters.min()), int(iters.max())],
             "matrix_GBps_over_launch": sig(launch_bytes / (avg_ms * 1e-3) / 1e9),
             "launch": cfg,
+            "error_vs_tight_fp64": sig(max(float(np.linalg.norm(x - r) / np.linalg.norm(r))
+                                           for x, r in zip(bs.split(X), ref_parts))),
         }
         log(f"batched {method}: {out[method]}")
     return out
@@ -459,64 +465,143 @@ def profiled_once(hip, run):
     return apply_ms, apply_cnt, prec_ms, prec_cnt
 
 
-def time_solver(hip, torch, solve, B):
-    """Solve each right-hand side of B in turn; solves/s and per-solve figures."""
+def _errors(torch, X, ref, modulo_constant: bool):
+    """Largest relative error of the columns of X against ref (lists of
+    device vectors); with modulo_constant, both are first shifted to zero
+    mean (the singular pressure operator fixes its solution up to a constant)."""
+    if ref is None:
+        return None
+    worst = 0.0
+    for x, r in zip(X, ref):
+        d, rr = x - r, r
+        if modulo_constant:
+            d = d - d.mean()
+            rr = r - r.mean()
+        worst = max(worst, float((torch.linalg.norm(d) / torch.linalg.norm(rr)).item()))
+    return sig(worst)
+
+
+def _stats(hip, res, dt, count):
+    iters = [int(r.iterations) for r in res]
+    return {"solves_per_s": sig(count / dt), "converged": all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL
+                                                              for r in res),
+            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
+            "true_rel_max": sig(max(float(r.true_rel) for r in res))}
+
+
+def time_solver(hip, torch, solve, B, ref=None, modulo_constant=False):
+    """Solve each right-hand side of B in turn; solves/s, per-solve figures
+    and, with ref, the largest relative error against it."""
     solve(B[0])  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = [solve(b)[1] for b in B]
+    out = [solve(b) for b in B]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     apply_ms, apply_cnt, prec_ms, prec_cnt = profiled_once(hip, lambda: solve(B[0]))
-    iters = [int(r.iterations) for r in res]
-    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
-    return {"solves_per_s": sig(len(B) / dt), "ms_per_solve": sig(1e3 * dt / len(B)), "converged": ok,
-            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
-            "true_rel_max": sig(max(float(r.true_rel) for r in res)),
-            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1)),
-            "learned_prec_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None}
+    o = _stats(hip, [r for _, r in out], dt, len(B))
+    o["ms_per_solve"] = sig(1e3 * dt / len(B))
+    o["apply_avg_us"] = sig(1e3 * apply_ms / max(apply_cnt, 1))
+    o["learned_prec_avg_us"] = sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None
+    o["error"] = _errors(torch, [x for x, _ in out], ref, modulo_constant)
+    return o
 
 
-def time_batched(hip, torch, dmf, B):
-    """All columns of the list B (len in {1, 2, 4, 8, 16}) in one batched
-    matrix-free Jacobi-PCG (each cut block read once per apply for all
-    columns)."""
-    Bk = torch.stack(B, dim=1).contiguous()
-    dmf.pcg_batched(Bk, tol=TOL, maxiter=20000)  # warm-up
+def _chunks(B, k=16):
+    return [B[i:i + k] for i in range(0, len(B), k)]
+
+
+def time_batched_stream(hip, torch, solve_k, B, profile, ref=None, modulo_constant=False):
+    """The stream B in batches of up to 16 columns, each batch one solve_k
+    call (columns share every operator apply); solves/s over the stream."""
+    chunks = [torch.stack(c, dim=1).contiguous() for c in _chunks(B)]
+    solve_k(chunks[0])  # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    _, res = dmf.pcg_batched(Bk, tol=TOL, maxiter=20000)
+    out = [solve_k(c) for c in chunks]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    apply_ms, apply_cnt, _, _ = profiled_once(hip, lambda: dmf.pcg_batched(Bk, tol=TOL, maxiter=20000))
-    iters = [int(r.iterations) for r in res]
-    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
-    return {"solves_per_s": sig(len(B) / dt), "ms_per_batch": sig(1e3 * dt), "columns": len(B), "converged": ok,
-            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
-            "true_rel_max": sig(max(float(r.true_rel) for r in res)),
-            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1))}
+    apply_ms, apply_cnt, prec_ms, prec_cnt = profiled_once(hip, lambda: solve_k(chunks[0]))
+    res = [r for _, rs in out for r in rs]
+    X = [Xk[:, c] for Xk, _ in out for c in range(Xk.shape[1])]
+    o = _stats(hip, res, dt, len(B))
+    o["ms_per_batch"] = sig(1e3 * dt / len(chunks))
+    o["columns"] = int(chunks[0].shape[1])
+    o["batches"] = len(chunks)
+    o["apply_avg_us"] = sig(1e3 * apply_ms / max(apply_cnt, 1))
+    if profile == "learned":
+        o["learned_prec_avg_us"] = sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None
+    o["error"] = _errors(torch, X, ref, modulo_constant)
+    return o
 
 
-def time_learned_batched(hip, torch, net, dmf, B):
-    """All columns of the list B in one batched flexible CG with the
-    multilevel learned preconditioner (one K-column operator apply and one
-    batched network pass per iteration)."""
-    Bk = torch.stack(B, dim=1).contiguous()
-    net.fcg_batched(dmf, Bk, tol=TOL, maxiter=20000)  # warm-up
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    _, res = net.fcg_batched(dmf, Bk, tol=TOL, maxiter=20000)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    apply_ms, apply_cnt, prec_ms, prec_cnt = profiled_once(hip, lambda: net.fcg_batched(dmf, Bk, tol=TOL,
-                                                                                         maxiter=20000))
-    iters = [int(r.iterations) for r in res]
-    ok = all(r.status == hip.CC_CONVERGED and r.true_rel <= 10 * TOL for r in res)
-    return {"solves_per_s": sig(len(B) / dt), "ms_per_batch": sig(1e3 * dt), "columns": len(B), "converged": ok,
-            "iterations_range": [min(iters), max(iters)], "iterations_mean": sig(float(np.mean(iters))),
-            "true_rel_max": sig(max(float(r.true_rel) for r in res)),
-            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1)),
-            "learned_prec_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1)) if prec_cnt else None}
+def time_batched(hip, torch, dmf, B, ref=None, modulo_constant=False):
+    """Batched matrix-free Jacobi-PCG (each cut block read once per apply for
+    all columns of a batch)."""
+    return time_batched_stream(hip, torch, lambda Bk: dmf.pcg_batched(Bk, tol=TOL, maxiter=20000), B, "jacobi",
+                               ref, modulo_constant)
+
+
+def time_learned_batched(hip, torch, net, dmf, B, ref=None, modulo_constant=False):
+    """Batched flexible CG with the multilevel learned preconditioner (one
+    K-column operator apply and one batched network pass per iteration)."""
+    return time_batched_stream(hip, torch, lambda Bk: net.fcg_batched(dmf, Bk, tol=TOL, maxiter=20000), B, "learned",
+                               ref, modulo_constant)
+
+
+ERR_BAR = 1.0e-8  # solutions within 1e-8 relative of the float64 reference
+MATCH_TOLS = (1.0e-8, 5.0e-9, 2.0e-9, 1.0e-9, 5.0e-10, 2.0e-10)
+
+
+def accuracy_matched(hip, torch, solve_k, B, ref, modulo_constant=False):
+    """The loosest residual tolerance in MATCH_TOLS at which every column's
+    solution error against ref is within ERR_BAR, and the batched solves/s
+    there (batches of up to 16 columns, one warm-up batch first)."""
+    chunks = [torch.stack(c, dim=1).contiguous() for c in _chunks(B)]
+    for tol in MATCH_TOLS:
+        solve_k(chunks[0], tol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = [solve_k(c, tol) for c in chunks]
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        X = [Xk[:, c] for Xk, _ in out for c in range(Xk.shape[1])]
+        err = _errors(torch, X, ref, modulo_constant)
+        its = [int(r.iterations) for _, rs in out for r in rs]
+        if err is not None and err <= ERR_BAR:
+            return {"tol": tol, "solves_per_s": sig(len(B) / dt), "error": err,
+                    "iterations_mean": sig(float(np.mean(its)))}
+    return {"tol": None, "error": err, "note": f"no tolerance down to {MATCH_TOLS[-1]} met {ERR_BAR}"}
+
+
+def divergence_stream(mf, dmf, torch, count: int, seed: int):
+    """Right-hand sides b = A psi of the pressure operator for smooth random
+    potentials psi: the discrete divergence of a velocity field sees only its
+    gradient part, so these are the divergences of random velocity fields
+    u = grad psi; they sum to zero (A 1 = 0) and psi is the exact solution up
+    to a constant. psi sums six random modes, periodic in x and z (integer
+    wave numbers over the channel's extents), cosines across y."""
+    g = mf.grid
+    lines = [a.greville() for a in g.axes]
+    lengths = [a.length for a in g.axes]
+    nb = g.nb
+    idx = np.asarray(mf.interior, dtype=np.int64)
+    ix, iy, iz = idx % nb[0], (idx // nb[0]) % nb[1], idx // (nb[0] * nb[1])
+    x, y, z = lines[0][ix], lines[1][iy], lines[2][iz]
+    rng = np.random.default_rng(seed)
+    B, P = [], []
+    for _ in range(count):
+        psi = np.zeros(idx.size)
+        for _ in range(6):
+            kx, ky, kz = rng.integers(1, 5), rng.integers(0, 4), rng.integers(1, 4)
+            ph = rng.uniform(0.0, 2.0 * np.pi, 2)
+            psi += rng.standard_normal() * (np.sin(2 * np.pi * kx * x / lengths[0] + ph[0])
+                                             * np.cos(np.pi * ky * y / lengths[1])
+                                             * np.sin(2 * np.pi * kz * z / lengths[2] + ph[1]))
+        p_d = torch.from_numpy(psi).to("cuda")
+        P.append(p_d)
+        B.append(dmf.apply(p_d))
+    return B, P
 
 
 def pressure_section(hip, torch, args):
@@ -536,20 +621,22 @@ def pressure_section(hip, torch, args):
     log(f"pressure: bump channel {mf.meta['shape']}, n={n}, {mf.meta['n_cut_cells']} cut cells ({build_s:.1f} s)")
     rng = np.random.default_rng(3000 + args.seed)
 
+    out = {"workload": "channel 128x64x64 elements, p=3, periodic in x and z (9 x 3.036 x 4.5), lower wall cut by "
+                       "a cosine bump (height 1, length 4); pressure operator E^T K E, matrix-free; a stream of "
+                       f"{args.c5_eval} right-hand sides b = A psi, the discrete divergences of random velocity "
+                       "fields grad psi (smooth random potentials psi); error against psi modulo constants",
+           "n": n, "n_cut_cells": mf.meta["n_cut_cells"], "min_cut_fraction": sig(mf.meta["min_cut_fraction"]),
+           "build_s": sig(build_s), "stream_timed": args.c5_eval}
+    B, PSI = divergence_stream(mf, dmf, torch, args.c5_eval, 3100 + args.seed)
+    out["jacobi"] = time_solver(hip, torch, lambda b: dmf.pcg(b, tol=TOL, maxiter=20000), B, PSI, True)
+    log(f"pressure, jacobi: {out['jacobi']}")
+    out["jacobi_batched"] = time_batched(hip, torch, dmf, B, PSI, True)
+    log(f"pressure, jacobi batched: {out['jacobi_batched']}")
+
     def rhs():
         b = rng.standard_normal(n)
         return torch.from_numpy(b - b.mean()).to("cuda")
 
-    out = {"workload": "channel 128x64x64 elements, p=3, periodic in x and z (9 x 3.036 x 4.5), lower wall cut by "
-                       "a cosine bump (height 1, length 4); pressure operator E^T K E, matrix-free; zero-sum "
-                       "Gaussian right-hand sides",
-           "n": n, "n_cut_cells": mf.meta["n_cut_cells"], "min_cut_fraction": sig(mf.meta["min_cut_fraction"]),
-           "build_s": sig(build_s), "stream_timed": args.c5_eval}
-    B = [rhs() for _ in range(args.c5_eval)]
-    out["jacobi"] = time_solver(hip, torch, lambda b: dmf.pcg(b, tol=TOL, maxiter=20000), B)
-    log(f"pressure, jacobi: {out['jacobi']}")
-    out["jacobi_batched"] = time_batched(hip, torch, dmf, B)
-    log(f"pressure, jacobi batched: {out['jacobi_batched']}")
     t0 = time.perf_counter()
     R, E = generate_pairs(dmf, [rhs() for _ in range(args.c5_pair_rhs)], (5, 20, 60, 150, 400))
     torch.cuda.synchronize()
@@ -571,15 +658,34 @@ def pressure_section(hip, torch, args):
                                   "loss_last20": sig(float(np.mean(rep.history[-20:])))}
     del R, E
     torch.cuda.empty_cache()
-    out["learned_multilevel"] = time_solver(hip, torch, lambda b: mlnet.fcg(dmf, b, tol=TOL, maxiter=20000), B)
+    out["learned_multilevel"] = time_solver(hip, torch, lambda b: mlnet.fcg(dmf, b, tol=TOL, maxiter=20000), B,
+                                            PSI, True)
     log(f"pressure, learned multilevel: {out['learned_multilevel']}")
-    out["learned_batched"] = time_learned_batched(hip, torch, mlnet, dmf, B)
+    out["learned_batched"] = time_learned_batched(hip, torch, mlnet, dmf, B, PSI, True)
     log(f"pressure, learned batched: {out['learned_batched']}")
     jac = out["jacobi"]["solves_per_s"]
     out["speedup_solves_per_s"] = sig(out["learned_multilevel"]["solves_per_s"] / jac)
     out["speedup_batched_jacobi"] = sig(out["jacobi_batched"]["solves_per_s"] / jac)
     out["speedup_learned_batched_vs_jacobi"] = sig(out["learned_batched"]["solves_per_s"] / jac)
+    # The ratio that isolates the network: batched learned against batched Jacobi.
+    out["ratio_learned_batched_vs_jacobi_batched"] = sig(out["learned_batched"]["solves_per_s"]
+                                                         / out["jacobi_batched"]["solves_per_s"])
+    out.update(_matched(hip, torch, dmf, mlnet, B, PSI, True))
+    log(f"pressure, accuracy-matched: {out['at_solution_error_1e-8']}")
     return out
+
+
+def _matched(hip, torch, dmf, net, B, ref, modulo_constant):
+    """Batched Jacobi-PCG and batched learned FCG each at the loosest
+    tolerance meeting the 1e-8 solution-error bar, and their ratio."""
+    jac = accuracy_matched(hip, torch, lambda Bk, t: dmf.pcg_batched(Bk, tol=t, maxiter=40000), B, ref,
+                           modulo_constant)
+    lrn = accuracy_matched(hip, torch, lambda Bk, t: net.fcg_batched(dmf, Bk, tol=t, maxiter=40000), B, ref,
+                           modulo_constant)
+    ratio = (sig(lrn["solves_per_s"] / jac["solves_per_s"]) if lrn.get("solves_per_s") and jac.get("solves_per_s")
+             else None)
+    return {"at_solution_error_1e-8": {"jacobi_batched": jac, "learned_batched": lrn,
+                                       "ratio_learned_vs_jacobi": ratio}}
 
 
 def learned_section(hip, torch, args):
@@ -652,18 +758,28 @@ def learned_section(hip, torch, args):
         "jacobi": lambda b: dmf.pcg(b, tol=TOL, maxiter=20000),
         "learned": lambda b: net.fcg(dmf, b, tol=TOL, maxiter=20000),
     }
+    # Reference solutions for the error figures: float64 Jacobi-PCG to 1e-12.
+    ref = []
+    for c in _chunks(B):
+        Xr, _ = dmf.pcg_batched(torch.stack(c, dim=1).contiguous(), tol=1e-12, maxiter=40000)
+        ref += [Xr[:, j].contiguous() for j in range(Xr.shape[1])]
     for method, solve in solvers.items():
-        out[method] = time_solver(hip, torch, solve, B)
+        out[method] = time_solver(hip, torch, solve, B, ref)
         log(f"learned section, {method}: {out[method]}")
-    out["jacobi_batched"] = time_batched(hip, torch, dmf, B)
+    out["jacobi_batched"] = time_batched(hip, torch, dmf, B, ref)
     log(f"learned section, jacobi batched: {out['jacobi_batched']}")
-    out["learned_batched"] = time_learned_batched(hip, torch, net, dmf, B)
+    out["learned_batched"] = time_learned_batched(hip, torch, net, dmf, B, ref)
     log(f"learned section, learned batched: {out['learned_batched']}")
+    out["error_reference"] = "float64 Jacobi-PCG to relative residual 1e-12"
     _, res, how = solve_learned_with_fallback(net, dmf, B[0], tol=TOL)
     out["fallback_check"] = {"path": how, "converged": res.status == hip.CC_CONVERGED}
     jac = out["jacobi"]["solves_per_s"]
     out["speedup_solves_per_s"] = sig(out["learned"]["solves_per_s"] / jac)
     out["speedup_learned_batched_vs_jacobi"] = sig(out["learned_batched"]["solves_per_s"] / jac)
+    out["ratio_learned_batched_vs_jacobi_batched"] = sig(out["learned_batched"]["solves_per_s"]
+                                                         / out["jacobi_batched"]["solves_per_s"])
+    out.update(_matched(hip, torch, dmf, net, B, ref, False))
+    log(f"learned section, accuracy-matched: {out['at_solution_error_1e-8']}")
     out["iteration_ratio"] = sig(out["jacobi"]["iterations_mean"] / out["learned"]["iterations_mean"])
     return out
 
@@ -719,8 +835,8 @@ def main(argv=None) -> int:
     ap.add_argument("--learned-eval", type=int, default=16, help="right-hand sides timed per method (1-16)")
     ap.add_argument("--ml-levels", type=int, default=5, help="grids of the multilevel learned preconditioner")
     ap.add_argument("--no-pressure", action="store_true", help="skip the bump-channel pressure-projection section")
-    ap.add_argument("--c5-eval", type=int, default=16,
-                    help="right-hand sides of the pressure stream timed per method (1, 2, 4, 8 or 16)")
+    ap.add_argument("--c5-eval", type=int, default=64,
+                    help="right-hand sides of the pressure stream timed per method (batched in groups of 16)")
     ap.add_argument("--c5-pair-rhs", type=int, default=30, help="pressure right-hand sides that give training pairs")
     ap.add_argument("--c5-steps", type=int, default=1000, help="Adam steps (8 pairs each), bump channel")
     ap.add_argument("--c5-lr", type=float, default=2e-3, help="Adam learning rate, bump channel")
@@ -823,6 +939,21 @@ def main(argv=None) -> int:
             den = torch.linalg.norm(Xref, dim=0)
             errors[m] = float((num / den).max().item())
 
+    head_matched = None
+    if not args.no_error_check:
+        best = max((r for r in runs.values() if r["converged"]), key=lambda r: r["solves_per_s"], default=None)
+        if best is not None:
+            prec = best["method"].split("+")[0]
+            use_rb = rb if best["method"].endswith("+rb") else None
+
+            def solve_c2(Bk, t):
+                return hip.pcg(Ad, Bk, tol=t, maxiter=40000, precision=prec, rb=use_rb)
+
+            head_matched = accuracy_matched(hip, torch, solve_c2, [Bd[:, c] for c in range(NRHS)],
+                                            [Xref[:, c] for c in range(NRHS)])
+            head_matched["method"] = best["method"]
+            log(f"headline at solution error 1e-8: {head_matched}")
+
     valid = {m: r for m, r in runs.items() if r["converged"]}
     head = max(valid.values(), key=lambda r: r["solves_per_s"]) if valid else next(iter(runs.values()))
     head_prec = head["method"].split("+")[0]
@@ -902,6 +1033,7 @@ def main(argv=None) -> int:
                "error_vs_tight_fp64": errors.get(m)}
             for m, r in runs.items()
         },
+        "value_at_solution_error_1e-8": head_matched,
         "roofline": roofline,
         "spmv_k1": spmv,
         "ekte_fused": ekte,

@@ -263,6 +263,92 @@ __global__ __launch_bounds__(kConvThreads) void k_conv3(GridArgs g, const uint16
   }
 }
 
+// The network's first layer (one input channel, u as single-channel float32
+// grids): out[v][co] = mask[v] sum_t W[t][0][co] u[v + off_t]. With a single
+// input channel the 27 taps are the GEMM's K dimension: two MFMAs (taps
+// 0..15 and 16..31, those past 26 zero) per row of 16 x-voxels, in place of
+// k_conv3's 27. The tile, the halo staging (channel 0 of the bf16 halo
+// planes) and the output transpose are k_conv3's.
+__global__ __launch_bounds__(kConvThreads) void k_conv3_first(GridArgs g, const float* __restrict__ u,
+                                                              const uint16_t* __restrict__ W,
+                                                              const uint8_t* __restrict__ mask,
+                                                              uint16_t* __restrict__ out, int tiles_z, size_t bstride,
+                                                              int nb, int ipb) {
+  __shared__ __align__(16) uint16_t halo[4 * kHaloPlane];
+  __shared__ __align__(16) float tile[kCz][16][20];
+  const int tid = threadIdx.x;
+  const int x0 = blockIdx.x * kCx, y0 = blockIdx.y * kCy;
+  const int it0 = blockIdx.z * ipb;
+  const int it1 = min(nb * tiles_z, it0 + ipb);
+  const int lane = tid & 63;
+  const int wv = tid >> 6;
+  const int row = lane % 16;
+  const int kg = lane / 16;
+  // B operands: W[t][0][co = row] for t = 16 j + 4 kg + i (zero past 26).
+  v4s bw[2];
+#pragma unroll
+  for (int j = 0; j < 2; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int t = 16 * j + 4 * kg + i;
+      bw[j][i] = t < 27 ? (short)W[(size_t)t * 256 + row] : (short)0;
+    }
+  uint4 reg[kHaloPerThread];
+  const size_t bstride1 = bstride / 16;
+  if (it0 < it1) {
+    const int b = it0 / tiles_z;
+    halo_fetch(g, nullptr, u + (size_t)b * bstride1, x0, y0, (it0 - b * tiles_z) * kCz, reg);
+  }
+  for (int it = it0; it < it1; ++it) {
+    const int b = it / tiles_z;
+    const int z0 = (it - b * tiles_z) * kCz;
+    const size_t off = (size_t)b * bstride;
+    __syncthreads();
+    halo_store(halo, reg);
+    __syncthreads();
+    if (it + 1 < it1) {
+      const int bn = (it + 1) / tiles_z;
+      halo_fetch(g, nullptr, u + (size_t)bn * bstride1, x0, y0, (it + 1 - bn * tiles_z) * kCz, reg);
+    }
+    const int vx = x0 + row, vz = z0 + wv;
+#pragma unroll
+    for (int r = 0; r < kCy; ++r) {
+      v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        v4s a;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int t = 16 * j + 4 * kg + i;
+          const int tt = t < 27 ? t : 26;
+          const int hv = ((wv + tt / 9) * kHy + (r + (tt / 3) % 3)) * kHx + (row + tt % 3);
+          const short v = (short)halo[hv * 4];  // channel 0 lives in plane 0, slot 0
+          a[i] = t < 27 ? v : (short)0;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw[j], acc, 0, 0, 0);
+      }
+      // acc[i] = D[voxel 4 kg + i][co = row]: transpose through this wave's
+      // slice so that each lane holds 4 channels of voxel `row`.
+#pragma unroll
+      for (int i = 0; i < 4; ++i) tile[wv][4 * kg + i][row] = acc[i];
+      const int vy = y0 + r;
+      if (vx < g.nx && vy < g.ny && vz < g.nz) {
+        const size_t v = ((size_t)vz * g.ny + vy) * g.nx + vx;
+        const bool live = mask[v] != 0;
+        const float4 tv = *reinterpret_cast<const float4*>(&tile[wv][row][4 * kg]);
+        const float tf[4] = {tv.x, tv.y, tv.z, tv.w};
+        uint16_t h[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) h[i] = f2bf(live ? tf[i] : 0.f);
+        uint2 raw;
+        raw.x = (uint32_t)h[0] | ((uint32_t)h[1] << 16);
+        raw.y = (uint32_t)h[2] | ((uint32_t)h[3] << 16);
+        *reinterpret_cast<uint2*>(out + off + v * 16 + 4 * kg) = raw;
+      }
+    }
+  }
+}
+
 static dim3 conv_grid(int nx, int ny, int nz, int nb = 1, int ipb = 1) {
   const int items = ((nz + kCz - 1) / kCz) * nb;
   return dim3((unsigned)((nx + kCx - 1) / kCx), (unsigned)((ny + kCy - 1) / kCy), (unsigned)((items + ipb - 1) / ipb));
@@ -490,8 +576,8 @@ int cnn_forward_ch0(const cc_cnn_desc& net, const float* u, uint16_t* act1, uint
   const dim3 blocks = conv_grid(net.nx, net.ny, net.nz, nb, ipb);
   if (blocks.z > 65535 || blocks.y > 65535) return CC_E_ARG;
   const size_t bs = (size_t)net.nx * net.ny * net.nz * 16;
-  hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, nullptr, u, net.weights[0], net.mask, nullptr,
-                     act1, nullptr, 0, tz, bs, nb, ipb);
+  hipLaunchKernelGGL(k_conv3_first, blocks, dim3(kConvThreads), 0, s, g, u, net.weights[0], net.mask, act1, tz, bs, nb,
+                     ipb);
   hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act1, nullptr, net.weights[1], net.mask, nullptr,
                      act2, nullptr, 0, tz, bs, nb, ipb);
   hipLaunchKernelGGL(k_conv3<0>, blocks, dim3(kConvThreads), 0, s, g, act2, nullptr, net.weights[2], net.mask, nullptr,
