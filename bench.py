@@ -374,6 +374,21 @@ def batched_section(hip, torch, steps: int, warmup: int, seed: int = 0, systems=
            "build_s": sig(build_s)}
     if cpu_ref is not None:
         out["cpu_reference"] = cpu_ref
+    # The same systems with H and K assembled on the device.
+    from cutcell.asm2d_gpu import c3_batch_gpu
+
+    c3_batch_gpu(n_elements=12, n_cuts=2, n_slivers=1, seed=seed)  # warm-up (library load, kernels)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu_systems = c3_batch_gpu(seed=seed)
+    torch.cuda.synchronize()
+    gpu_build_s = time.perf_counter() - t0
+    same = all(g.A.shape == h.A.shape and g.A.nnz == h.A.nnz and np.array_equal(g.A.indices, h.A.indices)
+               and abs(g.A - h.A).max() <= 1e-11 * abs(h.A).max() for g, h in zip(gpu_systems, systems))
+    out["build_device_assembly_s"] = sig(gpu_build_s)
+    out["device_assembly_matches_host"] = bool(same)
+    log(f"batched: device-assembled build {gpu_build_s:.2f} s, same systems as the host build: {same}")
+    del gpu_systems
     Xref = torch.empty_like(B)
     bs.solve_device(B, Xref, tol=1e-13, maxiter=40000, precision="fp64")
     torch.cuda.synchronize()
@@ -994,6 +1009,7 @@ def main(argv=None) -> int:
         log("batched section: 241 uneven 2D systems in one persistent launch ...")
         batched = batched_section(hip, torch, max(1, args.steps), args.warmup, systems=c3_systems, cpu_ref=c3_cpu)
         batched["build_s"] = sig(c3_build_s)
+        batched["build_s_processes"] = min(16, os.cpu_count() or 1)
 
     learned = None
     if not args.no_learned:

@@ -105,6 +105,9 @@ _SIGNATURES = {
     "cc_pcg_ekte": ([_P, _I32, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
     "cc_gmres_workspace_bytes": ([_I32, _I32], _SZ),
     "cc_gmres_jacobi_f64": ([_I32, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
+    "cc_cut2d_gram": ([_I32, _P, _I32, _I32, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_asm2d_count": ([_I32, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "cc_asm2d_fill": ([_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_build_count": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_build_fill": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),

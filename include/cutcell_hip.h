@@ -194,6 +194,40 @@ int cc_gmres_jacobi_f64(int32_t n, const int32_t* indptr, const int32_t* indices
                         int32_t preconditioner, void* workspace, size_t workspace_bytes, cc_gmres_result* result,
                         cc_stream_t stream);
 
+/* 2D cut-cell assembly on the device (replaces the per-cell loops of
+ * assemble_oblique_cut_matrices, code/oblique_cut_sbp.py:271-433, for the
+ * mass H and stiffness K): N x N cells of degree-p tensor B-splines on open
+ * uniform knots, the unit square cut by normal . x >= offset. */
+typedef struct cc_cut2d_cell {
+  int32_t ix, iy;            /* the cell */
+  double normal_x, normal_y; /* the half plane of this cell's system */
+  double offset;
+} cc_cut2d_cell;
+/* Local Gram blocks of ncells cut cells (any systems sharing N, p and q):
+ * the square clipped against the half plane (Sutherland-Hodgman, tolerance
+ * 1e-14, duplicates within 1e-13 dropped), fanned from its first vertex,
+ * each triangle by the q x q collapsed Gauss rule of gauss_x / gauss_w
+ * (Gauss-Legendre on [-1, 1]); H_blocks / K_blocks[cell][a][b] over the
+ * (p+1)^2 local functions a = ay (p+1) + ax; areas[cell] = the clipped
+ * area, or 0 when the polygon has fewer than 3 vertices or area <= 1e-16
+ * (such a cell contributes nothing). degree 1..5. */
+int cc_cut2d_gram(int32_t ncells, const cc_cut2d_cell* cells, int32_t n_elements, int32_t degree, int32_t q,
+                  const double* gauss_x, const double* gauss_w, double* H_blocks, double* K_blocks, double* areas,
+                  cc_stream_t stream);
+/* Row counts and row pointers (n = (N+p)^2 + 1) of H and K for one system:
+ * entry (i, j) exists when a contributing cell (cell_class[iy * N + ix]: 0
+ * empty, 1 full, 2 cut, 3 cut but contributing nothing) supports both i and
+ * j. */
+int cc_asm2d_count(int32_t n_elements, int32_t degree, const int8_t* cell_class, int32_t* row_counts,
+                   int32_t* indptr, cc_stream_t stream);
+/* Columns (ascending) and values of H and K: full cells from the 1D element
+ * forms elem_mass / elem_stiff ([N][p+1][p+1], the same basis in x and y),
+ * cut cells from the blocks at cut_slot[cell]; each entry summed over its
+ * covering cells in a fixed order (no atomics). */
+int cc_asm2d_fill(int32_t n_elements, int32_t degree, const int8_t* cell_class, const int32_t* cut_slot,
+                  const double* elem_mass, const double* elem_stiff, const double* H_blocks, const double* K_blocks,
+                  const int32_t* indptr, int32_t* indices, double* H_vals, double* K_vals, cc_stream_t stream);
+
 /* Row-block form of a CSR matrix (no reference counterpart: a derived layout
  * for products with several right-hand sides, built once per matrix). Rows
  * are taken in groups of CC_RB_ROWS consecutive rows; group g stores the
