@@ -823,6 +823,35 @@ def operator_probe(hip, torch, args, reps: int = 30):
     return out
 
 
+def prec_probe(hip, torch, args, iters: int = 40):
+    """Batched flexible CG with an untrained multilevel network (its kernels
+    and their cost do not depend on the weights): 16 columns, a fixed number
+    of iterations, per-iteration times of the whole loop, the operator apply
+    and the network pass (for a kernel trace of the learned path)."""
+    from cutcell.learned import MultilevelCNN
+    from cutcell.matfree import DeviceMatrixFree
+
+    mf = build_sphere_operator(args.seed, args.learned_n) if args.prec_probe == "sphere" \
+        else build_channel_operator()
+    dmf = DeviceMatrixFree(mf)
+    periodic = tuple(a.periodic for a in mf.grid.axes)
+    net = MultilevelCNN(mf.grid.nb, mf.interior, periodic=periodic, levels=args.ml_levels, seed=args.seed)
+    rng = np.random.default_rng(args.seed)
+    Bk = torch.from_numpy(rng.standard_normal((mf.n, 16))).to("cuda").contiguous()
+    run = lambda: net.fcg_batched(dmf, Bk, tol=0.0, maxiter=iters)
+    run()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    apply_ms, apply_cnt, prec_ms, prec_cnt = profiled_once(hip, run)
+    return {"operator": args.prec_probe, "n": mf.n, "grids": [list(d) for d in net.dims], "columns": 16,
+            "iterations": iters, "ms_per_iteration": sig(1e3 * dt / iters),
+            "apply_avg_us": sig(1e3 * apply_ms / max(apply_cnt, 1)),
+            "network_pass_avg_us": sig(1e3 * prec_ms / max(prec_cnt, 1))}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -863,6 +892,10 @@ def main(argv=None) -> int:
     ap.add_argument("--operator-probe", choices=("sphere", "channel"), default=None,
                     help="only time the matrix-free operator of the sphere box or the bump channel (16-column and "
                          "single-column applies, for a kernel trace) and print one JSON line")
+    ap.add_argument("--prec-probe", choices=("sphere", "channel"), default=None,
+                    help="only run 40 iterations of 16-column flexible CG with an untrained multilevel network on "
+                         "the sphere box or the bump channel (for a kernel trace of the learned path) and print "
+                         "one JSON line")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -873,7 +906,7 @@ def main(argv=None) -> int:
     # CPU legs first: their worker processes are forks, made before the GPU
     # is initialised in this process.
     cpu, c3_systems, c3_cpu, c3_build_s = None, None, None, None
-    if not args.operator_probe:
+    if not (args.operator_probe or args.prec_probe):
         log("building the 3D planar-cut system (p=3, ~81k unknowns) ...")
         t0 = time.perf_counter()
         system, ekte_parts = build_workload(args.seed, args.target)
@@ -909,6 +942,9 @@ def main(argv=None) -> int:
     hip.load_library()
     if args.operator_probe:
         print(json.dumps(operator_probe(hip, torch, args)))
+        return 0
+    if args.prec_probe:
+        print(json.dumps(prec_probe(hip, torch, args)))
         return 0
 
     A = system.A

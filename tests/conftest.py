@@ -29,3 +29,6 @@ def pytest_terminal_summary(terminalreporter):
         hist = [parity.GAPS.count(g) for g in range(max(parity.GAPS) + 1)]
         terminalreporter.write_line(f"parity stop gaps (count of |gap| = 0, 1, ...): {hist}, "
                                     f"largest {max(parity.GAPS)} of {len(parity.GAPS)} trajectories")
+    for gap, rng, it in parity.SPREAD_CASES:
+        terminalreporter.write_line(f"stop gap {gap} accepted: HIP stop {it}, oracle stops {rng[0]}..{rng[1]} "
+                                    "over reorderings of the same system")

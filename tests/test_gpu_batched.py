@@ -24,7 +24,7 @@ def batch():
 def _check_against_oracle(s, x, r, tol, rel_x=1e-6):
     x_or, it_or, st_or, rel_or, h_or = oracle.pcg_gpu_loop(s.A, s.b, tol=tol, history=True)
     assert r.status == 1 == st_or
-    check_stop_gap(r.iterations, it_or)
+    check_stop_gap(r.iterations, it_or, system=(s.A, s.b), tol=tol)
     if r.iterations < it_or:
         assert h_or[r.iterations] <= STOP_SLACK * tol
     assert r.recursive_rel <= tol

@@ -182,6 +182,45 @@ def test_summation_order_moves_the_stop_by_rounding_only():
             check_trajectory(h, it, h2, it2, 1e-8)
 
 
+def test_reordering_spread_of_a_c3_system_exceeds_three():
+    """On one 2D oblique p=5 system of the batched test set (N = 12, seed 5,
+    the sixth system) the float64 oracle's own stop moves over 239..244 under
+    random reorderings: rounding alone can move a stop by more than the
+    3-iteration bar, which is why ``parity.check_stop_gap`` falls back to the
+    oracle's reordering range there. Planting a wrong recurrence in the
+    oracle (beta from r.r) takes the stop far outside that range (200
+    reorderings here: 239..244; the system differs in its last bits between
+    hosts, so the range may move by one)."""
+    from cutcell.workloads import c3_batch
+    from parity import oracle_stop_range
+
+    s = c3_batch(n_elements=12, n_cuts=4, degrees=(3, 4, 5), n_slivers=3, seed=5, jobs=1)[5]
+    _, it, st, _ = oracle.pcg_gpu_loop(s.A, s.b, tol=1e-8)
+    lo, hi = oracle_stop_range(s.A, s.b, 1e-8)
+    assert st == oracle.CONVERGED and lo <= it <= hi
+    assert hi - lo > 3
+    A = s.A.tocsr()
+    d = A.diagonal()
+    x = np.zeros_like(s.b)
+    r = s.b.copy()
+    p = r / d
+    rr = r @ r
+    bn = np.linalg.norm(s.b)
+    itb = 0
+    for k in range(1, 5001):
+        q = A @ p
+        a = (r @ (r / d)) / (p @ q)
+        x += a * p
+        r -= a * q
+        itb = k
+        if np.linalg.norm(r) / bn <= 1e-8:
+            break
+        rr_new = r @ r
+        p = r / d + (rr_new / rr) * p
+        rr = rr_new
+    assert not (lo - 1 <= itb <= hi + 1)
+
+
 def test_parity_criterion_rejects_a_wrong_recurrence():
     """A planted mistake (beta from the unpreconditioned r.r) must fail the check."""
     from parity import check_trajectory
