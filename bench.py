@@ -892,6 +892,9 @@ def main(argv=None) -> int:
     ap.add_argument("--operator-probe", choices=("sphere", "channel"), default=None,
                     help="only time the matrix-free operator of the sphere box or the bump channel (16-column and "
                          "single-column applies, for a kernel trace) and print one JSON line")
+    ap.add_argument("--net-blocks", type=int, default=0, help="blocks per network-pass launch (0: default)")
+    ap.add_argument("--net-legacy", action="store_true",
+                    help="run the learned network as per-level four-layer launches (A/B timing)")
     ap.add_argument("--prec-probe", choices=("sphere", "channel"), default=None,
                     help="only run 40 iterations of 16-column flexible CG with an untrained multilevel network on "
                          "the sphere box or the bump channel (for a kernel trace of the learned path) and print "
@@ -940,6 +943,10 @@ def main(argv=None) -> int:
         print("bench.py needs a ROCm GPU", file=sys.stderr)
         return 2
     hip.load_library()
+    if args.net_blocks or args.net_legacy:
+        from cutcell import learned
+
+        hip._check(learned._lib().cc_net_set_config(args.net_blocks, int(args.net_legacy)), "net_set_config")
     if args.operator_probe:
         print(json.dumps(operator_probe(hip, torch, args)))
         return 0

@@ -40,6 +40,11 @@ int cnn_forward_layers(const cc_cnn_desc& net, uint16_t* act0, uint16_t* act1, u
 // as a single-channel float32 grid (cnn.hip).
 int cnn_forward_ch0(const cc_cnn_desc& net, const float* u, uint16_t* act1, uint16_t* act2, uint16_t* act3, float* y,
                     hipStream_t s, int nb = 1);
+// The same four layers on every level with a network, in three launches
+// (netpass.hip); net_pass_legacy() selects the per-level layers above.
+int net_pass(const cc_mlcnn_desc& ml, float* const* u, uint16_t* const* g1, uint16_t* const* g2, float* const* y,
+             int nb, hipStream_t s);
+bool net_pass_legacy();
 
 // Grid transfers (mg.hip); nb grids of a batch, each bs elements after the last.
 int mg_restrict(const MgDims& g, const float* fine, int fs, float* coarse, int cs, hipStream_t s, int nb = 1,
@@ -465,6 +470,24 @@ static int ml_setup(const cc_mlcnn_desc& ml, const MlBuffers& b, hipStream_t s) 
   return 0;
 }
 
+// y[l] = C_l^T C_l u[l] on every level with a network, nb grids per level.
+static int level_networks(const cc_mlcnn_desc& ml, const MlBuffers& mb, int nb, hipStream_t s) {
+  if (!net_pass_legacy()) {
+    uint16_t* g1[CC_ML_MAX_LEVELS] = {};
+    uint16_t* g2[CC_ML_MAX_LEVELS] = {};
+    for (int l = 0; l < ml.levels; ++l) {
+      g1[l] = mb.act[l][1];
+      g2[l] = mb.act[l][2];
+    }
+    return net_pass(ml, mb.u, g1, g2, mb.y, nb, s);
+  }
+  for (int l = 0; l < ml.levels; ++l) {
+    if (!ml.has_net[l]) continue;
+    CC_CHECK(cnn_forward_ch0(ml.net[l], mb.u[l], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s, nb));
+  }
+  return 0;
+}
+
 // z = D^-1 r + S sum_l Q_l C_l^T C_l Q_l^T S r; partial z.q when q is given.
 // (Running the coarse chain on a second stream beside the fine level's
 // network was measured slower: the fine convolutions fill the GPU, and the
@@ -480,10 +503,7 @@ static int ml_prec(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double* r
     mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
     CC_CHECK(mg_restrict(g[l], mb.u[l - 1], 1, mb.u[l], 1, s));
   }
-  for (int l = 0; l < L; ++l) {
-    if (!ml.has_net[l]) continue;
-    CC_CHECK(cnn_forward_ch0(ml.net[l], mb.u[l], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s));
-  }
+  CC_CHECK(level_networks(ml, mb, 1, s));
   const float* src = mb.y[L - 1];
   for (int l = L - 1; l >= 1; --l) {
     CC_CHECK(mg_prolong(g[l], src, 1, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 1, mb.t[l - 1], s));
@@ -856,10 +876,7 @@ static int ml_prec_k(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double*
     mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
     CC_CHECK(mg_restrict(g[l], mb.u[l - 1], 1, mb.u[l], 1, s, K, vox[l - 1], vox[l]));
   }
-  for (int l = 0; l < L; ++l) {
-    if (!ml.has_net[l]) continue;
-    CC_CHECK(cnn_forward_ch0(ml.net[l], mb.u[l], mb.act[l][1], mb.act[l][2], mb.act[l][3], mb.y[l], s, K));
-  }
+  CC_CHECK(level_networks(ml, mb, K, s));
   const float* src = mb.y[L - 1];
   for (int l = L - 1; l >= 1; --l) {
     CC_CHECK(mg_prolong(g[l], src, 1, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 1, mb.t[l - 1], s, K, vox[l],

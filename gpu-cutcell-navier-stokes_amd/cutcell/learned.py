@@ -75,6 +75,9 @@ def _bind(lib):
         "cc_fcg_sumfact_ml": [P, P, P, P, D, I, I, P, SZ, P, P, P],
         "cc_fcg_sumfact_ml_k": [P, P, I, P, P, D, I, I, P, SZ, P, P],
         "cc_mfma_probe": [P, P, P, P],
+        "cc_mfma32_probe": [P, P, P, P],
+        "cc_net_set_config": [I, I],
+        "cc_net_level_forward": [P, I, P, P, P, SZ, P],
         "cc_cnn_set_config": [I],
         "cc_cnn_conv3_forward": [I, I, I, P, P, P, P, P, I, P, P, P],
         "cc_cnn_flip_weights": [P, P, P],
@@ -98,6 +101,7 @@ def _bind(lib):
         "cc_mlcnn_precondition_workspace_bytes": [P],
         "cc_fcg_sumfact_ml_workspace_bytes": [P, P],
         "cc_fcg_sumfact_ml_k_workspace_bytes": [P, P, I],
+        "cc_net_level_workspace_bytes": [P, I],
     }.items():
         fn = getattr(lib, name)
         fn.argtypes = args

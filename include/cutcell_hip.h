@@ -577,6 +577,25 @@ int cc_fcg_sumfact_ml_k(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32
                         double tol, int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
                         cc_pcg_result* results, cc_stream_t stream);
 
+/* ---- Network pass of the multilevel preconditioner on gfx950 (no
+ * reference counterpart). The preconditioners above evaluate every level's
+ * C^T C in three launches that serve all levels and all grids of a batch:
+ * the first two layers fused (the W0 layer computed on each tile's halo in
+ * LDS), the flip W1 layer, and the flip W0 layer written as channel 0 in
+ * float32; all on the 16x16x32 bf16 MFMA with fp32 accumulation.
+ * cc_mfma32_probe: D = A B for bf16 A (16 x 32) and B (32 x 16), row-major,
+ * fp32 D (16 x 16), the lane layout those kernels rely on.
+ * cc_net_set_config: blocks per pass launch (0 = 512, two per CU) and
+ * legacy = 1 to run the four per-level layer launches of
+ * cc_cnn_conv3_forward instead.
+ * cc_net_level_forward: y = C^T C u (channel 0, masked) on nb grids of one
+ * level, u and y float32 [nb][voxel]. */
+int cc_mfma32_probe(const uint16_t* A, const uint16_t* B, float* D, cc_stream_t stream);
+int cc_net_set_config(int32_t blocks, int32_t legacy);
+size_t cc_net_level_workspace_bytes(const cc_cnn_desc* net, int32_t nb);
+int cc_net_level_forward(const cc_cnn_desc* net, int32_t nb, const float* u, float* y, void* workspace,
+                         size_t workspace_bytes, cc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
