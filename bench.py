@@ -281,6 +281,29 @@ def spmv_bandwidth(hip, torch, Ad, reps: int = 20):
         out[prec] = {"avg_us": sig(us), "bytes": int(nbytes), "GBps": sig(nbytes / (us * 1e-6) / 1e9),
                      "frac_of_8TBps": sig(nbytes / (us * 1e-6) / 1e9 / HBM_PEAK_GBS),
                      "launch": {k: v for k, v in cfg.items() if k != "ms"}}
+    # The same product from the run encoding of the rows (runs of consecutive
+    # columns in place of one index per nonzero): its own stored bytes, and
+    # the rate in the CSR bytes above (the product it replaces).
+    R = hip.DeviceRuns.from_csr(Ad)
+    out["runs"] = {"nruns": R.nruns, "nnz_per_run": sig(Ad.nnz / max(R.nruns, 1))}
+    for prec, dt, vb in (("fp64", torch.float64, 8), ("fp32", torch.float32, 4)):
+        x = torch.randn(Ad.n, dtype=dt, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            hip.spmv_runs(R, x, y)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            hip.spmv_runs(R, x, y)
+        e1.record()
+        torch.cuda.synchronize()
+        us = 1e3 * e0.elapsed_time(e1) / reps
+        own = R.bytes(vb) + 2 * Ad.n * vb
+        csr = Ad.nnz * (vb + 4) + 4 * (Ad.n + 1) + 2 * Ad.n * vb
+        out["runs"][prec] = {"avg_us": sig(us), "bytes": int(own), "GBps": sig(own / (us * 1e-6) / 1e9),
+                             "frac_of_8TBps": sig(own / (us * 1e-6) / 1e9 / HBM_PEAK_GBS),
+                             "csr_bytes": int(csr), "frac_b_spmv": sig(csr / (us * 1e-6) / 1e9 / HBM_PEAK_GBS)}
     return out
 
 

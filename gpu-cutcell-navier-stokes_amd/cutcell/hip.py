@@ -113,6 +113,10 @@ _SIGNATURES = {
     "cc_rb_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),
     "cc_spmm_rb_f64": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "cc_spmm_rb_f32": ([_P, _I32, _P, _P, _P], ctypes.c_int),
+    "cc_runs_count": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_runs_fill": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_spmv_runs_f64": ([_I32, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_spmv_runs_f32": ([_I32, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_pcg_jacobi_f64_rb": (
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int,
     ),
@@ -270,6 +274,54 @@ class DeviceRowBlocks:
         """Stored bytes one product reads: group offsets, union columns and
         the padded values of one precision."""
         return int(4 * (self.ngroups + 1) + self.nu * (4 + RB_ROWS * value_bytes))
+
+
+@dataclass
+class DeviceRuns:
+    """Run encoding of a device CSR matrix's rows (``cc_runs_count`` /
+    ``cc_runs_fill``): per row its runs of consecutive column indices, 8
+    bytes each (first column; offset within the row << 8 | length), in place
+    of one 4-byte index per nonzero. The values stay the CSR arrays."""
+
+    n: int
+    nruns: int
+    rptr: object
+    runs: object
+    csr: DeviceCSR
+
+    @staticmethod
+    def from_csr(A: DeviceCSR) -> "DeviceRuns":
+        lib = require_gpu()
+        torch = _torch()
+        dev = A.vals.device
+        rptr = torch.empty(A.n + 1, dtype=torch.int32, device=dev)
+        nr = ctypes.c_int64()
+        _check(lib.cc_runs_count(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(rptr), ctypes.addressof(nr), _stream()),
+               "runs_count (column indices must ascend strictly within rows)")
+        nr = int(nr.value)
+        runs = torch.empty(2 * max(nr, 1), dtype=torch.int32, device=dev)
+        _check(lib.cc_runs_fill(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(rptr), _ptr(runs), _stream()), "runs_fill")
+        return DeviceRuns(A.n, nr, rptr, runs, A)
+
+    def bytes(self, value_bytes: int = 8) -> int:
+        """Bytes one product reads from the matrix: the values, the runs and
+        both row pointers."""
+        return int(self.csr.nnz * value_bytes + 8 * self.nruns + 8 * (self.n + 1))
+
+
+def spmv_runs(R: DeviceRuns, x, y=None):
+    """y = A x from the run encoding (float64 or float32 by x's dtype)."""
+    lib = require_gpu()
+    torch = _torch()
+    if y is None:
+        y = torch.empty_like(x)
+    if x.dtype == torch.float64:
+        _check(lib.cc_spmv_runs_f64(R.n, _ptr(R.rptr), _ptr(R.runs), _ptr(R.csr.indptr), _ptr(R.csr.vals), _ptr(x),
+                                    _ptr(y), _stream()), "spmv_runs_f64")
+    else:
+        _check(lib.cc_spmv_runs_f32(R.n, _ptr(R.rptr), _ptr(R.runs), _ptr(R.csr.indptr), _ptr(R.csr.ensure_f32()),
+                                    _ptr(x), _ptr(y), _stream()), "spmv_runs_f32")
+    return y
 
 
 def spmm_rb(rb: DeviceRowBlocks, X, Y=None):

@@ -267,6 +267,27 @@ int cc_rb_set_config(int32_t k, int32_t value_bytes, int32_t nz_lanes, int32_t g
  * row-block form (the same product as cc_spmm_csr_f64 / _f32). */
 int cc_spmm_rb_f64(const cc_rb_desc* rb, int32_t k, const double* X, double* Y, cc_stream_t stream);
 int cc_spmm_rb_f32(const cc_rb_desc* rb, int32_t k, const float* X, float* Y, cc_stream_t stream);
+/* Run encoding of sorted column lists (no reference counterpart; it
+ * replaces the index stream of the cuSPARSE product A_gpu @ p,
+ * code/fast_solver_backends.py:250). A unit u (a CSR row) holds entries
+ * uptr[u] .. uptr[u+1]-1 of cols, strictly ascending; it is stored as its
+ * runs of consecutive columns, two uint32 per run: the first column, and
+ * (offset of the run's first entry within the unit) << 8 | length (1..255).
+ * cc_runs_count writes the run offsets rptr [nunits + 1] (device) and the
+ * total to *nruns (host; synchronises; CC_E_ARG if a unit's columns do not
+ * ascend strictly or it has 2^24 entries or more); cc_runs_fill writes the
+ * runs [2 * nruns]. */
+int cc_runs_count(int32_t nunits, const int32_t* uptr, const int32_t* cols, int32_t* rptr, int64_t* nruns,
+                  cc_stream_t stream);
+int cc_runs_fill(int32_t nunits, const int32_t* uptr, const int32_t* cols, const int32_t* rptr, uint32_t* runs,
+                 cc_stream_t stream);
+/* y = A x from the runs of A's rows and its CSR values (indptr, vals): the
+ * same product as cc_spmv_csr_f64 / _f32 (code/fast_solver_backends.py:250),
+ * each row's sum taken run by run. */
+int cc_spmv_runs_f64(int32_t n, const int32_t* rptr, const uint32_t* runs, const int32_t* indptr, const double* vals,
+                     const double* x, double* y, cc_stream_t stream);
+int cc_spmv_runs_f32(int32_t n, const int32_t* rptr, const uint32_t* runs, const int32_t* indptr, const float* vals,
+                     const float* x, float* y, cc_stream_t stream);
 /* cc_pcg_jacobi_f64 / cc_pcg_jacobi_mixed with the CG products taken from
  * the row-block form rb of the same matrix (rb->gval for float64,
  * rb->gval32 for the mixed inner solve); the diagonal and the float64 true
