@@ -147,17 +147,28 @@ __device__ inline void load_rb_vals<double>(const double* __restrict__ p, double
   v[3] = b.y;
 }
 
-template <typename MT, typename VT, int K, int NZ, int CPL, int UNR>
+// RUNS: the union columns come from their runs (grptr, grun: first column,
+// offset << 8 | length per run of consecutive columns, see runs.hip) in
+// place of gcol. A team is then one wave (T = 64); per group it expands the
+// runs into the wave's LDS slice (in chunks of kRbChunk columns) and the
+// slots read their columns from there. The values and the summation order
+// are those of the gcol product.
+constexpr int kRbChunk = 1024;
+
+template <typename MT, typename VT, int K, int NZ, int CPL, int UNR, bool RUNS>
 __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __restrict__ gptr,
                                                  const int* __restrict__ gcol, const MT* __restrict__ gval,
                                                  const VT* __restrict__ X, VT* __restrict__ Y,
-                                                 double* __restrict__ partial, const ColState* __restrict__ st) {
+                                                 double* __restrict__ partial, const ColState* __restrict__ st,
+                                                 const int* __restrict__ grptr, const uint32_t* __restrict__ grun) {
   constexpr int KL = K / CPL;
   constexpr int T = KL * NZ;
   constexpr int TPW = 64 / T;
+  static_assert(!RUNS || T == 64, "the run-driven product takes one wave per group");
   using AT = typename Accum<VT>::type;
   __shared__ double smem[4 * 16];
   __shared__ int s_rows[2];
+  __shared__ int colbuf[RUNS ? 4 : 1][RUNS ? kRbChunk : 1];
   if (all_stopped<K>(st)) return;
   const int lane = threadIdx.x & 63;
   const int c0 = (lane % KL) * CPL;
@@ -187,43 +198,65 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __res
   for (int g = g0 + team; g < g1; g += tpb) {
     const int beg = gptr[g];
     const int len = gptr[g + 1] - beg;
-    const int* __restrict__ cp = gcol + beg;
     const MT* __restrict__ vp = gval + (size_t)beg * kRb;
     AT acc[kRb][CPL];
 #pragma unroll
     for (int r = 0; r < kRb; ++r)
 #pragma unroll
       for (int m = 0; m < CPL; ++m) acc[r][m] = 0;
-    int j = zs;
-    if constexpr (UNR == 4) {
-      for (; j + 3 * NZ < len; j += 4 * NZ) {
-        int q[4];
-        MT v[4][kRb];
-        VT x[4][CPL];
+    // Union columns [ja, jb) with column j at cp[j].
+    auto columns = [&](const int* cp, int ja, int jb) {
+      int j = ja + zs;
+      if constexpr (UNR == 4) {
+        for (; j + 3 * NZ < jb; j += 4 * NZ) {
+          int q[4];
+          MT v[4][kRb];
+          VT x[4][CPL];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          q[u] = cp[j + u * NZ];
-          load_rb_vals<MT>(vp + (size_t)(j + u * NZ) * kRb, v[u]);
+          for (int u = 0; u < 4; ++u) {
+            q[u] = cp[j + u * NZ];
+            load_rb_vals<MT>(vp + (size_t)(j + u * NZ) * kRb, v[u]);
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) vload<VT, CPL>(X + (size_t)q[u] * K + c0, x[u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int r = 0; r < kRb; ++r)
+#pragma unroll
+              for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)v[u][r] * (AT)x[u][m];
         }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) vload<VT, CPL>(X + (size_t)q[u] * K + c0, x[u]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int r = 0; r < kRb; ++r)
-#pragma unroll
-            for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)v[u][r] * (AT)x[u][m];
       }
-    }
-    for (; j < len; j += NZ) {
-      MT v[kRb];
-      VT x[CPL];
-      load_rb_vals<MT>(vp + (size_t)j * kRb, v);
-      vload<VT, CPL>(X + (size_t)cp[j] * K + c0, x);
+      for (; j < jb; j += NZ) {
+        MT v[kRb];
+        VT x[CPL];
+        load_rb_vals<MT>(vp + (size_t)j * kRb, v);
+        vload<VT, CPL>(X + (size_t)cp[j] * K + c0, x);
 #pragma unroll
-      for (int r = 0; r < kRb; ++r)
+        for (int r = 0; r < kRb; ++r)
 #pragma unroll
-        for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)v[r] * (AT)x[m];
+          for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)v[r] * (AT)x[m];
+      }
+    };
+    if constexpr (RUNS) {
+      int* slice = colbuf[wave];
+      const int rb = grptr[g], re = grptr[g + 1];
+      for (int a = 0; a < len; a += kRbChunk) {
+        const int b = min(len, a + kRbChunk);
+        for (int k = rb + lane; k < re; k += 64) {
+          const uint2 rr = *reinterpret_cast<const uint2*>(grun + 2 * (size_t)k);
+          const int off = (int)(rr.y >> 8), ln = (int)(rr.y & 255u);
+          const int lo = max(off, a), hi = min(off + ln, b);
+          for (int e = lo; e < hi; ++e) slice[e - a] = (int)rr.x + (e - off);
+        }
+        // The slice is written and read by this wave only; its LDS accesses
+        // complete in order.
+        __builtin_amdgcn_wave_barrier();
+        columns(slice - a, a, b);
+        __builtin_amdgcn_wave_barrier();
+      }
+    } else {
+      columns(gcol + beg, 0, len);
     }
 #pragma unroll
     for (int o = KL; o < T; o <<= 1)
@@ -287,12 +320,21 @@ static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y
   if (grid > kMaxParts) grid = kMaxParts;
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
-  if (unroll == 4)
-    hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 4>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups, rb.gptr,
-                       rb.gcol, gval, X, Y, partial, st);
-  else
-    hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 1>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups, rb.gptr,
-                       rb.gcol, gval, X, Y, partial, st);
+  constexpr bool kTeamIsWave = T == 64;
+  if (kTeamIsWave && rb.grun && rb.grptr) {
+    if (unroll == 4)
+      hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 4, kTeamIsWave>), dim3(grid), dim3(kBlock), 0, s, rb.n,
+                         rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
+    else
+      hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 1, kTeamIsWave>), dim3(grid), dim3(kBlock), 0, s, rb.n,
+                         rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
+  } else if (unroll == 4) {
+    hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 4, false>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                       rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
+  } else {
+    hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 1, false>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                       rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
+  }
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }

@@ -139,6 +139,8 @@ class RbDesc(ctypes.Structure):
         ("gcol", ctypes.c_void_p),
         ("gval", ctypes.c_void_p),
         ("gval32", ctypes.c_void_p),
+        ("grptr", ctypes.c_void_p),
+        ("grun", ctypes.c_void_p),
     ]
 
 
@@ -248,9 +250,15 @@ class DeviceRowBlocks:
     gval: object | None
     gval32: object | None
     desc: RbDesc
+    grptr: object | None = None
+    grun: object | None = None
+    nruns: int = 0
 
     @staticmethod
-    def from_csr(A: DeviceCSR, fp64: bool = True, fp32: bool = True) -> "DeviceRowBlocks":
+    def from_csr(A: DeviceCSR, fp64: bool = True, fp32: bool = True, runs: bool = False) -> "DeviceRowBlocks":
+        """``runs``: also encode each group's union columns as runs of
+        consecutive columns (``grptr``, ``grun``); products whose group team
+        is one wave then read the columns from the runs."""
         lib = require_gpu()
         torch = _torch()
         dev = A.vals.device
@@ -268,12 +276,25 @@ class DeviceRowBlocks:
                "rb_build_fill")
         desc = RbDesc(A.n, ng, _ptr(gptr), _ptr(gcol), _ptr(gval) if fp64 else None,
                       _ptr(gval32) if fp32 else None)
-        return DeviceRowBlocks(A.n, ng, nu, gptr, gcol, gval, gval32, desc)
+        out = DeviceRowBlocks(A.n, ng, nu, gptr, gcol, gval, gval32, desc)
+        if runs:
+            grptr = torch.empty(ng + 1, dtype=torch.int32, device=dev)
+            nr = ctypes.c_int64()
+            _check(lib.cc_runs_count(ng, _ptr(gptr), _ptr(gcol), _ptr(grptr), ctypes.addressof(nr), _stream()),
+                   "runs_count (row-block union columns)")
+            nr = int(nr.value)
+            grun = torch.empty(2 * max(nr, 1), dtype=torch.int32, device=dev)
+            _check(lib.cc_runs_fill(ng, _ptr(gptr), _ptr(gcol), _ptr(grptr), _ptr(grun), _stream()), "runs_fill")
+            out.grptr, out.grun, out.nruns = grptr, grun, nr
+            desc.grptr, desc.grun = _ptr(grptr), _ptr(grun)
+        return out
 
     def bytes(self, value_bytes: int = 8) -> int:
-        """Stored bytes one product reads: group offsets, union columns and
-        the padded values of one precision."""
-        return int(4 * (self.ngroups + 1) + self.nu * (4 + RB_ROWS * value_bytes))
+        """Stored bytes one product reads: group offsets, the union columns
+        (or, with runs, the run offsets and runs) and the padded values of
+        one precision."""
+        cols = 4 * (self.ngroups + 1) + 8 * self.nruns if self.grun is not None else 4 * self.nu
+        return int(4 * (self.ngroups + 1) + cols + self.nu * RB_ROWS * value_bytes)
 
 
 @dataclass

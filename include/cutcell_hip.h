@@ -243,6 +243,13 @@ typedef struct cc_rb_desc {
   const int32_t* gcol;  /* [nu] */
   const double* gval;   /* [nu * CC_RB_ROWS] float64 values, or NULL */
   const float* gval32;  /* [nu * CC_RB_ROWS] float32 values, or NULL */
+  /* Optional (both NULL, or both set): the union columns of each group as
+   * runs of consecutive columns (cc_runs_count / cc_runs_fill with units =
+   * groups, uptr = gptr, cols = gcol): grptr [ngroups + 1], grun
+   * [2 * nruns]. When set, products whose group team is one wave read the
+   * columns from the runs instead of gcol (same values, same sums). */
+  const int32_t* grptr;
+  const uint32_t* grun;
 } cc_rb_desc;
 
 /* Build step 1: from a CSR matrix whose column indices ascend within each

@@ -167,3 +167,81 @@ def test_pcg_row_blocks_multi_rhs(hip, precision):
         assert abs(res_rb[c].iterations - res_csr[c].iterations) <= (3 if precision == "fp64" else 6)
         assert np.linalg.norm(X_rb[:, c] - X_csr[:, c]) / np.linalg.norm(X_csr[:, c]) < (
             1e-6 if precision == "fp64" else 1e-4)
+
+
+def host_union_runs(gptr, gcol):
+    """Runs of every group's union columns, from the definition (a run ends
+    where the next column is not the previous plus one, or at 255 entries)."""
+    rptr, recs = [0], []
+    for g in range(gptr.size - 1):
+        cols = gcol[gptr[g]:gptr[g + 1]]
+        i = 0
+        while i < cols.size:
+            j = i + 1
+            while j < cols.size and cols[j] == cols[j - 1] + 1 and j - i < 255:
+                j += 1
+            recs.append((int(cols[i]), (i << 8) | (j - i)))
+            i = j
+        rptr.append(len(recs))
+    return np.array(rptr), np.array(recs, dtype=np.uint32).reshape(-1, 2)
+
+
+@pytest.mark.parametrize("k,vb", [(1, 8), (16, 8), (4, 4), (16, 4)])
+def test_row_block_product_from_runs(hip, mats, k, vb):
+    """With the union columns given as runs, the one-wave-per-group shapes
+    read them from the runs: runs bit-exact against the definition, products
+    against the oracle at the gcol product's bar for every launch shape (the
+    other shapes keep reading gcol)."""
+    from cutcell.autotune import rb_candidates
+
+    lib = hip.load_library()
+    try:
+        for name, A in mats.items():
+            Ad = hip.DeviceCSR.from_scipy(A)
+            rb = hip.DeviceRowBlocks.from_csr(Ad, runs=True)
+            gptr, gcol, _ = host_row_blocks(A)
+            rptr, recs = host_union_runs(gptr, gcol)
+            assert rb.nruns == recs.shape[0]
+            np.testing.assert_array_equal(rb.grptr.cpu().numpy(), rptr)
+            np.testing.assert_array_equal(rb.grun.cpu().numpy().view(np.uint32)[: 2 * rb.nruns].reshape(-1, 2), recs)
+            X = np.random.default_rng(k + 1).standard_normal((A.shape[0], k))
+            if vb == 4:
+                X = X.astype(np.float32)
+            A_ref = A.astype(np.float32).astype(np.float64) if vb == 4 else A
+            ref = np.stack([oracle.spmv(A_ref.tocsr(), X[:, c].astype(np.float64)) for c in range(k)], axis=1)
+            scale = max((np.abs(A_ref) @ np.abs(X.astype(np.float64))).max(), 1e-300)
+            tol = 1e-5 if vb == 4 else 1e-13
+            for cand in rb_candidates(k, vb):
+                assert lib.cc_rb_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) == 0
+                Y = hip.spmm_rb(rb, _dev(X)).cpu().numpy().reshape(A.shape[0], k)
+                assert np.abs(Y - ref).max() <= tol * scale, (name, cand)
+    finally:
+        lib.cc_rb_set_config(k, vb, 0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pcg_row_block_runs_on_a_3d_planar_system(hip, precision):
+    """The headline's solve with the row-block products read from runs, on a
+    3D planar p=3 system, against the same solve reading gcol: same status,
+    iterations within the rounding bar, solutions within 1e-6 (fp64) / 1e-4
+    (mixed) and every column at the float64 true residual."""
+    from cutcell.cut3d import PlanarCut, assemble_levelset_3d, polynomial_preserving_extension_3d, restrict_spd
+
+    asm = assemble_levelset_3d(PlanarCut.random(1), 9, 3)
+    agg = polynomial_preserving_extension_3d(asm)
+    A = (restrict_spd(asm.K, agg.E) + restrict_spd(asm.H, agg.E)).tocsr()
+    A.sort_indices()
+    Ad = hip.DeviceCSR.from_scipy(A, with_f32=True)
+    B = np.random.default_rng(12).standard_normal((A.shape[0], 16))
+    rb_runs = hip.DeviceRowBlocks.from_csr(Ad, runs=True)
+    rb_cols = hip.DeviceRowBlocks.from_csr(Ad)
+    X_r, res_r = hip.pcg(Ad, _dev(B), tol=1e-8, precision=precision, rb=rb_runs)
+    X_c, res_c = hip.pcg(Ad, _dev(B), tol=1e-8, precision=precision, rb=rb_cols)
+    X_r, X_c = X_r.cpu().numpy(), X_c.cpu().numpy()
+    print(f"planar N=9 p=3 n={A.shape[0]} nnz={A.nnz} runs={rb_runs.nruns} union={rb_runs.nu}: "
+          f"iterations runs {[r.iterations for r in res_r][:4]} gcol {[r.iterations for r in res_c][:4]}", flush=True)
+    for c in range(16):
+        assert res_r[c].status == hip.CC_CONVERGED
+        assert np.linalg.norm(A @ X_r[:, c] - B[:, c]) / np.linalg.norm(B[:, c]) <= 1e-7
+        assert abs(res_r[c].iterations - res_c[c].iterations) <= (3 if precision == "fp64" else 6)
+        assert np.linalg.norm(X_r[:, c] - X_c[:, c]) / np.linalg.norm(X_c[:, c]) < (1e-6 if precision == "fp64" else 1e-4)
