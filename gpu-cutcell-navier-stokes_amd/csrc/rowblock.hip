@@ -79,6 +79,70 @@ __global__ __launch_bounds__(256) void k_rb_merge(int n, int ng, const int* __re
   if (!FILL) gptr[g] = bad ? -1 : out;
 }
 
+// ---------------------------------------------------------------------------
+// Paired row blocks: 8-row groups made of two 4-row groups (pairs[2 p] and
+// pairs[2 p + 1]; the second -1 for a group left alone), the ascending union
+// of their 8 rows' columns and 8 values per union column. On the 3D B-spline
+// systems a group's best partner is the group of the same x-range a few grid
+// lines away in y: on the 81k planar system the union then holds 0.23-0.25 of
+// the nonzeros against 0.36 for 4-row groups, so the gathered X rows drop by
+// a third while the stored values grow by about an eighth.
+// ---------------------------------------------------------------------------
+constexpr int kRb2 = 2 * kRb;
+
+// Row i (0..7) of paired group p, or -1.
+__device__ inline int pair_row(const int* __restrict__ pairs, int p, int i, int n) {
+  const int g = pairs[2 * p + (i >= kRb ? 1 : 0)];
+  const int row = g < 0 ? -1 : g * kRb + (i % kRb);
+  return row < n ? row : -1;
+}
+
+template <bool FILL>
+__global__ __launch_bounds__(256) void k_rb8_merge(int n, int np, const int* __restrict__ pairs,
+                                                   const int* __restrict__ indptr, const int* __restrict__ indices,
+                                                   const double* __restrict__ vals, int* __restrict__ gptr,
+                                                   int* __restrict__ gcol, double* __restrict__ gval,
+                                                   float* __restrict__ gval32) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= np) return;
+  int p[kRb2], e[kRb2];
+#pragma unroll
+  for (int r = 0; r < kRb2; ++r) {
+    const int row = pair_row(pairs, g, r, n);
+    p[r] = row >= 0 ? indptr[row] : 0;
+    e[r] = row >= 0 ? indptr[row + 1] : 0;
+  }
+  int out = FILL ? gptr[g] : 0;
+  bool bad = false;
+  while (true) {
+    int m = INT_MAX;
+#pragma unroll
+    for (int r = 0; r < kRb2; ++r)
+      if (p[r] < e[r]) m = min(m, indices[p[r]]);
+    if (m == INT_MAX) break;
+    double v[kRb2];
+#pragma unroll
+    for (int r = 0; r < kRb2; ++r) {
+      v[r] = 0.0;
+      while (p[r] < e[r] && indices[p[r]] == m) {
+        if (FILL) v[r] += vals[p[r]];
+        ++p[r];
+      }
+      if (p[r] < e[r] && indices[p[r]] < m) bad = true;
+    }
+    if (FILL) {
+      gcol[out] = m;
+#pragma unroll
+      for (int r = 0; r < kRb2; ++r) {
+        if (gval) gval[(size_t)out * kRb2 + r] = v[r];
+        if (gval32) gval32[(size_t)out * kRb2 + r] = (float)v[r];
+      }
+    }
+    ++out;
+  }
+  if (!FILL) gptr[g] = bad ? -1 : out;
+}
+
 // One block of 1024 threads: counts gptr[0, ng) -> exclusive offsets
 // gptr[0, ng], gptr[ng] = total, or -1 if a count was negative or the total
 // does not fit an int32.
@@ -147,6 +211,14 @@ __device__ inline void load_rb_vals<double>(const double* __restrict__ p, double
   v[3] = b.y;
 }
 
+// Value r of the 4 held by the lanes of this lane's quad (DPP quad
+// broadcast; no memory access).
+template <int R>
+__device__ inline float quad_bcast(float v) {
+  constexpr int ctrl = R | (R << 2) | (R << 4) | (R << 6);
+  return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+}
+
 // RUNS: the union columns come from their runs (grptr, grun: first column,
 // offset << 8 | length per run of consecutive columns, see runs.hip) in
 // place of gcol. A team is then one wave (T = 64); per group it expands the
@@ -165,6 +237,8 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __res
   constexpr int T = KL * NZ;
   constexpr int TPW = 64 / T;
   static_assert(!RUNS || T == 64, "the run-driven product takes one wave per group");
+  // float32 values, 4 right-hand sides per lane and 4 lanes per slot.
+  constexpr bool kQuadVals = sizeof(MT) == 4 && CPL == 4 && KL == 4;
   using AT = typename Accum<VT>::type;
   __shared__ double smem[4 * 16];
   __shared__ int s_rows[2];
@@ -207,7 +281,33 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __res
     // Union columns [ja, jb) with column j at cp[j].
     auto columns = [&](const int* cp, int ja, int jb) {
       int j = ja + zs;
-      if constexpr (UNR == 4) {
+      if constexpr (kQuadVals && UNR == 4) {
+        // The 4 lanes of a slot form a DPP quad: lane i of the quad loads
+        // row value i of each column (the team's loads cover its columns'
+        // values once, contiguously) and the quad shares them.
+        const int qi = lane & 3;
+        for (; j + 3 * NZ < jb; j += 4 * NZ) {
+          int q[4];
+          float vq[4];
+          VT x[4][CPL];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            q[u] = cp[j + u * NZ];
+            vq[u] = (float)vp[(size_t)(j + u * NZ) * kRb + qi];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) vload<VT, CPL>(X + (size_t)q[u] * K + c0, x[u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const float vr[kRb] = {quad_bcast<0>(vq[u]), quad_bcast<1>(vq[u]), quad_bcast<2>(vq[u]),
+                                   quad_bcast<3>(vq[u])};
+#pragma unroll
+            for (int r = 0; r < kRb; ++r)
+#pragma unroll
+              for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)vr[r] * (AT)x[u][m];
+          }
+        }
+      } else if constexpr (UNR == 4) {
         for (; j + 3 * NZ < jb; j += 4 * NZ) {
           int q[4];
           MT v[4][kRb];
@@ -301,6 +401,133 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __res
   }
 }
 
+// The product from paired row blocks: as k_spmm_rb with 8 rows per group
+// (a lane accumulates its CPL columns for all 8 rows) and the rows of group
+// g taken from the pair list.
+template <typename MT, typename VT, int K, int NZ, int CPL, int UNR>
+__global__ __launch_bounds__(256) void k_spmm_rb8(int n, int ng, const int* __restrict__ pairs,
+                                                  const int* __restrict__ gptr, const int* __restrict__ gcol,
+                                                  const MT* __restrict__ gval, const VT* __restrict__ X,
+                                                  VT* __restrict__ Y, double* __restrict__ partial,
+                                                  const ColState* __restrict__ st) {
+  constexpr int KL = K / CPL;
+  constexpr int T = KL * NZ;
+  constexpr int TPW = 64 / T;
+  using AT = typename Accum<VT>::type;
+  __shared__ double smem[4 * 16];
+  __shared__ int s_rows[2];
+  if (all_stopped<K>(st)) return;
+  const int lane = threadIdx.x & 63;
+  const int c0 = (lane % KL) * CPL;
+  const int zs = (lane / KL) % NZ;
+  const int wave = threadIdx.x >> 6;
+  const int wpb = blockDim.x >> 6;
+  const int tpb = wpb * TPW;
+  const int team = wave * TPW + lane / T;
+  const int G = gridDim.x;
+  const int b = blockIdx.x;
+  const int chunk = (G % kXcds == 0) ? (b % kXcds) * (G / kXcds) + b / kXcds : b;
+  if (wave == 0) {
+    const long long total = gptr[ng];
+    const int a0 = chunk == 0 ? 0 : row_lower_bound(gptr, ng, total * chunk / G);
+    const int a1 = chunk == G - 1 ? ng : row_lower_bound(gptr, ng, total * (chunk + 1) / G);
+    if (lane == 0) {
+      s_rows[0] = a0;
+      s_rows[1] = a1;
+    }
+  }
+  __syncthreads();
+  const int g0 = s_rows[0];
+  const int g1 = s_rows[1];
+  double dot[CPL];
+#pragma unroll
+  for (int m = 0; m < CPL; ++m) dot[m] = 0.0;
+  for (int g = g0 + team; g < g1; g += tpb) {
+    const int beg = gptr[g];
+    const int len = gptr[g + 1] - beg;
+    const int* __restrict__ cp = gcol + beg;
+    const MT* __restrict__ vp = gval + (size_t)beg * kRb2;
+    AT acc[kRb2][CPL];
+#pragma unroll
+    for (int r = 0; r < kRb2; ++r)
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) acc[r][m] = 0;
+    int j = zs;
+    if constexpr (UNR == 4) {
+      for (; j + 3 * NZ < len; j += 4 * NZ) {
+        int q[4];
+        MT v[4][kRb2];
+        VT x[4][CPL];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          q[u] = cp[j + u * NZ];
+          load_rb_vals<MT>(vp + (size_t)(j + u * NZ) * kRb2, v[u]);
+          load_rb_vals<MT>(vp + (size_t)(j + u * NZ) * kRb2 + kRb, v[u] + kRb);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) vload<VT, CPL>(X + (size_t)q[u] * K + c0, x[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < kRb2; ++r)
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)v[u][r] * (AT)x[u][m];
+      }
+    }
+    for (; j < len; j += NZ) {
+      MT v[kRb2];
+      VT x[CPL];
+      load_rb_vals<MT>(vp + (size_t)j * kRb2, v);
+      load_rb_vals<MT>(vp + (size_t)j * kRb2 + kRb, v + kRb);
+      vload<VT, CPL>(X + (size_t)cp[j] * K + c0, x);
+#pragma unroll
+      for (int r = 0; r < kRb2; ++r)
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) acc[r][m] += (AT)v[r] * (AT)x[m];
+    }
+#pragma unroll
+    for (int o = KL; o < T; o <<= 1)
+#pragma unroll
+      for (int r = 0; r < kRb2; ++r)
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) acc[r][m] += __shfl_xor(acc[r][m], o, 64);
+    if (zs == 0) {
+#pragma unroll
+      for (int r = 0; r < kRb2; ++r) {
+        const int row = pair_row(pairs, g, r, n);
+        if (row >= 0) {
+          const size_t e = (size_t)row * K + c0;
+          VT yv[CPL];
+#pragma unroll
+          for (int m = 0; m < CPL; ++m) yv[m] = (VT)acc[r][m];
+          vstore<VT, CPL>(Y + e, yv);
+          if (partial) {
+            VT xv[CPL];
+            vload<VT, CPL>(X + e, xv);
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) dot[m] += (double)xv[m] * (double)yv[m];
+          }
+        }
+      }
+    }
+  }
+  if (partial) {
+#pragma unroll
+    for (int o = KL; o < 64; o <<= 1)
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) dot[m] += __shfl_xor(dot[m], o, 64);
+    if (lane < KL)
+#pragma unroll
+      for (int m = 0; m < CPL; ++m) smem[wave * K + c0 + m] = dot[m];
+    __syncthreads();
+    if (threadIdx.x < K) {
+      double s = 0.0;
+      for (int w = 0; w < wpb; ++w) s += smem[w * K + threadIdx.x];
+      partial[(size_t)b * K + threadIdx.x] = s;
+    }
+  }
+}
+
 // Launch shape per value width and k (set by the autotuner; 0 = default).
 struct RbConfig {
   int nz;      // union-column slots per group team ((k/cpl)*nz <= 64)
@@ -321,7 +548,14 @@ static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
   constexpr bool kTeamIsWave = T == 64;
-  if (kTeamIsWave && rb.grun && rb.grptr) {
+  if (rb.pairs) {
+    if (unroll == 4)
+      hipLaunchKernelGGL((k_spmm_rb8<MT, VT, K, NZ, CPL, 4>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                         rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
+    else
+      hipLaunchKernelGGL((k_spmm_rb8<MT, VT, K, NZ, CPL, 1>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                         rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
+  } else if (kTeamIsWave && rb.grun && rb.grptr) {
     if (unroll == 4)
       hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 4, kTeamIsWave>), dim3(grid), dim3(kBlock), 0, s, rb.n,
                          rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
@@ -395,7 +629,9 @@ CC_RB_INST(16)
 #undef CC_RB_INST
 
 bool rb_desc_ok(const cc_rb_desc* rb, int n) {
-  return rb && rb->n == n && rb->ngroups == (n + kRb - 1) / kRb && rb->gptr && rb->gcol;
+  if (!rb || rb->n != n || !rb->gptr || !rb->gcol) return false;
+  if (rb->pairs) return rb->ngroups >= 1 && rb->ngroups <= (n + kRb - 1) / kRb && !rb->grun;
+  return rb->ngroups == (n + kRb - 1) / kRb;
 }
 
 }  // namespace cc
@@ -428,6 +664,33 @@ int cc_rb_build_fill(int32_t n, const int32_t* indptr, const int32_t* indices, c
   const int ng = (n + kRb - 1) / kRb;
   hipLaunchKernelGGL((k_rb_merge<true>), dim3((ng + 255) / 256), dim3(256), 0, s, n, ng, indptr, indices, vals,
                      const_cast<int32_t*>(gptr), gcol, gval, gval32);
+  return (int)hipGetLastError();
+}
+
+int cc_rb_build_pairs_count(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t npairs,
+                            const int32_t* pairs, int32_t* gptr, int64_t* nu, cc_stream_t stream) {
+  if (n <= 0 || npairs <= 0 || npairs > (n + kRb - 1) / kRb || !indptr || !indices || !pairs || !gptr || !nu)
+    return CC_E_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL((k_rb8_merge<false>), dim3((npairs + 255) / 256), dim3(256), 0, s, n, npairs, pairs, indptr,
+                     indices, nullptr, gptr, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL(k_rb_scan, dim3(1), dim3(1024), 0, s, npairs, gptr);
+  CC_CHECK(hipGetLastError());
+  int32_t total = 0;
+  CC_CHECK(hipMemcpyAsync(&total, gptr + npairs, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  if (total < 0) return CC_E_ARG;
+  *nu = total;
+  return 0;
+}
+
+int cc_rb_build_pairs_fill(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                           int32_t npairs, const int32_t* pairs, const int32_t* gptr, int32_t* gcol, double* gval,
+                           float* gval32, cc_stream_t stream) {
+  if (n <= 0 || npairs <= 0 || !indptr || !indices || !vals || !pairs || !gptr || !gcol || (!gval && !gval32))
+    return CC_E_ARG;
+  hipLaunchKernelGGL((k_rb8_merge<true>), dim3((npairs + 255) / 256), dim3(256), 0, (hipStream_t)stream, n, npairs,
+                     pairs, indptr, indices, vals, const_cast<int32_t*>(gptr), gcol, gval, gval32);
   return (int)hipGetLastError();
 }
 

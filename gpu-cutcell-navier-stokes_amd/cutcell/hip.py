@@ -110,6 +110,8 @@ _SIGNATURES = {
     "cc_asm2d_fill": ([_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_build_count": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_build_fill": ([_I32, _P, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_rb_build_pairs_count": ([_I32, _P, _P, _I32, _P, _P, _P, _P], ctypes.c_int),
+    "cc_rb_build_pairs_fill": ([_I32, _P, _P, _P, _I32, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_rb_set_config": ([_I32, _I32, _I32, _I32, _I32, _I32], ctypes.c_int),
     "cc_spmm_rb_f64": ([_P, _I32, _P, _P, _P], ctypes.c_int),
     "cc_spmm_rb_f32": ([_P, _I32, _P, _P, _P], ctypes.c_int),
@@ -141,6 +143,7 @@ class RbDesc(ctypes.Structure):
         ("gval32", ctypes.c_void_p),
         ("grptr", ctypes.c_void_p),
         ("grun", ctypes.c_void_p),
+        ("pairs", ctypes.c_void_p),
     ]
 
 
@@ -253,6 +256,7 @@ class DeviceRowBlocks:
     grptr: object | None = None
     grun: object | None = None
     nruns: int = 0
+    pairs: object | None = None
 
     @staticmethod
     def from_csr(A: DeviceCSR, fp64: bool = True, fp32: bool = True, runs: bool = False) -> "DeviceRowBlocks":
@@ -289,12 +293,46 @@ class DeviceRowBlocks:
             desc.grptr, desc.grun = _ptr(grptr), _ptr(grun)
         return out
 
+    @staticmethod
+    def from_csr_pairs(A: DeviceCSR, pairs: np.ndarray, fp64: bool = True, fp32: bool = True) -> "DeviceRowBlocks":
+        """Paired row blocks: the 8 rows of the two 4-row groups of each row
+        of ``pairs`` (``cutcell.rowpairs.row_group_pairs``; every group once)
+        form one group, with 8 values per union column."""
+        lib = require_gpu()
+        torch = _torch()
+        dev = A.vals.device
+        pairs = np.ascontiguousarray(pairs, dtype=np.int32).reshape(-1, 2)
+        ng4 = -(-A.n // RB_ROWS)
+        seen = np.concatenate([pairs[:, 0], pairs[pairs[:, 1] >= 0, 1]])
+        if seen.size != ng4 or not np.array_equal(np.sort(seen), np.arange(ng4)):
+            raise ValueError("pairs must list every 4-row group exactly once")
+        npairs = int(pairs.shape[0])
+        pd = torch.from_numpy(pairs.reshape(-1)).to(dev)
+        gptr = torch.empty(npairs + 1, dtype=torch.int32, device=dev)
+        nu = ctypes.c_int64()
+        _check(lib.cc_rb_build_pairs_count(A.n, _ptr(A.indptr), _ptr(A.indices), npairs, _ptr(pd), _ptr(gptr),
+                                           ctypes.addressof(nu), _stream()),
+               "rb_build_pairs_count (column indices must ascend within rows)")
+        nu = int(nu.value)
+        R2 = 2 * RB_ROWS
+        gcol = torch.empty(max(nu, 1), dtype=torch.int32, device=dev)
+        gval = torch.empty(max(nu, 1) * R2, dtype=torch.float64, device=dev) if fp64 else None
+        gval32 = torch.empty(max(nu, 1) * R2, dtype=torch.float32, device=dev) if fp32 else None
+        _check(lib.cc_rb_build_pairs_fill(A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), npairs, _ptr(pd),
+                                          _ptr(gptr), _ptr(gcol), _ptr(gval) if fp64 else None,
+                                          _ptr(gval32) if fp32 else None, _stream()), "rb_build_pairs_fill")
+        desc = RbDesc(A.n, npairs, _ptr(gptr), _ptr(gcol), _ptr(gval) if fp64 else None,
+                      _ptr(gval32) if fp32 else None, None, None, _ptr(pd))
+        return DeviceRowBlocks(A.n, npairs, nu, gptr, gcol, gval, gval32, desc, pairs=pd)
+
     def bytes(self, value_bytes: int = 8) -> int:
-        """Stored bytes one product reads: group offsets, the union columns
-        (or, with runs, the run offsets and runs) and the padded values of
-        one precision."""
+        """Stored bytes one product reads: group offsets (and the pair list),
+        the union columns (or, with runs, the run offsets and runs) and the
+        padded values of one precision."""
+        rows = 2 * RB_ROWS if self.pairs is not None else RB_ROWS
         cols = 4 * (self.ngroups + 1) + 8 * self.nruns if self.grun is not None else 4 * self.nu
-        return int(4 * (self.ngroups + 1) + cols + self.nu * RB_ROWS * value_bytes)
+        extra = 8 * self.ngroups if self.pairs is not None else 0
+        return int(4 * (self.ngroups + 1) + cols + extra + self.nu * rows * value_bytes)
 
 
 @dataclass

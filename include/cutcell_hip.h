@@ -250,6 +250,12 @@ typedef struct cc_rb_desc {
    * columns from the runs instead of gcol (same values, same sums). */
   const int32_t* grptr;
   const uint32_t* grun;
+  /* Optional: paired groups. When set, group g holds the 8 rows of the two
+   * 4-row groups pairs[2g] and pairs[2g+1] (the second -1 for a group left
+   * alone; every 4-row group appears exactly once), ngroups is the number of
+   * pairs, and gval / gval32 hold 8 values per union column (built by
+   * cc_rb_build_pairs_count / _fill; grptr, grun must then be NULL). */
+  const int32_t* pairs;
 } cc_rb_desc;
 
 /* Build step 1: from a CSR matrix whose column indices ascend within each
@@ -262,6 +268,14 @@ int cc_rb_build_count(int32_t n, const int32_t* indptr, const int32_t* indices, 
  * may be NULL, not both) from the same CSR matrix and the gptr of step 1. */
 int cc_rb_build_fill(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
                      const int32_t* gptr, int32_t* gcol, double* gval, float* gval32, cc_stream_t stream);
+/* The same two build steps for paired groups (pairs [npairs * 2], see
+ * cc_rb_desc.pairs): the union of each pair's 8 rows, 8 values per union
+ * column. */
+int cc_rb_build_pairs_count(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t npairs,
+                            const int32_t* pairs, int32_t* gptr, int64_t* nu, cc_stream_t stream);
+int cc_rb_build_pairs_fill(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                           int32_t npairs, const int32_t* pairs, const int32_t* gptr, int32_t* gcol, double* gval,
+                           float* gval32, cc_stream_t stream);
 /* Launch shape of the row-block product for k right-hand sides of
  * value_bytes-wide vectors (autotuner hook): nz_lanes union-column slots per
  * group team ((k/cols_per_lane)*nz_lanes <= 64, power of two, 0 = default),

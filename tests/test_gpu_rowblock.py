@@ -245,3 +245,106 @@ def test_pcg_row_block_runs_on_a_3d_planar_system(hip, precision):
         assert np.linalg.norm(A @ X_r[:, c] - B[:, c]) / np.linalg.norm(B[:, c]) <= 1e-7
         assert abs(res_r[c].iterations - res_c[c].iterations) <= (3 if precision == "fp64" else 6)
         assert np.linalg.norm(X_r[:, c] - X_c[:, c]) / np.linalg.norm(X_c[:, c]) < (1e-6 if precision == "fp64" else 1e-4)
+
+
+def host_paired_blocks(A, pairs):
+    """The paired row-block form from its definition: per pair, the
+    ascending union of its (up to) 8 rows' columns and the dense 8-row values
+    there (rows of a missing second group, or past n, are zero)."""
+    A = A.tocsr()
+    n = A.shape[0]
+    gptr, cols, vals = [0], [], []
+    for a, b in pairs:
+        rows = [a * R + i if a * R + i < n else -1 for i in range(R)]
+        rows += [b * R + i if b >= 0 and b * R + i < n else -1 for i in range(R)]
+        segs = [A.indices[A.indptr[r]:A.indptr[r + 1]] for r in rows if r >= 0]
+        u = np.unique(np.concatenate(segs + [np.zeros(0, int)]))
+        block = np.zeros((u.size, 2 * R))
+        for i, r in enumerate(rows):
+            if r >= 0:
+                seg = slice(A.indptr[r], A.indptr[r + 1])
+                block[np.searchsorted(u, A.indices[seg]), i] = A.data[seg]
+        cols.append(u)
+        vals.append(block)
+        gptr.append(gptr[-1] + u.size)
+    return np.array(gptr), np.concatenate(cols), np.concatenate(vals).reshape(-1)
+
+
+def _planar_csr(seed, N):
+    from cutcell.cut3d import PlanarCut, assemble_levelset_3d, polynomial_preserving_extension_3d, restrict_spd
+
+    asm = assemble_levelset_3d(PlanarCut.random(seed), N, 3)
+    agg = polynomial_preserving_extension_3d(asm)
+    A = (restrict_spd(asm.K, agg.E) + restrict_spd(asm.H, agg.E)).tocsr()
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("k,vb", [(1, 8), (16, 8), (4, 4), (16, 4)])
+def test_paired_row_blocks_match_definition_and_oracle(hip, mats, k, vb):
+    """Paired row blocks: form entry by entry against the definition
+    (integers bit-exact, float64 values exact, float32 the rounded ones), the
+    product against the oracle for every launch shape."""
+    from cutcell.autotune import rb_candidates
+    from cutcell.rowpairs import row_group_pairs
+
+    lib = hip.load_library()
+    cases = dict(mats)
+    cases["planar-p3-N8"] = _planar_csr(2, 8)
+    try:
+        for name, A in cases.items():
+            A = A.tocsr()
+            A.sort_indices()
+            pairs = row_group_pairs(A)
+            Ad = hip.DeviceCSR.from_scipy(A)
+            rb = hip.DeviceRowBlocks.from_csr_pairs(Ad, pairs)
+            gptr, gcol, gval = host_paired_blocks(A, pairs)
+            assert rb.ngroups == pairs.shape[0] and rb.nu == gcol.size
+            np.testing.assert_array_equal(rb.gptr.cpu().numpy(), gptr)
+            np.testing.assert_array_equal(rb.gcol.cpu().numpy()[: rb.nu], gcol)
+            np.testing.assert_array_equal(rb.gval.cpu().numpy()[: rb.nu * 2 * R], gval)
+            np.testing.assert_array_equal(rb.gval32.cpu().numpy()[: rb.nu * 2 * R], gval.astype(np.float32))
+            X = np.random.default_rng(k + 2).standard_normal((A.shape[0], k))
+            if vb == 4:
+                X = X.astype(np.float32)
+            A_ref = A.astype(np.float32).astype(np.float64) if vb == 4 else A
+            ref = np.stack([oracle.spmv(A_ref.tocsr(), X[:, c].astype(np.float64)) for c in range(k)], axis=1)
+            scale = max((np.abs(A_ref) @ np.abs(X.astype(np.float64))).max(), 1e-300)
+            tol = 1e-5 if vb == 4 else 1e-13
+            for cand in rb_candidates(k, vb):
+                assert lib.cc_rb_set_config(k, vb, cand["nz"], cand["grid"], cand["unroll"], cand["cpl"]) == 0
+                Y = hip.spmm_rb(rb, _dev(X)).cpu().numpy().reshape(A.shape[0], k)
+                assert np.abs(Y - ref).max() <= tol * scale, (name, cand)
+    finally:
+        lib.cc_rb_set_config(k, vb, 0, 0, 0, 0)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_pcg_paired_row_blocks_on_a_3d_planar_system(hip, precision):
+    """The headline solve with paired row blocks on a 3D planar p=3 system
+    against the 4-row form: same status, iterations within the rounding bar,
+    solutions within 1e-6 (fp64) / 1e-4 (mixed), float64 true residuals."""
+    from cutcell.rowpairs import row_group_pairs
+
+    A = _planar_csr(1, 9)
+    Ad = hip.DeviceCSR.from_scipy(A, with_f32=True)
+    B = np.random.default_rng(13).standard_normal((A.shape[0], 16))
+    rb8 = hip.DeviceRowBlocks.from_csr_pairs(Ad, row_group_pairs(A))
+    rb4 = hip.DeviceRowBlocks.from_csr(Ad)
+    X8, res8 = hip.pcg(Ad, _dev(B), tol=1e-8, precision=precision, rb=rb8)
+    X4, res4 = hip.pcg(Ad, _dev(B), tol=1e-8, precision=precision, rb=rb4)
+    X8, X4 = X8.cpu().numpy(), X4.cpu().numpy()
+    print(f"planar N=9 p=3 n={A.shape[0]}: union 4-row {rb4.nu}, paired {rb8.nu}; iterations paired "
+          f"{[r.iterations for r in res8][:4]} 4-row {[r.iterations for r in res4][:4]}", flush=True)
+    for c in range(16):
+        assert res8[c].status == hip.CC_CONVERGED
+        assert np.linalg.norm(A @ X8[:, c] - B[:, c]) / np.linalg.norm(B[:, c]) <= 1e-7
+        if precision == "fp64":
+            assert abs(res8[c].iterations - res4[c].iterations) <= 3
+        elif res8[c].outer == res4[c].outer:
+            assert abs(res8[c].iterations - res4[c].iterations) <= 6
+        else:
+            # Another summation order can leave a sweep's float64 residual
+            # just above tol (one more sweep) or just below it (one fewer).
+            assert abs(res8[c].outer - res4[c].outer) == 1
+        assert np.linalg.norm(X8[:, c] - X4[:, c]) / np.linalg.norm(X4[:, c]) < (1e-6 if precision == "fp64" else 1e-4)
