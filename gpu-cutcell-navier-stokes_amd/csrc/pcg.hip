@@ -1164,14 +1164,27 @@ static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_
   return 0;
 }
 
-// rb (may be NULL): take the products from the row-block form instead.
+// The runs of the rows (runs.hip), for single-right-hand-side products.
+struct RunsView {
+  const int* rptr;
+  const uint32_t* runs;
+};
+template <typename MT, typename VT>
+int runs_spmv_cg(int n, const int* rptr, const uint32_t* runs, const int* indptr, const MT* vals, const VT* P, VT* Q,
+                 double* partial, const ColState* st, hipStream_t s, int* grid_out);
+
+// rb (may be NULL): take the products from the row-block form instead;
+// rv (may be NULL, K = 1 only): from the run encoding of the rows.
 template <typename MT, typename VT, int K>
 static int cg_run(int n, double avg_row, const int* indptr, const int* indices, const MT* vals, const double* src,
                   const double* inv_scale, const int* run_mask, CgBuffers<VT>& b, int precond, double tol,
                   int maxiter, int check_every, hipStream_t s, std::vector<ColState>& host_st,
-                  double* hist = nullptr, const cc_rb_desc* rb = nullptr) {
+                  double* hist = nullptr, const cc_rb_desc* rb = nullptr, const RunsView* rv = nullptr) {
   const ColState* st = b.st;
   auto apply = [&](const VT* P, VT* Q, double* partial, int* gs) {
+    if constexpr (K == 1) {
+      if (rv) return runs_spmv_cg<MT, VT>(n, rv->rptr, rv->runs, indptr, vals, P, Q, partial, st, s, gs);
+    }
     if (rb) return rb_spmm<MT, VT, K>(*rb, P, Q, partial, st, s, gs);
     return spmm_launch<MT, VT, K, 0>(n, avg_row, indptr, indices, vals, P, Q, nullptr, partial, st, s, gs);
   };
@@ -1325,7 +1338,8 @@ static int true_residual(int n, double avg_row, const int* indptr, const int* in
 template <int K>
 static int pcg_f64(int n, const int* indptr, const int* indices, const double* vals, const double* B, double* X,
                    double tol, int maxiter, int precond, int check_every, void* ws, size_t wsb,
-                   cc_pcg_result* results, double* hist, hipStream_t s, const cc_rb_desc* rb = nullptr) {
+                   cc_pcg_result* results, double* hist, hipStream_t s, const cc_rb_desc* rb = nullptr,
+                   const RunsView* rv = nullptr) {
   Carve cv{(char*)ws, 0, wsb};
   CgBuffers<double> b = carve_cg<double>(cv, n, K, X);
   if (!cv.ok()) return CC_E_WORKSPACE;
@@ -1336,7 +1350,7 @@ static int pcg_f64(int n, const int* indptr, const int* indices, const double* v
   }
   std::vector<ColState> hs;
   CC_CHECK((cg_run<double, double, K>(n, avg, indptr, indices, vals, B, nullptr, nullptr, b, precond, tol, maxiter,
-                                      check_every, s, hs, hist, rb)));
+                                      check_every, s, hs, hist, rb, rv)));
   std::vector<double> rel(K);
   CC_CHECK(true_residual<K>(n, avg, indptr, indices, vals, B, X, b.R, b.pa, s, rel.data()));
   for (int c = 0; c < K; ++c) {
@@ -1366,7 +1380,7 @@ template <int K>
 static int pcg_mixed(int n, const int* indptr, const int* indices, const double* vals, const float* vals32,
                      const double* B, double* X, double tol, int maxiter, double inner_tol, int max_outer,
                      int precond, int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
-                     const cc_rb_desc* rb = nullptr) {
+                     const cc_rb_desc* rb = nullptr, const RunsView* rv = nullptr) {
   Carve cv{(char*)ws, 0, wsb};
   CgBuffers<float> b = carve_cg<float>(cv, n, K, nullptr);
   double* R64 = cv.take<double>((size_t)n * K);
@@ -1395,7 +1409,7 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
     for (int c = 0; c < K; ++c) any |= ho[c].status == CC_RUNNING;
     if (!any || used >= maxiter) break;
     CC_CHECK((cg_run<float, float, K>(n, avg, indptr, indices, vals32, R64, inv_scale, run_mask, b, precond,
-                                      inner_tol, maxiter - used, check_every, s, hs, nullptr, rb)));
+                                      inner_tol, maxiter - used, check_every, s, hs, nullptr, rb, rv)));
     int most = 0;
     for (int c = 0; c < K; ++c)
       if (ho[c].status == CC_RUNNING && hs[c].iterations > most) most = hs[c].iterations;
@@ -1660,6 +1674,32 @@ int cc_pcg_jacobi_mixed_rb(int32_t n, int32_t k, const int32_t* indptr, const in
                                   preconditioner, check_every, workspace, workspace_bytes, results, s, rb)));
 }
 
+
+int cc_pcg_jacobi_f64_runs(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                           const int32_t* rptr, const uint32_t* runs, const double* B, double* X, double tol,
+                           int32_t maxiter, int32_t preconditioner, int32_t check_every, void* workspace,
+                           size_t workspace_bytes, cc_pcg_result* results, double* history, cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !rptr || !runs || !B || !X || !results || !workspace || maxiter < 0)
+    return CC_E_ARG;
+  if (workspace_bytes < cc_pcg_workspace_bytes(n, 1, 0)) return CC_E_WORKSPACE;
+  const RunsView rv{rptr, runs};
+  return pcg_f64<1>(n, indptr, indices, vals, B, X, tol, maxiter, preconditioner, check_every, workspace,
+                    workspace_bytes, results, history, (hipStream_t)stream, nullptr, &rv);
+}
+
+int cc_pcg_jacobi_mixed_runs(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                             const float* vals32, const int32_t* rptr, const uint32_t* runs, const double* B,
+                             double* X, double tol, int32_t maxiter, double inner_tol, int32_t max_outer,
+                             int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                             cc_pcg_result* results, cc_stream_t stream) {
+  if (n <= 0 || !indptr || !indices || !vals || !vals32 || !rptr || !runs || !B || !X || !results || !workspace ||
+      maxiter < 0)
+    return CC_E_ARG;
+  if (workspace_bytes < cc_pcg_workspace_bytes(n, 1, 1)) return CC_E_WORKSPACE;
+  const RunsView rv{rptr, runs};
+  return pcg_mixed<1>(n, indptr, indices, vals, vals32, B, X, tol, maxiter, inner_tol, max_outer, preconditioner,
+                      check_every, workspace, workspace_bytes, results, (hipStream_t)stream, nullptr, &rv);
+}
 
 size_t cc_pcg_noalloc_max_bytes(void) { return kScratchBytes; }
 

@@ -123,14 +123,22 @@ constexpr int kRunChunk = 1024;  // columns per wave slice (4 KB of LDS)
 constexpr int kRunWaves = 4;     // waves per block
 constexpr int kRunFast = 512;    // entries of a one-pass row (8 per lane)
 
+// partial (may be NULL): per-block sum of X[r] Y[r] over the block's rows
+// (the p.q of CG), waves added in order; st (may be NULL): skip the launch
+// once the column has stopped.
 template <typename MT, typename VT>
 __global__ __launch_bounds__(64 * kRunWaves) void k_spmv_runs(int n, const int* __restrict__ rptr,
                                                               const uint32_t* __restrict__ runs,
                                                               const int* __restrict__ indptr,
                                                               const MT* __restrict__ vals,
-                                                              const VT* __restrict__ X, VT* __restrict__ Y) {
+                                                              const VT* __restrict__ X, VT* __restrict__ Y,
+                                                              double* __restrict__ partial,
+                                                              const ColState* __restrict__ st) {
   using AT = typename Accum<VT>::type;
   __shared__ int colbuf[kRunWaves][kRunChunk];
+  __shared__ double s_dot[kRunWaves];
+  if (all_stopped<1>(st)) return;
+  double dot = 0.0;
   const int lane = threadIdx.x & 63;
   const int wl = threadIdx.x >> 6;
   int* cols = colbuf[wl];
@@ -198,21 +206,49 @@ __global__ __launch_bounds__(64 * kRunWaves) void k_spmv_runs(int n, const int* 
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
-    if (lane == 0) Y[r] = (VT)acc;
+    if (lane == 0) {
+      const VT y = (VT)acc;
+      Y[r] = y;
+      if (partial) dot += (double)X[r] * (double)y;
+    }
+  }
+  if (partial) {
+    if (lane == 0) s_dot[wl] = dot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kRunWaves; ++w) t += s_dot[w];
+      partial[blockIdx.x] = t;
+    }
   }
 }
 
 template <typename MT, typename VT>
 static int spmv_runs_launch(int n, const int* rptr, const uint32_t* runs, const int* indptr, const MT* vals,
-                            const VT* X, VT* Y, hipStream_t s) {
+                            const VT* X, VT* Y, hipStream_t s, double* partial = nullptr,
+                            const ColState* st = nullptr, int* grid_out = nullptr) {
   if (n == 0) return 0;
-  // One row per wave on the first pass, at most 16 blocks per CU.
+  // One row per wave on the first pass, at most 16 blocks per CU; with
+  // partials at most kMaxParts blocks (one partial each).
   int grid = (n + kRunWaves - 1) / kRunWaves;
-  if (grid > 256 * 16) grid = 256 * 16;
+  const int cap = partial ? kMaxParts : 256 * 16;
+  if (grid > cap) grid = cap;
   hipLaunchKernelGGL((k_spmv_runs<MT, VT>), dim3(grid), dim3(64 * kRunWaves), 0, s, n, rptr, runs, indptr, vals, X,
-                     Y);
+                     Y, partial, st);
+  if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
+
+// The CG product from the runs (pcg.hip): Q = A P with per-block P.Q.
+template <typename MT, typename VT>
+int runs_spmv_cg(int n, const int* rptr, const uint32_t* runs, const int* indptr, const MT* vals, const VT* P, VT* Q,
+                 double* partial, const ColState* st, hipStream_t s, int* grid_out) {
+  return spmv_runs_launch<MT, VT>(n, rptr, runs, indptr, vals, P, Q, s, partial, st, grid_out);
+}
+template int runs_spmv_cg<double, double>(int, const int*, const uint32_t*, const int*, const double*, const double*,
+                                          double*, double*, const ColState*, hipStream_t, int*);
+template int runs_spmv_cg<float, float>(int, const int*, const uint32_t*, const int*, const float*, const float*,
+                                        float*, double*, const ColState*, hipStream_t, int*);
 
 }  // namespace cc
 

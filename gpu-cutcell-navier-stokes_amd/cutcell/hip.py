@@ -119,6 +119,12 @@ _SIGNATURES = {
     "cc_runs_fill": ([_I32, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_spmv_runs_f64": ([_I32, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
     "cc_spmv_runs_f32": ([_I32, _P, _P, _P, _P, _P, _P, _P], ctypes.c_int),
+    "cc_pcg_jacobi_f64_runs": (
+        [_I32, _P, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int,
+    ),
+    "cc_pcg_jacobi_mixed_runs": (
+        [_I32, _P, _P, _P, _P, _P, _P, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int,
+    ),
     "cc_pcg_jacobi_f64_rb": (
         [_I32, _I32, _P, _P, _P, _P, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P, _P], ctypes.c_int,
     ),
@@ -525,6 +531,7 @@ def pcg(
     history: bool = False,
     tune: bool = False,
     rb: DeviceRowBlocks | None = None,
+    runs: "DeviceRuns | None" = None,
 ):
     """Device Jacobi-PCG on ``k`` right-hand sides (``B`` is ``[n]`` or ``[n, k]``).
 
@@ -532,7 +539,9 @@ def pcg(
     (and, with ``history=True`` for ``precision="fp64"``, a ``[maxiter+1, k]``
     array of ``||r_j||/||b||`` per iteration, NaN past each column's stop).
     ``precision="mixed"`` runs float32 inner PCG under float64 refinement.
-    With ``rb`` (the row-block form of ``A``) the CG products use it.
+    With ``rb`` (the row-block form of ``A``) the CG products use it; with
+    ``runs`` (the run encoding of ``A``'s rows, one right-hand side only) they
+    come from the runs.
     """
     lib = require_gpu()
     torch = _torch()
@@ -554,7 +563,23 @@ def pcg(
     ws = (workspace or _default_ws).get(nbytes, A.vals.device)
     res = (PcgResult * k)()
     hist = None
-    if mode == 0:
+    if runs is not None and k != 1:
+        raise ValueError("the run-encoded products serve one right-hand side")
+    if runs is not None and mode == 0:
+        if history:
+            hist = torch.full((int(maxiter) + 1, k), float("nan"), dtype=torch.float64, device=A.vals.device)
+        st = lib.cc_pcg_jacobi_f64_runs(
+            A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(runs.rptr), _ptr(runs.runs), _ptr(B2), _ptr(X2),
+            float(tol), int(maxiter), prec, int(check_every), _ptr(ws), nbytes, ctypes.addressof(res),
+            _ptr(hist) if hist is not None else None, _stream(),
+        )
+    elif runs is not None:
+        st = lib.cc_pcg_jacobi_mixed_runs(
+            A.n, _ptr(A.indptr), _ptr(A.indices), _ptr(A.vals), _ptr(A.ensure_f32()), _ptr(runs.rptr),
+            _ptr(runs.runs), _ptr(B2), _ptr(X2), float(tol), int(maxiter), float(inner_tol), int(max_outer), prec,
+            int(check_every), _ptr(ws), nbytes, ctypes.addressof(res), _stream(),
+        )
+    elif mode == 0:
         if history:
             hist = torch.full((int(maxiter) + 1, k), float("nan"), dtype=torch.float64, device=A.vals.device)
         if rb is not None:

@@ -309,6 +309,22 @@ int cc_spmv_runs_f64(int32_t n, const int32_t* rptr, const uint32_t* runs, const
                      const double* x, double* y, cc_stream_t stream);
 int cc_spmv_runs_f32(int32_t n, const int32_t* rptr, const uint32_t* runs, const int32_t* indptr, const float* vals,
                      const float* x, float* y, cc_stream_t stream);
+/* cc_pcg_jacobi_f64 / cc_pcg_jacobi_mixed for one right-hand side with the
+ * CG products y = A p taken from the runs of A's rows (rptr, runs of
+ * cc_runs_fill) and its CSR values: the reference's GPU loop
+ * (code/fast_solver_backends.py:233-274) with its product
+ * (code/fast_solver_backends.py:250) from the run encoding. The diagonal and
+ * the float64 true residual come from the CSR arrays. Workspace as
+ * cc_pcg_workspace_bytes(n, 1, precision). */
+int cc_pcg_jacobi_f64_runs(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                           const int32_t* rptr, const uint32_t* runs, const double* B, double* X, double tol,
+                           int32_t maxiter, int32_t preconditioner, int32_t check_every, void* workspace,
+                           size_t workspace_bytes, cc_pcg_result* results, double* history, cc_stream_t stream);
+int cc_pcg_jacobi_mixed_runs(int32_t n, const int32_t* indptr, const int32_t* indices, const double* vals,
+                             const float* vals32, const int32_t* rptr, const uint32_t* runs, const double* B,
+                             double* X, double tol, int32_t maxiter, double inner_tol, int32_t max_outer,
+                             int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
+                             cc_pcg_result* results, cc_stream_t stream);
 /* cc_pcg_jacobi_f64 / cc_pcg_jacobi_mixed with the CG products taken from
  * the row-block form rb of the same matrix (rb->gval for float64,
  * rb->gval32 for the mixed inner solve); the diagonal and the float64 true
