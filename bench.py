@@ -52,13 +52,12 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the headline SpMM on this workload, from the
 # rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised in
 # profiles/spmm_planar81k_p3_k16.md (2.0 FETCH + 1.0 WRITE, calibrated on a
-# 256 MiB copy); keyed by kernel, nnz and n so that another workload reports
-# null.
+# 256 MiB copy in the same runs), taken on the kernels this file launches;
+# keyed by kernel, nnz and n so that another workload reports null.
 PROFILED_TRAFFIC = {
-    ("k_spmm<float,k=16>", 27369621, 81195): 242_892_000,
-    ("k_spmm<double,k=16>", 27369621, 81195): 481_145_000,
-    ("k_spmm_rb<float,k=16>", 27369621, 81195): 225_142_000,
-    ("k_spmm_rb<double,k=16>", 27369621, 81195): 453_754_000,
+    ("k_spmm_rb<float,k=16>", 27369621, 81195): 235_733_000,
+    ("k_spmm_rb<double,k=16>", 27369621, 81195): 479_566_000,
+    ("k_spmm<double,k=16>", 27369621, 81195): 494_493_000,
 }
 
 
