@@ -14,12 +14,19 @@
 // of the hierarchy and every grid of a batch; a block finds its level from
 // the per-level block offsets.
 //
-// Tiles: a block of 8 waves owns 16 x 8 x 8 (x, y, z) output voxels. Wave
+// Tiles: a block of 4 waves owns 16 x 8 x 4 (x, y, z) output voxels. Wave
 // (wy, wz) computes 4 rows (y) in each of 2 planes (z) of 16 x-consecutive
 // voxels; each 16-voxel row is one MFMA accumulator. The tile's input halo
-// (18 x 10 x 10 voxels, all 16 channels, bf16) sits in LDS as two channel
-// halves [half][voxel][8 channels], so a lane's MFMA operand is one 16-byte
-// read and the 16 lanes of an operand row read 256 consecutive bytes.
+// (18 x 10 x 6 voxels, all 16 channels, bf16) sits in LDS as two channel
+// halves per plane [half][voxel][8 channels], so a lane's MFMA operand is one
+// 16-byte read and the 16 lanes of an operand row read 256 consecutive bytes.
+//
+// Halo planes live in a ring of LDS plane slots. A block runs consecutive
+// z-tiles of a column, and the next tile's halo shares two planes with the
+// current one: those stay in their slots and only the four new planes are
+// loaded (six at the start of a column). The first pass keeps its u planes
+// and its hidden-layer planes in rings the same way, so W0 runs once per
+// halo plane of a column, not once per tile.
 //
 // MFMA v_mfma_f32_16x16x32_bf16 (wave64), lane l = j + 16 q (j < 16, q < 4):
 //   A (16 x 32, M x K): a[i] = A[j][8 q + i]
@@ -76,10 +83,29 @@ constexpr int UV = UP * UZ;
 // one plane (and one channel half) into consecutive LDS slots.
 constexpr int HSEG = (HP + 63) / 64;  // 3
 constexpr int USEG = (UP + 63) / 64;  // 4
-constexpr int NDMA16 = HZ * 2 * HSEG;  // 16-byte instructions per 16-channel halo
-constexpr int NDMAU = UZ * USEG;       // 4-byte instructions per u-halo
-constexpr int NMASK = HZ * HSEG;       // mask loads per halo (one byte per lane)
+constexpr int NMASK = HZ * HSEG;       // mask loads of a whole halo (one byte per lane)
 constexpr int MREG = (NMASK + NW - 1) / NW;
+constexpr int PL16 = 2 * HP;  // 16-byte slots of one 16-channel halo plane [half][voxel]
+// Ring slots. Halo plane P (0 .. HZ-1) of an item sits in slot (zc + P) mod
+// the ring size, and the plane counter zc advances by TZ when the next item
+// continues the column (its planes 0, 1 are the current planes TZ, TZ+1) and
+// by HZ otherwise. The next item's new planes are loaded while the current
+// item is computed, so they must land in slots the current item does not use:
+// zc + HZ .. zc + 2 HZ - 1 at most, hence 2 HZ slots for the input ring and a
+// power of two at least that for the mask ring. The hidden-layer ring of the
+// first pass is written only after the block has finished the current item,
+// so it needs HZ slots plus the rounding to a power of two. The u ring steps
+// by TZ or UZ, with 2 UZ slots. ZMOD, a multiple of every ring size, keeps the
+// counters small.
+constexpr int RING = 2 * HZ;   // 12
+constexpr int MRING = 16;
+constexpr int HRING = 8;
+constexpr int URING = 2 * UZ;  // 16
+constexpr int ZMOD = 48;
+static_assert(MRING >= 2 * HZ && HRING >= HZ && URING >= 2 * UZ, "ring sizes");
+static_assert(ZMOD % RING == 0 && ZMOD % MRING == 0 && ZMOD % HRING == 0, "counter modulus");
+static_assert((MRING & (MRING - 1)) == 0 && (HRING & (HRING - 1)) == 0 && (URING & (URING - 1)) == 0,
+              "power-of-two rings");
 
 __device__ inline uint16_t to_bf(float f) {
   uint32_t u;
@@ -218,16 +244,12 @@ __device__ inline void dma(const void* base, uint32_t byte_off, void* lds_base) 
 
 template <int MODE>
 __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
-  // Two input buffers (the next item's input lands in one while the other is
-  // read): the 16-channel halo as [half][voxel] 16-byte slots, or for the
-  // first pass the float32 u-halo. The first pass computes its hidden layer
-  // into hh. Two mask buffers follow the input buffers.
-  constexpr int INSLOTS = MODE == kFirst ? (UV + 3) / 4 : 2 * HV;
-  __shared__ __align__(16) u32x4 in0[INSLOTS];
-  __shared__ __align__(16) u32x4 in1[INSLOTS];
-  __shared__ __align__(16) u32x4 hh[MODE == kFirst ? 2 * HV : 1];
-  __shared__ __align__(16) uint8_t mk0[HV];
-  __shared__ __align__(16) uint8_t mk1[HV];
+  // Input ring: 16-channel halo planes as [slot][half][voxel] 16-byte slots.
+  // The first pass instead holds float32 u planes in uring and computes its
+  // hidden-layer planes into ring. Halo masks, one byte per voxel, in mring.
+  __shared__ __align__(16) u32x4 ring[(MODE == kFirst ? HRING : RING) * PL16];
+  __shared__ __align__(16) float uring[MODE == kFirst ? URING * UP : 4];
+  __shared__ __align__(16) uint8_t mring[MRING * HP];
 
   const int lb = logical_block((int)blockIdx.x, (int)gridDim.x);
   if (lb >= L.nblocks) return;
@@ -235,7 +257,6 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
 #pragma unroll 1
   while (l + 1 < L.nlev && lb >= L.lv[l + 1].block0) ++l;
   const NetLevel& V = L.lv[l];
-  const int nb = L.nb;
   const int nx = V.nx, ny = V.ny, nz = V.nz;
   const int nxy = nx * ny;
   const int px = V.px, py = V.py, pz = V.pz;
@@ -281,10 +302,10 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       WB2[dz] = pack8(h2);
     }
   }
-  // First pass: A = W0[tap = 8q + i][0][co = j] (taps past 26 zero), and the
-  // u-halo offsets of this lane's 8 taps.
+  // First pass: A = W0[tap = 8q + i][0][co = j] (taps past 26 zero), and for
+  // this lane's 8 taps the plane step dz and the in-plane u offset.
   bf16x8 W0A;
-  int toff[8];
+  int tdz[8], tpl[8];
   if (MODE == kFirst) {
     uint16_t h[8];
 #pragma unroll
@@ -293,16 +314,17 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       const int tt = t < 27 ? t : 26;
       h[i] = V.w0[tt * 256 + j];
       if (t >= 27) h[i] = 0;
-      toff[i] = (tt / 9) * UP + ((tt / 3) % 3) * UX + tt % 3;
+      tdz[i] = tt / 9;
+      tpl[i] = ((tt / 3) % 3) * UX + tt % 3;
     }
     W0A = pack8(h);
   }
 
   // Per tile column (tx, ty) and lane: the in-plane grid offset gy * nx + gx
   // of halo plane position sg * 64 + lane (hoff) and of u-halo position
-  // (uoff), with a bit per entry for "inside the plane and the grid". They change only when a
-  // block moves to another column; within a column every address is a
-  // per-plane scalar base plus these offsets.
+  // (uoff), with a bit per entry for "inside the plane and the grid". They
+  // change only when a block moves to another column; within a column every
+  // address is a per-plane scalar base plus these offsets.
   uint32_t hoff[HSEG], uoff[MODE == kFirst ? USEG : 1];
   uint32_t hok = 0, uok = 0;
   int col_tx = -1, col_ty = -1;
@@ -334,23 +356,27 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       }
     }
   };
+  auto in_slot = [](int c) { return MODE == kFirst ? (c & (HRING - 1)) : (c % RING); };
 
+  // Issue the loads of item P's halo planes h0 .. HZ-1 (h0 = 2 when P
+  // continues the current item's column) into the ring slots of plane
+  // counter zcn (u counter ucn): LDS-DMA of the input, zeros written directly
+  // where the halo leaves a clamped axis, and the halo mask into registers
+  // (written to LDS by put_mask).
   uint32_t mreg[MREG];
-  // Issue the loads of item P into input buffer BUF: LDS-DMA of the input
-  // (zeros written directly where the halo leaves a clamped axis) and the
-  // halo mask into registers (written to LDS by put_mask).
-  auto fetch = [&](const ItemPos& P, auto bufc) {
-    constexpr int BUF = decltype(bufc)::value;
-    u32x4* in = BUF ? in1 : in0;
+  int mk_h0 = 0, mk_zc = 0;
+  auto fetch = [&](const ItemPos& P, int h0, int zcn, int ucn) {
     column_tables(P);
     const int z0 = P.tz * TZ;
     if (MODE == kFirst) {
+      // New h planes h0 .. HZ-1 read u planes 2 h0 / TZ ... : 2 h0 is 0 or TZ.
+      const int u0 = h0 ? TZ : 0;
       const float* __restrict__ src = (const float*)V.in + (size_t)P.b * vox;
 #pragma unroll 1
-      for (int k = wave; k < NDMAU; k += NW) {
-        const int uz = k / USEG, sg = k % USEG;
+      for (int k = wave; k < (UZ - u0) * USEG; k += NW) {
+        const int uz = u0 + k / USEG, sg = k % USEG;
         const int gz = wrap_index(z0 - 2 + uz, nz, pz);
-        float* slots = (float*)in + uz * UP + sg * 64;
+        float* slots = uring + ((ucn + uz) & (URING - 1)) * UP + sg * 64;
         if (sg * 64 + lane < UP) {
           if (gz >= 0 && ((uok >> sg) & 1))
             dma<4>(src + (size_t)gz * nxy, uoff[sg] * 4u, slots);
@@ -361,10 +387,10 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
     } else {
       const u32x4* __restrict__ src = (const u32x4*)V.in + (size_t)P.b * vox * 2;
 #pragma unroll 1
-      for (int k = wave; k < NDMA16; k += NW) {
-        const int hz = k / (2 * HSEG), h = (k / HSEG) & 1, sg = k % HSEG;
+      for (int k = wave; k < (HZ - h0) * 2 * HSEG; k += NW) {
+        const int hz = h0 + k / (2 * HSEG), h = (k / HSEG) & 1, sg = k % HSEG;
         const int gz = wrap_index(z0 - 1 + hz, nz, pz);
-        u32x4* slots = in + h * HV + hz * HP + sg * 64;
+        u32x4* slots = ring + in_slot(zcn + hz) * PL16 + h * HP + sg * 64;
         if (sg * 64 + lane < HP) {
           if (gz >= 0 && ((hok >> sg) & 1))
             dma<16>(src + 2 * (size_t)gz * nxy + h, hoff[sg] * 32u, slots);
@@ -373,27 +399,27 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
         }
       }
     }
+    mk_h0 = h0;
+    mk_zc = zcn;
 #pragma unroll
     for (int km = 0; km < MREG; ++km) {
       const int k = wave + NW * km;
       uint32_t m = 0;
-      if (k < NMASK) {
-        const int hz = k / HSEG, sg = k % HSEG;
+      if (k < (HZ - h0) * HSEG) {
+        const int hz = h0 + k / HSEG, sg = k % HSEG;
         const int gz = wrap_index(z0 - 1 + hz, nz, pz);
         if (gz >= 0 && ((hok >> sg) & 1)) m = (mask + (size_t)gz * nxy)[hoff[sg]];
       }
       mreg[km] = m;
     }
   };
-  auto put_mask = [&](auto bufc) {
-    constexpr int BUF = decltype(bufc)::value;
-    uint8_t* mk = BUF ? mk1 : mk0;
+  auto put_mask = [&]() {
 #pragma unroll
     for (int km = 0; km < MREG; ++km) {
       const int k = wave + NW * km;
-      if (k < NMASK) {
-        const int hz = k / HSEG, sg = k % HSEG;
-        if (sg * 64 + lane < HP) mk[hz * HP + sg * 64 + lane] = (uint8_t)(mreg[km] != 0);
+      if (k < (HZ - mk_h0) * HSEG) {
+        const int hz = mk_h0 + k / HSEG, sg = k % HSEG;
+        if (sg * 64 + lane < HP) mring[((mk_zc + hz) & (MRING - 1)) * HP + sg * 64 + lane] = (uint8_t)(mreg[km] != 0);
       }
     }
   };
@@ -429,40 +455,41 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
 
   ItemPos cur = item_decode(V, it0);
   ItemPos nxt = cur;
-  ItemPos prev = cur;
-
-  auto process = [&](int it, auto bufc) {
-    constexpr int BUF = decltype(bufc)::value;
-    const u32x4* inb = BUF ? in1 : in0;
-    const uint8_t* mk = BUF ? mk1 : mk0;
+  int zc = 0, uc = 0, h0cur = 0;
+  fetch(cur, 0, zc, uc);
+#pragma unroll 1
+  for (int it = it0; it < it1; ++it) {
     // This item's input (issued one item ago) has landed; the block then
     // sees every wave's share of it.
     wait_vmem();
-    put_mask(bufc);
+    put_mask();
     lds_barrier();
+    int zn = zc, un = uc, h0n = 0;
     if (it + 1 < it1) {
       item_next(V, nxt);
-      fetch(nxt, std::integral_constant<int, 1 - BUF>{});
+      // item_next steps z first: a nonzero z-tile continues this column.
+      h0n = nxt.tz != 0 ? 2 : 0;
+      zn = (zc + (h0n ? TZ : HZ)) % ZMOD;
+      un = (uc + (h0n ? TZ : UZ)) % ZMOD;
+      fetch(nxt, h0n, zn, un);
     }
     if (it > it0) store_pending();
-    prev = cur;  // the item computed now
-    cur = nxt;
 
-    const u32x4* src = inb;
     if (MODE == kFirst) {
-      // h = M (W0 * u) on the halo voxels, 16 per MFMA: B = the 27 taps of u
-      // (K, padded to 32) for voxel j of the group.
-      const float* uf = (const float*)inb;
+      // h = M (W0 * u) on the new halo planes h0cur .. HZ-1, 16 voxels per
+      // MFMA: B = the 27 taps of u (K, padded to 32) for voxel j of the group.
+      const int hv0 = h0cur * HP, nv = HV - hv0;
 #pragma unroll 1
-      for (int grp = wave; grp < (HV + 15) / 16; grp += NW) {
-        const int hv = grp * 16 + j;
+      for (int grp = wave; grp < (nv + 15) / 16; grp += NW) {
+        const int hv = hv0 + grp * 16 + j;
         const int hc = hv < HV ? hv : HV - 1;
-        const int hx = hc % HX, hy = (hc / HX) % HY, hz = hc / HP;
-        const int base = (hz * UY + hy) * UX + hx;
+        const int hz = hc / HP, rem = hc - hz * HP;
+        const int hy = rem / HX, hx = rem - hy * HX;
+        const int inpl = hy * UX + hx;
         float uv[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-          uv[i] = uf[base + toff[i]];
+          uv[i] = uring[((uc + hz + tdz[i]) & (URING - 1)) * UP + inpl + tpl[i]];
           if (8 * q + i >= 27) uv[i] = 0.f;
         }
         u32x4 tv;
@@ -471,18 +498,17 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
         tv[2] = pack_bf16(uv[4], uv[5]);
         tv[3] = pack_bf16(uv[6], uv[7]);
         const f32x4 hacc = mfma32(W0A, as_bf8(tv), f32x4{0.f, 0.f, 0.f, 0.f});
-        const bool live = mk[hc] != 0;
+        const bool live = mring[((zc + hz) & (MRING - 1)) * HP + rem] != 0;
         if (hv < HV) {
           uint2 raw;
           raw.x = live ? pack_bf16(hacc[0], hacc[1]) : 0u;
           raw.y = live ? pack_bf16(hacc[2], hacc[3]) : 0u;
           // channels 4q .. 4q+3: half q / 2, slot 4 (q % 2)
-          uint16_t* dst = reinterpret_cast<uint16_t*>(&hh[(q >> 1) * HV + hv]) + 4 * (q & 1);
+          uint16_t* dst = reinterpret_cast<uint16_t*>(&ring[in_slot(zc + hz) * PL16 + (q >> 1) * HP + rem]) + 4 * (q & 1);
           *reinterpret_cast<uint2*>(dst) = raw;
         }
       }
       lds_barrier();
-      src = hh;
     }
 
     f32x4 acc[PW][RW];
@@ -492,12 +518,14 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       for (int r = 0; r < RW; ++r) acc[p][r] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int hz = 0; hz < PW + 2; ++hz) {
+      const int slot = __builtin_amdgcn_readfirstlane(in_slot(zc + wz * PW + hz));
+      const u32x4* pl = ring + slot * PL16 + (q & 1) * HP;
 #pragma unroll
       for (int Y = 0; Y < RW + 2; ++Y) {
-        const int row = ((wz * PW + hz) * HY + (wy * RW + Y)) * HX;
-        const bf16x8 ra = as_bf8(src[(q & 1) * HV + row + j + (q >> 1)]);
+        const int row = (wy * RW + Y) * HX;
+        const bf16x8 ra = as_bf8(pl[row + j + (q >> 1)]);
         bf16x8 rb = ra;
-        if (Y <= RW) rb = as_bf8(src[(q & 1) * HV + row + (q >> 1) * HX + j + 2]);
+        if (Y <= RW) rb = as_bf8(pl[row + (q >> 1) * HX + j + 2]);
 #pragma unroll
         for (int p = 0; p < PW; ++p) {
           const int dz = hz - p;
@@ -515,21 +543,22 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
 
     // Masked, rounded results stay in registers until the next item's loads
     // are issued.
-    pend_z = prev.tz * TZ + wz * PW;
-    pend_b = prev.b;
+    pend_z = cur.tz * TZ + wz * PW;
+    pend_b = cur.b;
     pend_ok = 0;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
-      const int gx = prev.tx * TX + j, gy = prev.ty * TY + wy * RW + r;
+      const int gx = cur.tx * TX + j, gy = cur.ty * TY + wy * RW + r;
       const bool ok = gx < nx && gy < ny;
       pend_off[r] = ok ? (uint32_t)(gy * nx + gx) : 0u;
       pend_ok |= (uint32_t)ok << r;
     }
 #pragma unroll
-    for (int p = 0; p < PW; ++p)
+    for (int p = 0; p < PW; ++p) {
+      const uint8_t* mp = mring + ((zc + wz * PW + p + 1) & (MRING - 1)) * HP;
 #pragma unroll
       for (int r = 0; r < RW; ++r) {
-        const bool live = mk[((wz * PW + p + 1) * HY + wy * RW + r + 1) * HX + j + 1] != 0;
+        const bool live = mp[(wy * RW + r + 1) * HX + j + 1] != 0;
         if (MODE == kLast) {
           pend_f[p][r] = live ? acc[p][r][0] : 0.f;
         } else {
@@ -537,13 +566,11 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
           pend[p][r].y = live ? pack_bf16(acc[p][r][2], acc[p][r][3]) : 0u;
         }
       }
-  };
-
-  fetch(cur, std::integral_constant<int, 0>{});
-#pragma unroll 1
-  for (int it = it0; it < it1; it += 2) {
-    process(it, std::integral_constant<int, 0>{});
-    if (it + 1 < it1) process(it + 1, std::integral_constant<int, 1>{});
+    }
+    cur = nxt;
+    zc = zn;
+    uc = un;
+    h0cur = h0n;
   }
   store_pending();
 }
@@ -565,10 +592,26 @@ __global__ void k_mfma32_probe(const uint16_t* __restrict__ A, const uint16_t* _
 }
 
 // Blocks per pass launch: the items of every level are split into runs of the
-// same length, about g_net_blocks runs (cc_net_set_config; default two per CU,
-// the most that fit by LDS), so that all blocks run in one round.
-static int g_net_blocks = 512;
+// same length, at most g_net_blocks runs in all levels together
+// (cc_net_set_config; by default two per CU of the current device, the most
+// that fit by LDS), so that every block is resident from the start. A block
+// beyond that starts only when another ends and doubles the launch time.
+static int g_net_blocks = 0;
 static int g_net_legacy = 0;
+constexpr int kBlocksPerCu = 2;
+constexpr int kMaxDevices = 64;
+static int g_resident[kMaxDevices];
+
+static int resident_blocks() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return kBlocksPerCu * 256;
+  if (g_resident[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 1) cus = 256;
+    g_resident[dev] = kBlocksPerCu * cus;
+  }
+  return g_resident[dev];
+}
 
 bool net_pass_legacy() { return g_net_legacy != 0; }
 
@@ -612,8 +655,19 @@ int net_pass(const cc_mlcnn_desc& ml, float* const* u, uint16_t* const* g1, uint
     ++nlev;
   }
   if (nlev == 0) return 0;
-  long long ipb_all = (total + g_net_blocks - 1) / g_net_blocks;
+  // The shortest run length whose per-level block counts (each rounded up)
+  // fit in max_blocks; one block per level when the levels outnumber them.
+  const long long max_blocks = g_net_blocks > 0 ? g_net_blocks : resident_blocks();
+  long long max_items = 1;
+  for (int k = 0; k < nlev; ++k) max_items = L[0].lv[k].items > max_items ? L[0].lv[k].items : max_items;
+  long long ipb_all = (total + max_blocks - 1) / max_blocks;
   if (ipb_all < 1) ipb_all = 1;
+  for (;;) {
+    long long nblk = 0;
+    for (int k = 0; k < nlev; ++k) nblk += (L[0].lv[k].items + ipb_all - 1) / ipb_all;
+    if (nblk <= max_blocks || ipb_all >= max_items) break;
+    ++ipb_all;
+  }
   int block0 = 0;
   for (int k = 0; k < nlev; ++k) {
     const int l = lev_of[k];
@@ -662,7 +716,7 @@ int cc_mfma32_probe(const uint16_t* A, const uint16_t* B, float* D, cc_stream_t 
 
 int cc_net_set_config(int32_t blocks, int32_t legacy) {
   if (blocks < 0 || blocks > (1 << 20) || (legacy != 0 && legacy != 1)) return CC_E_ARG;
-  g_net_blocks = blocks > 0 ? blocks : 512;
+  g_net_blocks = blocks;
   g_net_legacy = legacy;
   return 0;
 }

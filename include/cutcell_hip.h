@@ -643,7 +643,8 @@ int cc_fcg_sumfact_ml_k(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32
  * float32; all on the 16x16x32 bf16 MFMA with fp32 accumulation.
  * cc_mfma32_probe: D = A B for bf16 A (16 x 32) and B (32 x 16), row-major,
  * fp32 D (16 x 16), the lane layout those kernels rely on.
- * cc_net_set_config: blocks per pass launch (0 = 512, two per CU) and
+ * cc_net_set_config: most blocks per pass launch (0 = two per CU of the
+ * current device, all resident at once) and
  * legacy = 1 to run the four per-level layer launches of
  * cc_cnn_conv3_forward instead.
  * cc_net_level_forward: y = C^T C u (channel 0, masked) on nb grids of one
