@@ -104,6 +104,7 @@ constexpr int URING = 2 * UZ;  // 16
 constexpr int ZMOD = 48;
 static_assert(MRING >= 2 * HZ && HRING >= HZ && URING >= 2 * UZ, "ring sizes");
 static_assert(ZMOD % RING == 0 && ZMOD % MRING == 0 && ZMOD % HRING == 0, "counter modulus");
+constexpr int HB = 4;  // first pass: W0 groups a wave computes together
 static_assert((MRING & (MRING - 1)) == 0 && (HRING & (HRING - 1)) == 0 && (URING & (URING - 1)) == 0,
               "power-of-two rings");
 
@@ -478,34 +479,55 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
     if (MODE == kFirst) {
       // h = M (W0 * u) on the new halo planes h0cur .. HZ-1, 16 voxels per
       // MFMA: B = the 27 taps of u (K, padded to 32) for voxel j of the group.
-      const int hv0 = h0cur * HP, nv = HV - hv0;
+      // A wave takes HB groups at a time, their loads first, so that their
+      // LDS and MFMA latencies overlap; a group past the last is computed on
+      // the last voxel and not written.
+      const int hv0 = h0cur * HP, ng = (HV - hv0 + 15) / 16;
 #pragma unroll 1
-      for (int grp = wave; grp < (nv + 15) / 16; grp += NW) {
-        const int hv = hv0 + grp * 16 + j;
-        const int hc = hv < HV ? hv : HV - 1;
-        const int hz = hc / HP, rem = hc - hz * HP;
-        const int hy = rem / HX, hx = rem - hy * HX;
-        const int inpl = hy * UX + hx;
-        float uv[8];
+      for (int g0 = wave; g0 < ng; g0 += NW * HB) {
+        int hvb[HB], remb[HB], hzb[HB];
+        bool liveb[HB];
+        float uv[HB][8];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          uv[i] = uring[((uc + hz + tdz[i]) & (URING - 1)) * UP + inpl + tpl[i]];
-          if (8 * q + i >= 27) uv[i] = 0.f;
+        for (int bi = 0; bi < HB; ++bi) {
+          const int hv = hv0 + (g0 + NW * bi) * 16 + j;
+          hvb[bi] = hv;
+          const int hc = hv < HV ? hv : HV - 1;
+          const int hz = hc / HP, rem = hc - hz * HP;
+          const int hy = rem / HX, hx = rem - hy * HX;
+          hzb[bi] = hz;
+          remb[bi] = rem;
+          const int inpl = hy * UX + hx;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) uv[bi][i] = uring[((uc + hz + tdz[i]) & (URING - 1)) * UP + inpl + tpl[i]];
+          liveb[bi] = mring[((zc + hz) & (MRING - 1)) * HP + rem] != 0;
         }
-        u32x4 tv;
-        tv[0] = pack_bf16(uv[0], uv[1]);
-        tv[1] = pack_bf16(uv[2], uv[3]);
-        tv[2] = pack_bf16(uv[4], uv[5]);
-        tv[3] = pack_bf16(uv[6], uv[7]);
-        const f32x4 hacc = mfma32(W0A, as_bf8(tv), f32x4{0.f, 0.f, 0.f, 0.f});
-        const bool live = mring[((zc + hz) & (MRING - 1)) * HP + rem] != 0;
-        if (hv < HV) {
-          uint2 raw;
-          raw.x = live ? pack_bf16(hacc[0], hacc[1]) : 0u;
-          raw.y = live ? pack_bf16(hacc[2], hacc[3]) : 0u;
-          // channels 4q .. 4q+3: half q / 2, slot 4 (q % 2)
-          uint16_t* dst = reinterpret_cast<uint16_t*>(&ring[in_slot(zc + hz) * PL16 + (q >> 1) * HP + rem]) + 4 * (q & 1);
-          *reinterpret_cast<uint2*>(dst) = raw;
+        f32x4 hacc[HB];
+#pragma unroll
+        for (int bi = 0; bi < HB; ++bi) {
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if (8 * q + i >= 27) uv[bi][i] = 0.f;
+          u32x4 tv;
+          tv[0] = pack_bf16(uv[bi][0], uv[bi][1]);
+          tv[1] = pack_bf16(uv[bi][2], uv[bi][3]);
+          tv[2] = pack_bf16(uv[bi][4], uv[bi][5]);
+          tv[3] = pack_bf16(uv[bi][6], uv[bi][7]);
+          hacc[bi] = mfma32(W0A, as_bf8(tv), f32x4{0.f, 0.f, 0.f, 0.f});
+        }
+#pragma unroll
+        for (int bi = 0; bi < HB; ++bi) {
+          const int hz = hzb[bi], rem = remb[bi];
+          const bool live = liveb[bi];
+          if (hvb[bi] < HV) {
+            uint2 raw;
+            raw.x = live ? pack_bf16(hacc[bi][0], hacc[bi][1]) : 0u;
+            raw.y = live ? pack_bf16(hacc[bi][2], hacc[bi][3]) : 0u;
+            // channels 4q .. 4q+3: half q / 2, slot 4 (q % 2)
+            uint16_t* dst =
+                reinterpret_cast<uint16_t*>(&ring[in_slot(zc + hz) * PL16 + (q >> 1) * HP + rem]) + 4 * (q & 1);
+            *reinterpret_cast<uint2*>(dst) = raw;
+          }
         }
       }
       lds_barrier();
