@@ -104,7 +104,14 @@ constexpr int URING = 2 * UZ;  // 16
 constexpr int ZMOD = 48;
 static_assert(MRING >= 2 * HZ && HRING >= HZ && URING >= 2 * UZ, "ring sizes");
 static_assert(ZMOD % RING == 0 && ZMOD % MRING == 0 && ZMOD % HRING == 0, "counter modulus");
-constexpr int HB = 4;  // first pass: W0 groups a wave computes together
+// First pass: the W0 layer runs plane by plane in groups of 16 in-plane
+// voxels (HG groups, the last partly past the plane); each wave takes HGW
+// groups of every plane. The u ring mirrors its first UMIR slots after the
+// last, so the three u planes of a tap column are linear in LDS.
+constexpr int HG = (HP + 15) / 16;  // 12
+constexpr int HGW = HG / NW;        // 3
+constexpr int UMIR = 2;
+static_assert(HG % NW == 0 && HZ % 2 == 0 && TZ % 2 == 0, "first-pass W0 plane pairs");
 static_assert((MRING & (MRING - 1)) == 0 && (HRING & (HRING - 1)) == 0 && (URING & (URING - 1)) == 0,
               "power-of-two rings");
 
@@ -249,7 +256,7 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
   // The first pass instead holds float32 u planes in uring and computes its
   // hidden-layer planes into ring. Halo masks, one byte per voxel, in mring.
   __shared__ __align__(16) u32x4 ring[(MODE == kFirst ? HRING : RING) * PL16];
-  __shared__ __align__(16) float uring[MODE == kFirst ? URING * UP : 4];
+  __shared__ __align__(16) float uring[MODE == kFirst ? (URING + UMIR) * UP : 4];
   __shared__ __align__(16) uint8_t mring[MRING * HP];
 
   const int lb = logical_block((int)blockIdx.x, (int)gridDim.x);
@@ -303,10 +310,13 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       WB2[dz] = pack8(h2);
     }
   }
-  // First pass: A = W0[tap = 8q + i][0][co = j] (taps past 26 zero), and for
-  // this lane's 8 taps the plane step dz and the in-plane u offset.
+  // First pass: A = W0[tap = 8q + i][0][co = j] (taps past 26 zero); for
+  // this lane's 8 taps the u offset from the tap column's first plane
+  // (toff), and for its W0 groups the in-plane halo voxel (hrem), its u-halo
+  // offset (hin) and whether it lies in the plane (hval, as a bit).
   bf16x8 W0A;
-  int tdz[8], tpl[8];
+  int toff[8], hrem[HGW], hin[HGW];
+  uint32_t hval = 0;
   if (MODE == kFirst) {
     uint16_t h[8];
 #pragma unroll
@@ -315,10 +325,17 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       const int tt = t < 27 ? t : 26;
       h[i] = V.w0[tt * 256 + j];
       if (t >= 27) h[i] = 0;
-      tdz[i] = tt / 9;
-      tpl[i] = ((tt / 3) % 3) * UX + tt % 3;
+      toff[i] = (tt / 9) * UP + ((tt / 3) % 3) * UX + tt % 3;
     }
     W0A = pack8(h);
+#pragma unroll
+    for (int k = 0; k < HGW; ++k) {
+      const int rem = (wave + NW * k) * 16 + j;
+      const int rc = rem < HP ? rem : HP - 1;
+      hrem[k] = rc;
+      hin[k] = (rc / HX) * UX + rc % HX;
+      hval |= (uint32_t)(rem < HP) << k;
+    }
   }
 
   // Per tile column (tx, ty) and lane: the in-plane grid offset gy * nx + gx
@@ -370,19 +387,26 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
     column_tables(P);
     const int z0 = P.tz * TZ;
     if (MODE == kFirst) {
-      // New h planes h0 .. HZ-1 read u planes 2 h0 / TZ ... : 2 h0 is 0 or TZ.
+      // The new h planes h0 .. HZ-1 read u planes h0 .. UZ-1; for a tile that
+      // continues its column, u planes h0 .. TZ-1 are the current tile's
+      // planes TZ+h0 .. 2 TZ-1, already in the ring.
       const int u0 = h0 ? TZ : 0;
       const float* __restrict__ src = (const float*)V.in + (size_t)P.b * vox;
 #pragma unroll 1
       for (int k = wave; k < (UZ - u0) * USEG; k += NW) {
         const int uz = u0 + k / USEG, sg = k % USEG;
         const int gz = wrap_index(z0 - 2 + uz, nz, pz);
-        float* slots = uring + ((ucn + uz) & (URING - 1)) * UP + sg * 64;
+        const int us = (ucn + uz) & (URING - 1);
+        float* slots = uring + us * UP + sg * 64;
+        float* mirror = uring + (us + URING) * UP + sg * 64;
         if (sg * 64 + lane < UP) {
-          if (gz >= 0 && ((uok >> sg) & 1))
+          if (gz >= 0 && ((uok >> sg) & 1)) {
             dma<4>(src + (size_t)gz * nxy, uoff[sg] * 4u, slots);
-          else
+            if (us < UMIR) dma<4>(src + (size_t)gz * nxy, uoff[sg] * 4u, mirror);
+          } else {
             slots[lane] = 0.f;
+            if (us < UMIR) mirror[lane] = 0.f;
+          }
         }
       }
     } else {
@@ -477,56 +501,50 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
     if (it > it0) store_pending();
 
     if (MODE == kFirst) {
-      // h = M (W0 * u) on the new halo planes h0cur .. HZ-1, 16 voxels per
-      // MFMA: B = the 27 taps of u (K, padded to 32) for voxel j of the group.
-      // A wave takes HB groups at a time, their loads first, so that their
-      // LDS and MFMA latencies overlap; a group past the last is computed on
-      // the last voxel and not written.
-      const int hv0 = h0cur * HP, ng = (HV - hv0 + 15) / 16;
+      // h = M (W0 * u) on the new halo planes h0cur .. HZ-1, two planes at a
+      // time, 16 voxels per MFMA: B = the 27 taps of u (K, padded to 32) for
+      // voxel j of the group. All loads of the two planes go out first.
 #pragma unroll 1
-      for (int g0 = wave; g0 < ng; g0 += NW * HB) {
-        int hvb[HB], remb[HB], hzb[HB];
-        bool liveb[HB];
-        float uv[HB][8];
+      for (int hz = h0cur; hz < HZ; hz += 2) {
+        float uv[2][HGW][8];
+        bool live[2][HGW];
 #pragma unroll
-        for (int bi = 0; bi < HB; ++bi) {
-          const int hv = hv0 + (g0 + NW * bi) * 16 + j;
-          hvb[bi] = hv;
-          const int hc = hv < HV ? hv : HV - 1;
-          const int hz = hc / HP, rem = hc - hz * HP;
-          const int hy = rem / HX, hx = rem - hy * HX;
-          hzb[bi] = hz;
-          remb[bi] = rem;
-          const int inpl = hy * UX + hx;
+        for (int a = 0; a < 2; ++a) {
+          const float* ub = uring + ((uc + hz + a) & (URING - 1)) * UP;
+          const uint8_t* mb = mring + ((zc + hz + a) & (MRING - 1)) * HP;
 #pragma unroll
-          for (int i = 0; i < 8; ++i) uv[bi][i] = uring[((uc + hz + tdz[i]) & (URING - 1)) * UP + inpl + tpl[i]];
-          liveb[bi] = mring[((zc + hz) & (MRING - 1)) * HP + rem] != 0;
+          for (int k = 0; k < HGW; ++k) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) uv[a][k][i] = ub[hin[k] + toff[i]];
+            live[a][k] = mb[hrem[k]] != 0;
+          }
         }
-        f32x4 hacc[HB];
+        f32x4 hacc[2][HGW];
 #pragma unroll
-        for (int bi = 0; bi < HB; ++bi) {
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-          for (int i = 0; i < 8; ++i)
-            if (8 * q + i >= 27) uv[bi][i] = 0.f;
-          u32x4 tv;
-          tv[0] = pack_bf16(uv[bi][0], uv[bi][1]);
-          tv[1] = pack_bf16(uv[bi][2], uv[bi][3]);
-          tv[2] = pack_bf16(uv[bi][4], uv[bi][5]);
-          tv[3] = pack_bf16(uv[bi][6], uv[bi][7]);
-          hacc[bi] = mfma32(W0A, as_bf8(tv), f32x4{0.f, 0.f, 0.f, 0.f});
-        }
+          for (int k = 0; k < HGW; ++k) {
 #pragma unroll
-        for (int bi = 0; bi < HB; ++bi) {
-          const int hz = hzb[bi], rem = remb[bi];
-          const bool live = liveb[bi];
-          if (hvb[bi] < HV) {
+            for (int i = 0; i < 8; ++i)
+              if (8 * q + i >= 27) uv[a][k][i] = 0.f;
+            u32x4 tv;
+            tv[0] = pack_bf16(uv[a][k][0], uv[a][k][1]);
+            tv[1] = pack_bf16(uv[a][k][2], uv[a][k][3]);
+            tv[2] = pack_bf16(uv[a][k][4], uv[a][k][5]);
+            tv[3] = pack_bf16(uv[a][k][6], uv[a][k][7]);
+            hacc[a][k] = mfma32(W0A, as_bf8(tv), f32x4{0.f, 0.f, 0.f, 0.f});
+          }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          // channels 4q .. 4q+3: half q / 2, slot 4 (q % 2)
+          uint16_t* hb = reinterpret_cast<uint16_t*>(ring + in_slot(zc + hz + a) * PL16 + (q >> 1) * HP) + 4 * (q & 1);
+#pragma unroll
+          for (int k = 0; k < HGW; ++k) {
+            if (!((hval >> k) & 1)) continue;
             uint2 raw;
-            raw.x = live ? pack_bf16(hacc[bi][0], hacc[bi][1]) : 0u;
-            raw.y = live ? pack_bf16(hacc[bi][2], hacc[bi][3]) : 0u;
-            // channels 4q .. 4q+3: half q / 2, slot 4 (q % 2)
-            uint16_t* dst =
-                reinterpret_cast<uint16_t*>(&ring[in_slot(zc + hz) * PL16 + (q >> 1) * HP + rem]) + 4 * (q & 1);
-            *reinterpret_cast<uint2*>(dst) = raw;
+            raw.x = live[a][k] ? pack_bf16(hacc[a][k][0], hacc[a][k][1]) : 0u;
+            raw.y = live[a][k] ? pack_bf16(hacc[a][k][2], hacc[a][k][3]) : 0u;
+            *reinterpret_cast<uint2*>(hb + 8 * hrem[k]) = raw;
           }
         }
       }
