@@ -556,16 +556,25 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
     for (int p = 0; p < PW; ++p)
 #pragma unroll
       for (int r = 0; r < RW; ++r) acc[p][r] = f32x4{0.f, 0.f, 0.f, 0.f};
+    // Per halo plane the wave's operands (ra of RW + 2 rows, rb of RW + 1)
+    // are read together, then consumed by the plane's MFMAs: with two waves
+    // per SIMD, a read issued just before its MFMA leaves the LDS latency
+    // exposed.
 #pragma unroll
     for (int hz = 0; hz < PW + 2; ++hz) {
       const int slot = __builtin_amdgcn_readfirstlane(in_slot(zc + wz * PW + hz));
       const u32x4* pl = ring + slot * PL16 + (q & 1) * HP;
+      bf16x8 ra[RW + 2], rb[RW + 1];
 #pragma unroll
       for (int Y = 0; Y < RW + 2; ++Y) {
         const int row = (wy * RW + Y) * HX;
-        const bf16x8 ra = as_bf8(pl[row + j + (q >> 1)]);
-        bf16x8 rb = ra;
-        if (Y <= RW) rb = as_bf8(pl[row + (q >> 1) * HX + j + 2]);
+        ra[Y] = as_bf8(pl[row + j + (q >> 1)]);
+        if (Y <= RW) rb[Y] = as_bf8(pl[row + (q >> 1) * HX + j + 2]);
+      }
+      // Keeps the scheduler from sinking the reads next to their MFMAs.
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int Y = 0; Y < RW + 2; ++Y) {
 #pragma unroll
         for (int p = 0; p < PW; ++p) {
           const int dz = hz - p;
@@ -573,10 +582,10 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
 #pragma unroll
           for (int r = 0; r < RW; ++r) {
             const int dy = Y - r;
-            if (dy >= 0 && dy <= 2) acc[p][r] = mfma32(WA[dz * 3 + dy], ra, acc[p][r]);
+            if (dy >= 0 && dy <= 2) acc[p][r] = mfma32(WA[dz * 3 + dy], ra[Y], acc[p][r]);
           }
-          if (Y < RW) acc[p][Y] = mfma32(WB1[dz], rb, acc[p][Y]);
-          if (Y >= 1 && Y <= RW) acc[p][Y - 1] = mfma32(WB2[dz], rb, acc[p][Y - 1]);
+          if (Y < RW) acc[p][Y] = mfma32(WB1[dz], rb[Y], acc[p][Y]);
+          if (Y >= 1 && Y <= RW) acc[p][Y - 1] = mfma32(WB2[dz], rb[Y], acc[p][Y - 1]);
         }
       }
     }
