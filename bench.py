@@ -969,6 +969,17 @@ def main(argv=None) -> int:
         from cutcell import learned
 
         hip._check(learned._lib().cc_net_set_config(args.net_blocks, int(args.net_legacy)), "net_set_config")
+    if args.copy_probe_mb > 0:
+        # A copy of known size: its FETCH_SIZE / WRITE_SIZE counters calibrate
+        # the traffic of the kernels profiled in the same run.
+        words = int(args.copy_probe_mb * 2**20) // 8
+        src = torch.ones(words, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {words * 8} bytes read and written once")
+        del src, dst
     if args.operator_probe:
         print(json.dumps(operator_probe(hip, torch, args)))
         return 0
@@ -982,15 +993,6 @@ def main(argv=None) -> int:
     Ad = hip.DeviceCSR.from_scipy(A, with_f32=True)
     Bd = torch.from_numpy(B).to("cuda")
     torch.cuda.synchronize()
-    if args.copy_probe_mb > 0:
-        words = int(args.copy_probe_mb * 2**20) // 8
-        src = torch.ones(words, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"copy probe: {words * 8} bytes read and written once")
-        del src, dst
 
     rb, rb_build_s = None, None
     if any(m.endswith("+rb") for m in args.methods.split(",")):
