@@ -91,6 +91,17 @@ int cg_operator_multi(int n, int k, const std::function<int(const double*, doubl
                       int check_every, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s,
                       const std::function<int(const double*, double*, double*)>& residual, size_t tail = 0);
 size_t cg_operator_multi_bytes(size_t n, int k, size_t tail = 0);
+// FP32 inner / FP64 outer refinement over an operator for k in {1, 2, 4, 8,
+// 16} columns: residual(X, R64, partial, &nparts) writes R64 = B - A X with
+// per-block, per-column partials of R64.R64 (at most kMaxParts blocks);
+// apply32(P, Q, partial, &nparts) is the float32 operator with per-block
+// partials of P.Q. X is zeroed first (pcg.hip).
+int cg_operator_mixed_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& residual,
+                            const std::function<int(const float*, float*, double*, int*)>& apply32,
+                            const double* diag_src, double* X, double tol, int maxiter, double inner_tol,
+                            int max_outer, int precond, int check_every, void* ws, size_t wsb,
+                            cc_pcg_result* results, hipStream_t s);
+size_t cg_operator_mixed_multi_bytes(size_t n, int k);
 // Y = A X (CSR, float64, k in {1, 2, 4, 8, 16}) with the autotuned launch
 // shape; partial[block][c] gets per-block X.Y sums when not null (pcg.hip).
 int csr_spmm_f64(int n, int k, double avg_row, const int* indptr, const int* indices, const double* vals,

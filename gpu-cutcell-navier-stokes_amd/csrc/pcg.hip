@@ -1433,6 +1433,102 @@ static int pcg_mixed(int n, const int* indptr, const int* indices, const double*
   return 0;
 }
 
+// FP32 inner / FP64 outer refinement over an operator, as pcg_mixed over a
+// CSR matrix: residual(X, R64, partial, &nparts) writes R64 = B - A X in
+// float64 with per-block, per-column partials of R64.R64 ([nparts][K],
+// nparts <= kMaxParts); apply32(P, Q, partial, &nparts) is the float32
+// operator with per-block partials of P.Q. The inner solves start from zero
+// on the scaled residual and stop at inner_tol; the outer loop stops on the
+// float64 residual.
+template <int K>
+static int mixed_operator_k(int n, const std::function<int(const double*, double*, double*, int*)>& residual,
+                            const std::function<int(const float*, float*, double*, int*)>& apply32,
+                            const double* diag_src, double* X, double tol, int maxiter, double inner_tol,
+                            int max_outer, int precond, int check_every, void* ws, size_t wsb,
+                            cc_pcg_result* results, hipStream_t s) {
+  Carve cv{(char*)ws, 0, wsb};
+  CgBuffers<float> b = carve_cg<float>(cv, n, K, nullptr);
+  double* R64 = cv.take<double>((size_t)n * K);
+  double* pr = cv.take<double>((size_t)kMaxParts * K);
+  OuterState* ost = cv.take<OuterState>(K);
+  int* run_mask = cv.take<int>(K);
+  double* inv_scale = cv.take<double>(K);
+  if (!cv.ok()) return CC_E_WORKSPACE;
+  const int pc = precond && diag_src ? 1 : 0;
+  if (pc) CC_CHECK(hipMemcpyAsync(b.diag, diag_src, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  const int sg = stream_grid((size_t)n * K);
+  hipLaunchKernelGGL(k_zero, dim3(sg), dim3(kBlock), 0, s, X, (size_t)n * K);
+  int gs = 0;
+  CC_CHECK(residual(X, R64, pr, &gs));
+  if (gs < 1 || gs > kMaxParts) return CC_E_ARG;
+  hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 1);
+  CC_CHECK(hipGetLastError());
+  std::vector<OuterState> ho(K);
+  std::vector<ColState> hs;
+  int used = 0;
+  for (int sweep = 0; sweep < max_outer; ++sweep) {
+    CC_CHECK(hipMemcpyAsync(ho.data(), ost, K * sizeof(OuterState), hipMemcpyDeviceToHost, s));
+    CC_CHECK(hipStreamSynchronize(s));
+    bool any = false;
+    for (int c = 0; c < K; ++c) any |= ho[c].status == CC_RUNNING;
+    if (!any || used >= maxiter) break;
+    CC_CHECK((cg_loop<float, K>(n, apply32, R64, inv_scale, run_mask, b, pc, inner_tol, maxiter - used, check_every,
+                                s, hs, nullptr)));
+    int most = 0;
+    for (int c = 0; c < K; ++c)
+      if (ho[c].status == CC_RUNNING && hs[c].iterations > most) most = hs[c].iterations;
+    used += most > 0 ? most : 1;
+    hipLaunchKernelGGL((k_outer_correct<K>), dim3(sg), dim3(kBlock), 0, s, n, X, b.X, ost, run_mask);
+    hipLaunchKernelGGL((k_outer_account<K>), dim3(1), dim3(64), 0, s, ost, b.st, run_mask);
+    CC_CHECK(residual(X, R64, pr, &gs));
+    if (gs < 1 || gs > kMaxParts) return CC_E_ARG;
+    hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 0);
+    CC_CHECK(hipGetLastError());
+  }
+  CC_CHECK(hipMemcpyAsync(ho.data(), ost, K * sizeof(OuterState), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c) {
+    results[c].iterations = ho[c].inner_total;
+    results[c].status = ho[c].status == CC_CONVERGED ? CC_CONVERGED : CC_MAXITER;
+    results[c].outer = ho[c].outer;
+    results[c].inner_total = ho[c].inner_total;
+    results[c].recursive_rel = ho[c].rel;
+    results[c].true_rel = ho[c].rel;
+  }
+  return 0;
+}
+
+int cg_operator_mixed_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& residual,
+                            const std::function<int(const float*, float*, double*, int*)>& apply32,
+                            const double* diag_src, double* X, double tol, int maxiter, double inner_tol,
+                            int max_outer, int precond, int check_every, void* ws, size_t wsb,
+                            cc_pcg_result* results, hipStream_t s) {
+#define CC_MXK(KK)                                                                                          \
+  case KK:                                                                                                  \
+    return mixed_operator_k<KK>(n, residual, apply32, diag_src, X, tol, maxiter, inner_tol, max_outer, precond, \
+                                check_every, ws, wsb, results, s);
+  switch (k) {
+    CC_MXK(1)
+    CC_MXK(2)
+    CC_MXK(4)
+    CC_MXK(8)
+    CC_MXK(16)
+    default: return CC_E_ARG;
+  }
+#undef CC_MXK
+}
+
+size_t cg_operator_mixed_multi_bytes(size_t n, int k) {
+  switch (k) {
+    case 1: return mixed_bytes<1>(n);
+    case 2: return mixed_bytes<2>(n);
+    case 4: return mixed_bytes<4>(n);
+    case 8: return mixed_bytes<8>(n);
+    case 16: return mixed_bytes<16>(n);
+    default: return 0;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Fixed device scratch for cc_pcg_jacobi_f64_noalloc.
 //

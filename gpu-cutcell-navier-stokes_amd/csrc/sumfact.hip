@@ -20,12 +20,20 @@
 
 #include <cmath>
 #include <cstdint>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/cutcell_hip.h"
 #include "common.h"
 
 namespace cc {
+
+// Correctly rounded multiply-add and product in the value type T of an apply
+// (float64 for the solver, float32 for the inner solve of the mixed one).
+__device__ inline double fma_t(double a, double b, double c) { return __fma_rn(a, b, c); }
+__device__ inline float fma_t(float a, float b, float c) { return __fmaf_rn(a, b, c); }
+__device__ inline double mul_t(double a, double b) { return __dmul_rn(a, b); }
+__device__ inline float mul_t(float a, float b) { return __fmul_rn(a, b); }
 
 __device__ inline int wrap_or_invalid(int i, int n, int periodic) {
   if (periodic) {
@@ -75,19 +83,19 @@ __global__ __launch_bounds__(256) void k_mf_gather_small(cc_sumfact_desc d, cons
 constexpr int kSx = 16, kSy = 16;
 constexpr int kMaxZc = 32;  // output planes per block at most
 
-template <int P1>
-__global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const double* __restrict__ u_all,
-                                                     double* __restrict__ v_all, int nzc, int zc) {
+template <typename T, int P1>
+__global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const T* __restrict__ u_all,
+                                                     T* __restrict__ v_all, int nzc, int zc) {
   constexpr int p = P1 - 1;
   constexpr int W = 2 * p + 1;
   constexpr int HX = kSx + 2 * p, HY = kSy + 2 * p;
   constexpr int HALO = HX * HY;
   constexpr int PER = (HALO + 255) / 256;
-  __shared__ double su[HY][HX];
-  __shared__ double sa[HY][kSx];
-  __shared__ double sb[HY][kSx];
-  __shared__ double rc[W][kSy][kSx];
-  __shared__ double rd[W][kSy][kSx];
+  __shared__ T su[HY][HX];
+  __shared__ T sa[HY][kSx];
+  __shared__ T sb[HY][kSx];
+  __shared__ T rc[W][kSy][kSx];
+  __shared__ T rd[W][kSy][kSx];
   const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
   const size_t plane = (size_t)nbx * nby;
   // Blocks go to the 8 XCDs round-robin by linear id; each XCD is given one
@@ -101,8 +109,8 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const do
   const unsigned tile = xcd * xq + min(xcd, xr) + lin / kX;
   const int bx = (int)(tile % gx), by = (int)((tile / gx) % gy), bz = (int)(tile / (gx * gy));
   const size_t col_off = (size_t)(bz / nzc) * plane * nbz;
-  const double* __restrict__ u = u_all + col_off;
-  double* __restrict__ v = v_all + col_off;
+  const T* __restrict__ u = u_all + col_off;
+  T* __restrict__ v = v_all + col_off;
   const int zs = (bz % nzc) * zc;
   const int ze = min(nbz, zs + zc);
   const int x0 = bx * kSx, y0 = by * kSy;
@@ -121,30 +129,30 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const do
       if (gx >= 0 && gy >= 0) hoff[k] = (long long)gy * nbx + gx;
     }
   }
-  auto fetch = [&](int zi, double (&val)[PER]) {
+  auto fetch = [&](int zi, T (&val)[PER]) {
     const int gz = wrap_or_invalid(zi, nbz, d.periodic[2]);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) val[k] = (gz >= 0 && hoff[k] >= 0) ? u[(size_t)gz * plane + hoff[k]] : 0.0;
+    for (int k = 0; k < PER; ++k) val[k] = (gz >= 0 && hoff[k] >= 0) ? u[(size_t)gz * plane + hoff[k]] : T(0);
   };
   const int gxo = x0 + tx, gyo = y0 + ty;
   const int z_first = zs - p;
   // Band rows held for the whole z run: this thread's x row (both of its
   // x-pass items have x = x0 + tx) and y row in registers, the z rows of the
   // block's output planes in LDS.
-  double mx[W], kx[W], my[W], ky[W];
+  T mx[W], kx[W], my[W], ky[W];
 #pragma unroll
   for (int k = 0; k < W; ++k) {
-    mx[k] = gxo < nbx ? d.band_m[0][gxo * W + k] : 0.0;
-    kx[k] = gxo < nbx ? d.band_k[0][gxo * W + k] + cm * d.band_m[0][gxo * W + k] : 0.0;
-    my[k] = gyo < nby ? d.band_m[1][gyo * W + k] : 0.0;
-    ky[k] = gyo < nby ? d.band_k[1][gyo * W + k] : 0.0;
+    mx[k] = gxo < nbx ? (T)d.band_m[0][gxo * W + k] : T(0);
+    kx[k] = gxo < nbx ? (T)(d.band_k[0][gxo * W + k] + cm * d.band_m[0][gxo * W + k]) : T(0);
+    my[k] = gyo < nby ? (T)d.band_m[1][gyo * W + k] : T(0);
+    ky[k] = gyo < nby ? (T)d.band_k[1][gyo * W + k] : T(0);
   }
-  __shared__ double mz[kMaxZc][W], kz[kMaxZc][W];
+  __shared__ T mz[kMaxZc][W], kz[kMaxZc][W];
   for (int i = tid; i < (ze - zs) * W; i += 256) {
-    mz[i / W][i % W] = d.band_m[2][(size_t)zs * W + i];
-    kz[i / W][i % W] = d.band_k[2][(size_t)zs * W + i];
+    mz[i / W][i % W] = (T)d.band_m[2][(size_t)zs * W + i];
+    kz[i / W][i % W] = (T)d.band_k[2][(size_t)zs * W + i];
   }
-  double val[PER];
+  T val[PER];
   fetch(z_first, val);
   for (int zi = z_first; zi < ze + p; ++zi) {
     // Every thread finished the previous plane's x pass (it ended with the
@@ -159,10 +167,10 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const do
     // x pass: a = Mx u, b = (Kx + c Mx) u on the HY rows of the halo.
     for (int i = tid; i < HY * kSx; i += 256) {
       const int ly = i / kSx, lx = i % kSx;  // lx == tx
-      double a = 0.0, b = 0.0;
+      T a = 0, b = 0;
 #pragma unroll
       for (int k = 0; k < W; ++k) {
-        const double uu = su[ly][lx + k];
+        const T uu = su[ly][lx + k];
         a += mx[k] * uu;
         b += kx[k] * uu;
       }
@@ -173,7 +181,7 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const do
     // y pass into the ring: c = My b + Ky a, d = My a.
     const int slot = (zi - z_first) % W;
     {
-      double cc_ = 0.0, dd = 0.0;
+      T cc_ = 0, dd = 0;
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         cc_ += my[k] * sb[ty + k][tx] + ky[k] * sa[ty + k][tx];
@@ -186,7 +194,7 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const do
     // z pass: output plane zo = zi - p from ring planes zo - p .. zo + p.
     const int zo = zi - p;
     if (zo >= zs && gxo < nbx && gyo < nby) {
-      double out = 0.0;
+      T out = 0;
 #pragma unroll
       for (int k = 0; k < W; ++k) {
         const int sl = (zo - p + k - z_first) % W;
@@ -207,12 +215,12 @@ static int tensor_zc(const cc_sumfact_desc& d, int K) {
   return (int)zc;
 }
 
-template <int P1>
-static void tensor_launch(const cc_sumfact_desc& d, const double* u, double* v, int K, hipStream_t s) {
+template <typename T, int P1>
+static void tensor_launch(const cc_sumfact_desc& d, const T* u, T* v, int K, hipStream_t s) {
   const int zc = tensor_zc(d, K);
   const int nzc = (d.nb[2] + zc - 1) / zc;
   const dim3 grid((d.nb[0] + kSx - 1) / kSx, (d.nb[1] + kSy - 1) / kSy, nzc * K);
-  hipLaunchKernelGGL((k_mf_tensor_z<P1>), grid, dim3(256), 0, s, d, u, v, nzc, zc);
+  hipLaunchKernelGGL((k_mf_tensor_z<T, P1>), grid, dim3(256), 0, s, d, u, v, nzc, zc);
 }
 
 // v[g] += sum of the shell-cell outputs W that belong to dof g; a 16-lane
@@ -306,11 +314,11 @@ __global__ __launch_bounds__(256) void k_mf_residual(int n, const double* __rest
 // u[c][g] = x[full2red[g]][c] (0 off the interior), through an LDS tile of
 // 1024 / K dofs x K columns: the reads are whole K-column rows of x, the
 // writes runs of consecutive dofs of one column.
-template <int K>
-__global__ __launch_bounds__(256) void k_mfk_gather(cc_sumfact_desc d, const double* __restrict__ x,
-                                                    double* __restrict__ u) {
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_mfk_gather(cc_sumfact_desc d, const T* __restrict__ x,
+                                                    T* __restrict__ u) {
   constexpr int TG = 1024 / K;
-  __shared__ double tile[K][TG + 1];
+  __shared__ T tile[K][TG + 1];
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   const size_t ntiles = (n_full + TG - 1) / TG;
   for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
@@ -321,7 +329,7 @@ __global__ __launch_bounds__(256) void k_mfk_gather(cc_sumfact_desc d, const dou
       const int k = threadIdx.x + 256 * q;
       const int gi = k / K, c = k % K;
       const size_t g = g0 + gi;
-      double val = 0.0;
+      T val = 0;
       if (g < n_full) {
         const int r = d.full2red[g];
         if (r >= 0) val = x[(size_t)r * K + c];
@@ -340,32 +348,32 @@ __global__ __launch_bounds__(256) void k_mfk_gather(cc_sumfact_desc d, const dou
 }
 
 // One wave per small dof: lane (entry slot e, column c), 64 / K entry slots.
-template <int K>
-__global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, const double* __restrict__ x,
-                                                          double* __restrict__ u) {
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_mfk_gather_small(cc_sumfact_desc d, const T* __restrict__ x,
+                                                          T* __restrict__ u) {
   constexpr int EL = 64 / K;
   const int lane = threadIdx.x & 63;
   const int c = lane % K, el = lane / K;
   const int sm = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
   if (sm >= d.n_small) return;
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
-  double s = 0.0;
+  T s = 0;
   const int tend = d.small_ptr[sm + 1];
   int t = d.small_ptr[sm] + el;
   // Four entries' loads in flight at once, summed in entry order.
   for (; t + 3 * EL < tend; t += 4 * EL) {
     int col[4];
-    double w[4], xv[4];
+    T w[4], xv[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) col[q] = d.small_col[t + q * EL];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) w[q] = d.small_w[t + q * EL];
+    for (int q = 0; q < 4; ++q) w[q] = (T)d.small_w[t + q * EL];
 #pragma unroll
     for (int q = 0; q < 4; ++q) xv[q] = x[(size_t)col[q] * K + c];
 #pragma unroll
     for (int q = 0; q < 4; ++q) s += w[q] * xv[q];
   }
-  for (; t < tend; t += EL) s += d.small_w[t] * x[(size_t)d.small_col[t] * K + c];
+  for (; t < tend; t += EL) s += (T)d.small_w[t] * x[(size_t)d.small_col[t] * K + c];
   for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
   if (lane < K) u[(size_t)c * n_full + d.small_dof[sm]] = s;
 }
@@ -407,10 +415,11 @@ struct ShellLds {
   static __device__ __forceinline__ int at(int h, int r, int q) { return h * WST + r * ROW + (q / QQ) * SLAB + q % QQ; }
 };
 
-template <int P1, int K>
+template <typename T, int P1, int K>
 __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfact_desc d,
-                                                                      const double* __restrict__ u,
-                                                                      double* __restrict__ Wout) {
+                                                                      const T* __restrict__ blocks,
+                                                                      const T* __restrict__ u,
+                                                                      T* __restrict__ Wout) {
   constexpr int QQ = P1 * P1;
   constexpr int PP = QQ * P1;
   constexpr int CK = K < 8 ? K : 8;
@@ -419,8 +428,8 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
   // Per wave: rows [0, CK) u of its columns, x slowest ((x*P1 + z)*P1 + y),
   // later the first stage value of each column; [CK, 2 CK) the second.
   using L = ShellLds<P1, CK>;
-  __shared__ double su[WPC * L::WST];
-  __shared__ double sf[6][QQ];  // mx kx my ky mz kz of the cell
+  __shared__ T su[WPC * L::WST];
+  __shared__ T sf[6][QQ];  // mx kx my ky mz kz of the cell
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int c0 = wv * CK;
@@ -441,30 +450,30 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
   }
   if (wv == 0) {
     for (int i = lane; i < QQ; i += 64) {
-      sf[0][i] = d.elem_m[0][(size_t)ex * QQ + i];
-      sf[1][i] = d.elem_k[0][(size_t)ex * QQ + i];
-      sf[2][i] = d.elem_m[1][(size_t)ey * QQ + i];
-      sf[3][i] = d.elem_k[1][(size_t)ey * QQ + i];
-      sf[4][i] = d.elem_m[2][(size_t)ez * QQ + i];
-      sf[5][i] = d.elem_k[2][(size_t)ez * QQ + i];
+      sf[0][i] = (T)d.elem_m[0][(size_t)ex * QQ + i];
+      sf[1][i] = (T)d.elem_k[0][(size_t)ex * QQ + i];
+      sf[2][i] = (T)d.elem_m[1][(size_t)ey * QQ + i];
+      sf[3][i] = (T)d.elem_k[1][(size_t)ey * QQ + i];
+      sf[4][i] = (T)d.elem_m[2][(size_t)ez * QQ + i];
+      sf[5][i] = (T)d.elem_k[2][(size_t)ez * QQ + i];
     }
   }
   __syncthreads();
   const bool act = lane < PP;
   const int l = act ? lane : 0;
   const int lz = l / QQ, ly = (l / P1) % P1, lx = l % P1;
-  double w[CK];
+  T w[CK];
 #pragma unroll
-  for (int c = 0; c < CK; ++c) w[c] = 0.0;
+  for (int c = 0; c < CK; ++c) w[c] = 0;
   const int blk = d.corr_block[s];
-  if constexpr (K == 16 && P1 == 4) {
+  if constexpr (K == 16 && P1 == 4 && std::is_same<T, double>::value) {
     if (blk >= 0) {
       // W rows [32 wv, 32 wv + 32) = two 16-row tiles. Step t covers block
       // rows j = 4t + kk (kk = lane / 16): A[i][kk] = D[16 m + i][j] is read
       // as D[j][16 m + i]; B[kk][c] = u_j of column c = lane % 16, which sits
       // in wave c / 8, row c % 8, at stage-order position 16 kk + t.
       const int i16 = lane & 15, kk = lane >> 4;
-      const double* __restrict__ Db = d.blocks + (size_t)blk * PP * PP + (size_t)kk * PP + 32 * wv + i16;
+      const double* __restrict__ Db = blocks + (size_t)blk * PP * PP + (size_t)kk * PP + 32 * wv + i16;
       double a0[16], a1[16];
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
@@ -490,11 +499,42 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       return;
     }
   }
+  if constexpr (K == 16 && P1 == 4 && std::is_same<T, float>::value) {
+    if (blk >= 0) {
+      // The same tiles on the f32 MFMA 16x16x4, whose A and B operands sit
+      // as the f64 one's; its D tile differs: register r of lane (i16, kk)
+      // holds row 4 kk + r of column i16.
+      const int i16 = lane & 15, kk = lane >> 4;
+      const float* __restrict__ Db = blocks + (size_t)blk * PP * PP + (size_t)kk * PP + 32 * wv + i16;
+      float a0[16], a1[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        a0[t] = Db[(size_t)(4 * t) * PP];
+        a1[t] = Db[(size_t)(4 * t) * PP + 16];
+      }
+      typedef float v4f __attribute__((ext_vector_type(4)));
+      v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+      const int bbase = L::at(i16 >> 3, i16 & 7, 16 * kk);
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const float bv = su[bbase + t];
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[t], bv, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[t], bv, acc1, 0, 0, 0);
+      }
+      const size_t row0 = (size_t)s * PP + 32 * wv + 4 * kk;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        Wout[(row0 + r) * K + i16] = acc0[r];
+        Wout[(row0 + 16 + r) * K + i16] = acc1[r];
+      }
+      return;
+    }
+  }
   if (blk >= 0) {
     // Rows in groups of G, the next group's loads in flight meanwhile.
     constexpr int G = PP % 8 == 0 ? 8 : QQ;
-    const double* __restrict__ Db = d.blocks + (size_t)blk * PP * PP + l;
-    double cur[G], nxt[G];
+    const T* __restrict__ Db = blocks + (size_t)blk * PP * PP + l;
+    T cur[G], nxt[G];
 #pragma unroll
     for (int r = 0; r < G; ++r) cur[r] = Db[(size_t)r * PP];
 #pragma unroll 1
@@ -514,18 +554,18 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
       for (int r = 0; r < G; ++r) cur[r] = nxt[r];
     }
   } else {
-    const double cm = d.mass_coef;
+    const T cm = (T)d.mass_coef;
     // x: a = mx u, b = kx u, stored y slowest.
-    double a[CK], b[CK], am[CK];
+    T a[CK], b[CK], am[CK];
 #pragma unroll
-    for (int c = 0; c < CK; ++c) a[c] = b[c] = 0.0;
+    for (int c = 0; c < CK; ++c) a[c] = b[c] = 0;
 #pragma unroll
     for (int jx = 0; jx < P1; ++jx) {
-      const double mx = sf[0][lx * P1 + jx], kx = sf[1][lx * P1 + jx];
+      const T mx = sf[0][lx * P1 + jx], kx = sf[1][lx * P1 + jx];
       const int q = (jx * P1 + lz) * P1 + ly;
 #pragma unroll
       for (int c = 0; c < CK; ++c) {
-        const double uj = su[L::at(wv, c, q)];
+        const T uj = su[L::at(wv, c, q)];
         a[c] += mx * uj;
         b[c] += kx * uj;
       }
@@ -542,14 +582,14 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
     __syncthreads();
     // y: ky a + my b (summed, both meet mz) and my a; stored z slowest.
 #pragma unroll
-    for (int c = 0; c < CK; ++c) a[c] = b[c] = am[c] = 0.0;
+    for (int c = 0; c < CK; ++c) a[c] = b[c] = am[c] = 0;
 #pragma unroll
     for (int jy = 0; jy < P1; ++jy) {
-      const double my = sf[2][ly * P1 + jy], ky = sf[3][ly * P1 + jy];
+      const T my = sf[2][ly * P1 + jy], ky = sf[3][ly * P1 + jy];
       const int q = (jy * P1 + lz) * P1 + lx;
 #pragma unroll
       for (int c = 0; c < CK; ++c) {
-        const double va = su[L::at(wv, c, q)], vb = su[L::at(wv, CK + c, q)];
+        const T va = su[L::at(wv, c, q)], vb = su[L::at(wv, CK + c, q)];
         am[c] += my * va;
         a[c] += ky * va;
         b[c] += my * vb;
@@ -568,12 +608,12 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
     // z: -(mz (ky mx + my kx) u + (kz + c mz) my mx u).
 #pragma unroll
     for (int jz = 0; jz < P1; ++jz) {
-      const double mz = sf[4][lz * P1 + jz];
-      const double kzc = __fma_rn(cm, mz, sf[5][lz * P1 + jz]);
+      const T mz = sf[4][lz * P1 + jz];
+      const T kzc = fma_t(cm, mz, sf[5][lz * P1 + jz]);
       const int q = (jz * P1 + ly) * P1 + lx;
       // Explicit roundings, so every K gives each column the same result.
 #pragma unroll
-      for (int c = 0; c < CK; ++c) w[c] -= __fma_rn(mz, su[L::at(wv, c, q)], __dmul_rn(kzc, su[L::at(wv, CK + c, q)]));
+      for (int c = 0; c < CK; ++c) w[c] -= fma_t(mz, su[L::at(wv, c, q)], mul_t(kzc, su[L::at(wv, CK + c, q)]));
     }
   }
   if (act) {
@@ -584,9 +624,9 @@ __global__ __launch_bounds__(64 * (K < 8 ? 1 : K / 8)) void k_mfk_shell(cc_sumfa
 
 // v[c][g] += sum of the shell outputs of dof g: lane (entry slot, column),
 // 64 / K entry slots per touched dof, summed by a butterfly.
-template <int K>
-__global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const double* __restrict__ Win,
-                                                   double* __restrict__ v, int PP) {
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const T* __restrict__ Win,
+                                                   T* __restrict__ v, int PP) {
   constexpr int EL = 64 / K;
   const int lane = threadIdx.x & 63;
   const int c = lane % K, el = lane / K;
@@ -594,7 +634,7 @@ __global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const doub
   const int nwaves = (gridDim.x * blockDim.x) >> 6;
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   for (int t = wave; t < d.n_touch; t += nwaves) {
-    double s = 0.0;
+    T s = 0;
     const int iend = d.touch_ptr[t + 1];
     int i = d.touch_ptr[t] + el;
     // Four entries' indices, then their values, in flight at once; the sum
@@ -604,8 +644,8 @@ __global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const doub
       const int a2 = d.touch_idx[i + 2 * EL], a3 = d.touch_idx[i + 3 * EL];
       // Win[cell][local dof][K]: the K columns of one entry are one
       // contiguous 8 K-byte segment.
-      const double w0 = Win[(size_t)a0 * K + c], w1 = Win[(size_t)a1 * K + c];
-      const double w2 = Win[(size_t)a2 * K + c], w3 = Win[(size_t)a3 * K + c];
+      const T w0 = Win[(size_t)a0 * K + c], w1 = Win[(size_t)a1 * K + c];
+      const T w2 = Win[(size_t)a2 * K + c], w3 = Win[(size_t)a3 * K + c];
       s += w0;
       s += w1;
       s += w2;
@@ -618,9 +658,9 @@ __global__ __launch_bounds__(256) void k_mfk_touch(cc_sumfact_desc d, const doub
 }
 
 // extra[j][c] = sum_b E[b, j] v[c][small dof b] for the unknowns with E^T entries.
-template <int K>
-__global__ __launch_bounds__(256) void k_mfk_et_extra(cc_sumfact_desc d, const double* __restrict__ v,
-                                                      double* __restrict__ extra) {
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_mfk_et_extra(cc_sumfact_desc d, const T* __restrict__ v,
+                                                      T* __restrict__ extra) {
   constexpr int EL = 64 / K;
   const int lane = threadIdx.x & 63;
   const int c = lane % K, el = lane / K;
@@ -629,17 +669,17 @@ __global__ __launch_bounds__(256) void k_mfk_et_extra(cc_sumfact_desc d, const d
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   for (int i = wave; i < d.n_et_rows; i += nwaves) {
     const int j = d.et_rows[i];
-    double s = 0.0;
+    T s = 0;
     const int tend = d.et_ptr[j + 1];
     int t = d.et_ptr[j] + el;
     // Four entries' index chains in flight at once, summed in entry order.
     for (; t + 3 * EL < tend; t += 4 * EL) {
       int b[4];
-      double w[4], vv[4];
+      T w[4], vv[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) b[q] = d.et_small[t + q * EL];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) w[q] = d.et_w[t + q * EL];
+      for (int q = 0; q < 4; ++q) w[q] = (T)d.et_w[t + q * EL];
 #pragma unroll
       for (int q = 0; q < 4; ++q) b[q] = d.small_dof[b[q]];
 #pragma unroll
@@ -647,21 +687,21 @@ __global__ __launch_bounds__(256) void k_mfk_et_extra(cc_sumfact_desc d, const d
 #pragma unroll
       for (int q = 0; q < 4; ++q) s += w[q] * vv[q];
     }
-    for (; t < tend; t += EL) s += d.et_w[t] * v[(size_t)c * n_full + d.small_dof[d.et_small[t]]];
+    for (; t < tend; t += EL) s += (T)d.et_w[t] * v[(size_t)c * n_full + d.small_dof[d.et_small[t]]];
     for (int o = K; o < 64; o <<= 1) s += __shfl_xor(s, o, 64);
     if (lane < K) extra[(size_t)j * K + c] = s;
   }
 }
 
 // y = E^T v for K columns; partial[block][c] = sum_j x[j][c] y[j][c].
-template <int K>
-__global__ __launch_bounds__(256) void k_mfk_scatter(cc_sumfact_desc d, const double* __restrict__ v,
-                                                     const double* __restrict__ extra, double* __restrict__ y,
-                                                     const double* __restrict__ x, double* __restrict__ partial) {
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_mfk_scatter(cc_sumfact_desc d, const T* __restrict__ v,
+                                                     const T* __restrict__ extra, T* __restrict__ y,
+                                                     const T* __restrict__ x, double* __restrict__ partial) {
   // Through an LDS tile of 1024 / K unknowns x K columns: the reads of v are
   // runs of one column's grid, the writes of y whole K-column rows.
   constexpr int TJ = 1024 / K;
-  __shared__ double tile[TJ][K + 1];
+  __shared__ T tile[TJ][K + 1];
   __shared__ double red[4 * K];
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   const size_t n_red = (size_t)d.n_red;
@@ -685,10 +725,10 @@ __global__ __launch_bounds__(256) void k_mfk_scatter(cc_sumfact_desc d, const do
       const size_t j = j0 + ji;
       if (j < n_red) {
         const size_t e = j * K + c;
-        double sv = tile[ji][c];
+        T sv = tile[ji][c];
         if (d.et_ptr[j + 1] > d.et_ptr[j]) sv += extra[e];
         y[e] = sv;
-        if (partial) dot += x[e] * sv;
+        if (partial) dot += (double)x[e] * (double)sv;
       }
     }
   }
@@ -733,9 +773,40 @@ __global__ __launch_bounds__(256) void k_mfk_residual(int n, const double* __res
   }
 }
 
-struct MfBuffers {
-  double *u, *v, *W, *extra;
+// R = B - R (R holds A X on entry) with per-block, per-column partials of
+// R.R: partial[block][c]. The outer residual of the mixed solve.
+template <int K>
+__global__ __launch_bounds__(256) void k_mfk_sub_colsq(int n, const double* __restrict__ B, double* __restrict__ R,
+                                                       double* __restrict__ partial) {
+  __shared__ double red[4 * K];
+  const size_t total = (size_t)n * K;
+  double rr = 0.0;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
+    const double r = B[e] - R[e];
+    R[e] = r;
+    rr += r * r;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int o = K; o < 64; o <<= 1) rr += __shfl_xor(rr, o, 64);
+  if (lane < K) red[wave * K + lane] = rr;
+  __syncthreads();
+  if (threadIdx.x < K)
+    partial[(size_t)blockIdx.x * K + threadIdx.x] =
+        (red[threadIdx.x] + red[K + threadIdx.x]) + (red[2 * K + threadIdx.x] + red[3 * K + threadIdx.x]);
+}
+
+template <typename T>
+struct MfBuf {
+  T *u, *v, *W, *extra;
 };
+using MfBuffers = MfBuf<double>;
+
+// The float32 view of K-column buffers carved for float64 (each array at
+// the same offset, half of it used).
+static MfBuf<float> as_f32(const MfBuffers& b) {
+  return MfBuf<float>{reinterpret_cast<float*>(b.u), reinterpret_cast<float*>(b.v), reinterpret_cast<float*>(b.W),
+                      reinterpret_cast<float*>(b.extra)};
+}
 
 static size_t mf_bytes(const cc_sumfact_desc* d) {
   const size_t n_full = (size_t)d->nb[0] * d->nb[1] * d->nb[2];
@@ -780,9 +851,9 @@ static int mf_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   hipLaunchKernelGGL(k_mf_gather, dim3(grid1d(n_full)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0) hipLaunchKernelGGL(k_mf_gather_small, dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
-  tensor_launch<P1>(d, b.u, b.v, 1, s);
+  tensor_launch<double, P1>(d, b.u, b.v, 1, s);
   if (d.n_corr > 0) {
-    hipLaunchKernelGGL((k_mfk_shell<P1, 1>), dim3(d.n_corr), dim3(64), 0, s, d, b.u, b.W);
+    hipLaunchKernelGGL((k_mfk_shell<double, P1, 1>), dim3(d.n_corr), dim3(64), 0, s, d, d.blocks, b.u, b.W);
     hipLaunchKernelGGL(k_mf_touch, dim3(grid_items(d.n_touch, 16)), dim3(256), 0, s, d, b.W, b.v);
   }
   if (d.n_et_rows > 0)
@@ -849,43 +920,52 @@ static MfBuffers carve_mfk(const cc_sumfact_desc* d, int K, char* base) {
   return b;
 }
 
-template <int P1, int K>
-static int mfk_apply_t(const cc_sumfact_desc& d, const MfBuffers& b, const double* x, double* y, double* partial,
-                       int* nparts, hipStream_t s) {
+// One K-column apply in value type T; blocks: the cut-cell corrections in T
+// (the descriptor's float64 blocks, or a float32 copy).
+template <typename T, int P1, int K>
+static int mfk_apply_t(const cc_sumfact_desc& d, const T* blocks, const MfBuf<T>& b, const T* x, T* y,
+                       double* partial, int* nparts, hipStream_t s) {
   const size_t n_full = (size_t)d.nb[0] * d.nb[1] * d.nb[2];
   constexpr int PP = P1 * P1 * P1;
   // One block per 1024-element tile: no partials, so no cap on the grid.
-  hipLaunchKernelGGL((k_mfk_gather<K>), dim3(grid_items(n_full * K, 1024)), dim3(256), 0, s, d, x, b.u);
+  hipLaunchKernelGGL((k_mfk_gather<T, K>), dim3(grid_items(n_full * K, 1024)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0)
-    hipLaunchKernelGGL((k_mfk_gather_small<K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
-  tensor_launch<P1>(d, b.u, b.v, K, s);
+    hipLaunchKernelGGL((k_mfk_gather_small<T, K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
+  tensor_launch<T, P1>(d, b.u, b.v, K, s);
   if (d.n_corr > 0) {
-    hipLaunchKernelGGL((k_mfk_shell<P1, K>), dim3(d.n_corr), dim3(64 * (K < 8 ? 1 : K / 8)), 0, s, d, b.u, b.W);
-    hipLaunchKernelGGL((k_mfk_touch<K>), dim3(grid_items(d.n_touch, 4)), dim3(256), 0, s, d, b.W, b.v, PP);
+    hipLaunchKernelGGL((k_mfk_shell<T, P1, K>), dim3(d.n_corr), dim3(64 * (K < 8 ? 1 : K / 8)), 0, s, d, blocks, b.u,
+                       b.W);
+    hipLaunchKernelGGL((k_mfk_touch<T, K>), dim3(grid_items(d.n_touch, 4)), dim3(256), 0, s, d, b.W, b.v, PP);
   }
   if (d.n_et_rows > 0)
-    hipLaunchKernelGGL((k_mfk_et_extra<K>), dim3(grid_items(d.n_et_rows, 4)), dim3(256), 0, s, d, b.v, b.extra);
+    hipLaunchKernelGGL((k_mfk_et_extra<T, K>), dim3(grid_items(d.n_et_rows, 4)), dim3(256), 0, s, d, b.v, b.extra);
   const int gs = grid1d((size_t)d.n_red * K);
-  hipLaunchKernelGGL((k_mfk_scatter<K>), dim3(gs), dim3(256), 0, s, d, b.v, b.extra, y, x, partial);
+  hipLaunchKernelGGL((k_mfk_scatter<T, K>), dim3(gs), dim3(256), 0, s, d, b.v, b.extra, y, x, partial);
   if (nparts) *nparts = gs;
   return (int)hipGetLastError();
 }
 
-static int mfk_apply(const cc_sumfact_desc& d, int K, const MfBuffers& b, const double* x, double* y,
-                     double* partial, int* nparts, hipStream_t s) {
-#define CC_MFK(P1V)                                                              \
-  switch (K) {                                                                  \
-    case 1: return mfk_apply_t<P1V, 1>(d, b, x, y, partial, nparts, s);         \
-    case 2: return mfk_apply_t<P1V, 2>(d, b, x, y, partial, nparts, s);         \
-    case 4: return mfk_apply_t<P1V, 4>(d, b, x, y, partial, nparts, s);         \
-    case 8: return mfk_apply_t<P1V, 8>(d, b, x, y, partial, nparts, s);         \
-    case 16: return mfk_apply_t<P1V, 16>(d, b, x, y, partial, nparts, s);       \
-    default: return CC_E_ARG;                                                   \
+template <typename T>
+static int mfk_apply_any(const cc_sumfact_desc& d, const T* blocks, int K, const MfBuf<T>& b, const T* x, T* y,
+                         double* partial, int* nparts, hipStream_t s) {
+#define CC_MFK(P1V)                                                                    \
+  switch (K) {                                                                        \
+    case 1: return mfk_apply_t<T, P1V, 1>(d, blocks, b, x, y, partial, nparts, s);   \
+    case 2: return mfk_apply_t<T, P1V, 2>(d, blocks, b, x, y, partial, nparts, s);   \
+    case 4: return mfk_apply_t<T, P1V, 4>(d, blocks, b, x, y, partial, nparts, s);   \
+    case 8: return mfk_apply_t<T, P1V, 8>(d, blocks, b, x, y, partial, nparts, s);   \
+    case 16: return mfk_apply_t<T, P1V, 16>(d, blocks, b, x, y, partial, nparts, s); \
+    default: return CC_E_ARG;                                                         \
   }
   if (d.degree == 2) CC_MFK(3)
   if (d.degree == 3) CC_MFK(4)
 #undef CC_MFK
   return CC_E_ARG;
+}
+
+static int mfk_apply(const cc_sumfact_desc& d, int K, const MfBuffers& b, const double* x, double* y,
+                     double* partial, int* nparts, hipStream_t s) {
+  return mfk_apply_any<double>(d, d.blocks, K, b, x, y, partial, nparts, s);
 }
 
 // ||b_c - (A x)_c|| / ||b_c|| for the K columns (||b_c|| = 0 -> 1); partials
@@ -1075,6 +1155,56 @@ int cc_pcg_sumfact_k(const cc_sumfact_desc* d, int32_t k, const double* B, doubl
   };
   return cg_operator_multi(desc.n_red, k, apply, preconditioner ? desc.diag : nullptr, B, X, tol, maxiter,
                            preconditioner, check_every, cgws, cgb, results, s, residual);
+}
+
+size_t cc_sumfact_mixed_k_workspace_bytes(const cc_sumfact_desc* d, int32_t k) {
+  if (!desc_ok(d) || !k_ok(k)) return 0;
+  return mfk_bytes(d, k) + 1024 + cg_operator_mixed_multi_bytes((size_t)d->n_red, k) + 512;
+}
+
+int cc_sumfact_apply_k_f32(const cc_sumfact_desc* d, const float* blocks32, int32_t k, const float* X, float* Y,
+                           void* workspace, size_t workspace_bytes, cc_stream_t stream) {
+  if (!desc_ok(d) || !k_ok(k) || !X || !Y || !workspace || (d->n_corr > 0 && !blocks32)) return CC_E_ARG;
+  if (workspace_bytes < cc_sumfact_k_workspace_bytes(d, k, 0)) return CC_E_WORKSPACE;
+  const MfBuffers b = carve_mfk(d, k, (char*)workspace);
+  return mfk_apply_any<float>(*d, blocks32, k, as_f32(b), X, Y, nullptr, nullptr, (hipStream_t)stream);
+}
+
+int cc_pcg_sumfact_mixed_k(const cc_sumfact_desc* d, const float* blocks32, int32_t k, const double* B, double* X,
+                           double tol, int32_t maxiter, double inner_tol, int32_t max_outer, int32_t preconditioner,
+                           int32_t check_every, void* workspace, size_t workspace_bytes, cc_pcg_result* results,
+                           cc_stream_t stream) {
+  if (!desc_ok(d) || !k_ok(k) || !B || !X || !workspace || !results || maxiter < 0 || max_outer < 1 ||
+      !(inner_tol > 0.0 && inner_tol < 1.0) || (d->n_corr > 0 && !blocks32))
+    return CC_E_ARG;
+  if (workspace_bytes < cc_sumfact_mixed_k_workspace_bytes(d, k)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  const MfBuffers mb = carve_mfk(d, k, base);
+  const MfBuf<float> mb32 = as_f32(mb);
+  const size_t off = (mfk_bytes(d, k) + 511) & ~size_t(255);
+  char* cgws = base + off;
+  const size_t cgb = workspace_bytes - off;
+  const cc_sumfact_desc desc = *d;
+  const int K = k;
+  auto residual = [&](const double* Xv, double* R64, double* partial, int* gs) {
+    CC_CHECK(mfk_apply(desc, K, mb, Xv, R64, nullptr, nullptr, s));
+    const int g = grid1d((size_t)desc.n_red * K);
+    switch (K) {
+      case 1: hipLaunchKernelGGL((k_mfk_sub_colsq<1>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      case 2: hipLaunchKernelGGL((k_mfk_sub_colsq<2>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      case 4: hipLaunchKernelGGL((k_mfk_sub_colsq<4>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      case 8: hipLaunchKernelGGL((k_mfk_sub_colsq<8>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      default: hipLaunchKernelGGL((k_mfk_sub_colsq<16>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial);
+    }
+    *gs = g;
+    return (int)hipGetLastError();
+  };
+  auto apply32 = [&](const float* P, float* Q, double* partial, int* gs) {
+    return mfk_apply_any<float>(desc, blocks32, K, mb32, P, Q, partial, gs, s);
+  };
+  return cg_operator_mixed_multi(desc.n_red, k, residual, apply32, preconditioner ? desc.diag : nullptr, X, tol,
+                                 maxiter, inner_tol, max_outer, preconditioner, check_every, cgws, cgb, results, s);
 }
 
 size_t cc_fcg_sumfact_ml_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml) {

@@ -353,6 +353,8 @@ class DeviceMatrixFree:
         d.corr_cell = t(mf.corr_cells, np.int32)
         d.corr_block = t(mf.corr_block, np.int32)
         d.blocks = t(cut_corrections(mf) if mf.blocks.size else np.zeros(1), np.float64)
+        self._blocks64 = keep[-1]
+        self._blocks32 = None
         d.n_touch = mf.touch_dof.size
         d.touch_dof = t(mf.touch_dof, np.int32)
         d.touch_ptr = t(mf.touch_ptr, np.int32)
@@ -473,6 +475,65 @@ class DeviceMatrixFree:
         hip._check(lib.cc_sumfact_apply_k(ctypes.addressof(self.desc), int(k), hip._ptr(X), hip._ptr(Y), hip._ptr(ws),
                                           nbytes, hip._stream()), "sumfact_apply_k")
         return Y
+
+    @property
+    def blocks32(self) -> int:
+        """Device address of the float32 copy of the cut-cell corrections (made
+        on first use) for the float32 operator of the mixed-precision PCG."""
+        import torch
+
+        if self._blocks32 is None:
+            self._blocks32 = self._blocks64.to(torch.float32)
+        return self._blocks32.data_ptr()
+
+    def apply_k_f32(self, X, Y=None):
+        """``Y = E^T (K + c M) E X`` in float32 (the inner operator of
+        ``pcg_mixed_batched``) for a row-major ``[n, k]`` float32 block."""
+        import torch
+
+        from . import hip
+
+        k = X.shape[1]
+        if X.dtype != torch.float32:
+            raise TypeError("apply_k_f32 takes a float32 block")
+        lib, ws, nbytes = self._ws_k(k, False)
+        X = X.contiguous()
+        Y = torch.empty_like(X) if Y is None else Y
+        hip._check(lib.cc_sumfact_apply_k_f32(ctypes.addressof(self.desc), self.blocks32, int(k), hip._ptr(X),
+                                              hip._ptr(Y), hip._ptr(ws), nbytes, hip._stream()),
+                   "sumfact_apply_k_f32")
+        return Y
+
+    def pcg_mixed_batched(self, B, *, tol=1.0e-8, maxiter=5000, inner_tol=1.0e-4, max_outer=12,
+                          preconditioner="jacobi", check_every=16, X=None):
+        """FP32 inner / FP64 outer Jacobi-PCG on the ``k`` columns of ``B``:
+        the inner solves apply the operator in float32 to ``inner_tol``, the
+        outer loop stops on the float64 residual ``||B - A X|| / ||B|| <= tol``.
+        Returns ``(X, [result per column])``; ``iterations`` counts inner
+        iterations and ``outer`` the sweeps."""
+        import torch
+
+        from . import hip
+
+        k = B.shape[1]
+        lib = hip.require_gpu()
+        nbytes = int(lib.cc_sumfact_mixed_k_workspace_bytes(ctypes.addressof(self.desc), int(k)))
+        if nbytes == 0:
+            raise ValueError(f"invalid matrix-free descriptor or k={k} (k in {hip.SUPPORTED_K})")
+        key = f"ws_mixed_k{k}"
+        ws = getattr(self, key, None)
+        if ws is None or ws.numel() < nbytes:
+            ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
+            setattr(self, key, ws)
+        B = B.contiguous()
+        X = torch.empty_like(B) if X is None else X
+        res = (hip.PcgResult * k)()
+        hip._check(lib.cc_pcg_sumfact_mixed_k(ctypes.addressof(self.desc), self.blocks32, int(k), hip._ptr(B),
+                                              hip._ptr(X), float(tol), int(maxiter), float(inner_tol),
+                                              int(max_outer), {"none": 0, "jacobi": 1}[preconditioner],
+                                              int(check_every), hip._ptr(ws), nbytes, res, hip._stream()),
+                   "pcg_sumfact_mixed_k")
+        return X, list(res)
 
     def pcg_batched(self, B, *, tol=1.0e-8, maxiter=5000, preconditioner="jacobi", check_every=16, X=None):
         """Jacobi-PCG on the ``k`` columns of ``B`` (``[n, k]``, k in {1, 2, 4,

@@ -502,6 +502,26 @@ int cc_pcg_sumfact_k(const cc_sumfact_desc* d, int32_t k, const double* B, doubl
                      int32_t preconditioner, int32_t check_every, void* workspace, size_t workspace_bytes,
                      cc_pcg_result* results, cc_stream_t stream);
 
+/* FP32 inner / FP64 outer Jacobi-PCG on the matrix-free operator for k in
+ * {1, 2, 4, 8, 16} right-hand sides: the inner solves apply the operator in
+ * float32 (cut-cell corrections from blocks32, a float32 copy of d->blocks
+ * in the same layout; vectors, the tensor product and the shell in float32)
+ * to inner_tol, and the outer loop recomputes B - A X with the float64
+ * operator and stops on that residual (<= tol), at most max_outer sweeps.
+ * results[c].iterations counts inner iterations, .outer the sweeps.
+ * Replaces the loop of solve_gpu_cg (code/fast_solver_backends.py:233-274) on
+ * E^T A E (code/bell_3d_core.py:550-552) with a mixed-precision apply; the
+ * reference has no counterpart for the precision split.
+ * cc_sumfact_apply_k_f32: Y = A X in float32 (workspace as
+ * cc_sumfact_k_workspace_bytes(d, k, 0)). */
+size_t cc_sumfact_mixed_k_workspace_bytes(const cc_sumfact_desc* d, int32_t k);
+int cc_sumfact_apply_k_f32(const cc_sumfact_desc* d, const float* blocks32, int32_t k, const float* X, float* Y,
+                           void* workspace, size_t workspace_bytes, cc_stream_t stream);
+int cc_pcg_sumfact_mixed_k(const cc_sumfact_desc* d, const float* blocks32, int32_t k, const double* B, double* X,
+                           double tol, int32_t maxiter, double inner_tol, int32_t max_outer, int32_t preconditioner,
+                           int32_t check_every, void* workspace, size_t workspace_bytes, cc_pcg_result* results,
+                           cc_stream_t stream);
+
 /* ---- Learned preconditioner (no reference counterpart: the reference has
  * only Jacobi, code/fast_solver_backends.py:74-87 and :237-238). ---------
  * Activations are channel-last bfloat16 grids [voxel][16] on the background
