@@ -579,6 +579,21 @@ def time_batched(hip, torch, dmf, B, ref=None, modulo_constant=False):
                                ref, modulo_constant)
 
 
+def time_mixed_batched(hip, torch, dmf, B, ref=None, modulo_constant=False):
+    """Batched matrix-free Jacobi-PCG with FP32 inner solves (the operator in
+    float32) under FP64 refinement; iterations count inner iterations."""
+    outer = []
+
+    def solve_k(Bk):
+        X, res = dmf.pcg_mixed_batched(Bk, tol=TOL, maxiter=20000)
+        outer.extend(int(r.outer) for r in res)
+        return X, res
+
+    o = time_batched_stream(hip, torch, solve_k, B, "jacobi", ref, modulo_constant)
+    o["outer_range"] = [min(outer), max(outer)]
+    return o
+
+
 def time_learned_batched(hip, torch, net, dmf, B, ref=None, modulo_constant=False):
     """Batched flexible CG with the multilevel learned preconditioner (one
     K-column operator apply and one batched network pass per iteration)."""
@@ -669,6 +684,8 @@ def pressure_section(hip, torch, args):
     log(f"pressure, jacobi: {out['jacobi']}")
     out["jacobi_batched"] = time_batched(hip, torch, dmf, B, PSI, True)
     log(f"pressure, jacobi batched: {out['jacobi_batched']}")
+    out["jacobi_mixed_batched"] = time_mixed_batched(hip, torch, dmf, B, PSI, True)
+    log(f"pressure, jacobi batched FP32/FP64: {out['jacobi_mixed_batched']}")
 
     def rhs():
         b = rng.standard_normal(n)
@@ -707,6 +724,8 @@ def pressure_section(hip, torch, args):
     # The ratio that isolates the network: batched learned against batched Jacobi.
     out["ratio_learned_batched_vs_jacobi_batched"] = sig(out["learned_batched"]["solves_per_s"]
                                                          / out["jacobi_batched"]["solves_per_s"])
+    out["ratio_learned_batched_vs_jacobi_mixed_batched"] = sig(out["learned_batched"]["solves_per_s"]
+                                                               / out["jacobi_mixed_batched"]["solves_per_s"])
     out.update(_matched(hip, torch, dmf, mlnet, B, PSI, True))
     log(f"pressure, accuracy-matched: {out['at_solution_error_1e-8']}")
     return out
@@ -805,6 +824,8 @@ def learned_section(hip, torch, args):
         log(f"learned section, {method}: {out[method]}")
     out["jacobi_batched"] = time_batched(hip, torch, dmf, B, ref)
     log(f"learned section, jacobi batched: {out['jacobi_batched']}")
+    out["jacobi_mixed_batched"] = time_mixed_batched(hip, torch, dmf, B, ref)
+    log(f"learned section, jacobi batched FP32/FP64: {out['jacobi_mixed_batched']}")
     out["learned_batched"] = time_learned_batched(hip, torch, net, dmf, B, ref)
     log(f"learned section, learned batched: {out['learned_batched']}")
     out["error_reference"] = "float64 Jacobi-PCG to relative residual 1e-12"
@@ -815,6 +836,8 @@ def learned_section(hip, torch, args):
     out["speedup_learned_batched_vs_jacobi"] = sig(out["learned_batched"]["solves_per_s"] / jac)
     out["ratio_learned_batched_vs_jacobi_batched"] = sig(out["learned_batched"]["solves_per_s"]
                                                          / out["jacobi_batched"]["solves_per_s"])
+    out["ratio_learned_batched_vs_jacobi_mixed_batched"] = sig(out["learned_batched"]["solves_per_s"]
+                                                               / out["jacobi_mixed_batched"]["solves_per_s"])
     out.update(_matched(hip, torch, dmf, net, B, ref, False))
     log(f"learned section, accuracy-matched: {out['at_solution_error_1e-8']}")
     out["iteration_ratio"] = sig(out["jacobi"]["iterations_mean"] / out["learned"]["iterations_mean"])
@@ -822,7 +845,8 @@ def learned_section(hip, torch, args):
 
 
 def operator_probe(hip, torch, args, reps: int = 30):
-    """Back-to-back applies of one matrix-free operator: 16 columns, then one."""
+    """Back-to-back applies of one matrix-free operator: 16 columns, one, and
+    16 columns in float32."""
     from cutcell.matfree import DeviceMatrixFree
 
     mf = build_sphere_operator(args.seed, args.learned_n) if args.operator_probe == "sphere" \
@@ -833,7 +857,9 @@ def operator_probe(hip, torch, args, reps: int = 30):
     x1 = X[:, 0].contiguous()
     out = {"operator": args.operator_probe, "n": mf.n, "n_shell_cells": int(mf.corr_cells.shape[0]),
            "n_cut_cells": int(mf.blocks.shape[0]), "reps": reps}
-    for name, f in (("apply_k16_us", lambda: dmf.apply_k(X)), ("apply_k1_us", lambda: dmf.apply(x1))):
+    X32 = X.float()
+    for name, f in (("apply_k16_us", lambda: dmf.apply_k(X)), ("apply_k1_us", lambda: dmf.apply(x1)),
+                    ("apply_k16_f32_us", lambda: dmf.apply_k_f32(X32))):
         for _ in range(3):
             f()
         torch.cuda.synchronize()
