@@ -594,6 +594,21 @@ def time_mixed_batched(hip, torch, dmf, B, ref=None, modulo_constant=False):
     return o
 
 
+def time_learned_mixed_batched(hip, torch, net, dmf, B, ref=None, modulo_constant=False):
+    """The batched learned flexible CG inside float64 refinement, its operator
+    applied in float32; iterations count inner iterations."""
+    outer = []
+
+    def solve_k(Bk):
+        X, res = net.fcg_mixed_batched(dmf, Bk, tol=TOL, maxiter=20000)
+        outer.extend(int(r.outer) for r in res)
+        return X, res
+
+    o = time_batched_stream(hip, torch, solve_k, B, "learned", ref, modulo_constant)
+    o["outer_range"] = [min(outer), max(outer)]
+    return o
+
+
 def time_learned_batched(hip, torch, net, dmf, B, ref=None, modulo_constant=False):
     """Batched flexible CG with the multilevel learned preconditioner (one
     K-column operator apply and one batched network pass per iteration)."""
@@ -717,6 +732,8 @@ def pressure_section(hip, torch, args):
     log(f"pressure, learned multilevel: {out['learned_multilevel']}")
     out["learned_batched"] = time_learned_batched(hip, torch, mlnet, dmf, B, PSI, True)
     log(f"pressure, learned batched: {out['learned_batched']}")
+    out["learned_mixed_batched"] = time_learned_mixed_batched(hip, torch, mlnet, dmf, B, PSI, True)
+    log(f"pressure, learned batched FP32/FP64: {out['learned_mixed_batched']}")
     jac = out["jacobi"]["solves_per_s"]
     out["speedup_solves_per_s"] = sig(out["learned_multilevel"]["solves_per_s"] / jac)
     out["speedup_batched_jacobi"] = sig(out["jacobi_batched"]["solves_per_s"] / jac)
@@ -828,6 +845,8 @@ def learned_section(hip, torch, args):
     log(f"learned section, jacobi batched FP32/FP64: {out['jacobi_mixed_batched']}")
     out["learned_batched"] = time_learned_batched(hip, torch, net, dmf, B, ref)
     log(f"learned section, learned batched: {out['learned_batched']}")
+    out["learned_mixed_batched"] = time_learned_mixed_batched(hip, torch, net, dmf, B, ref)
+    log(f"learned section, learned batched FP32/FP64: {out['learned_mixed_batched']}")
     out["error_reference"] = "float64 Jacobi-PCG to relative residual 1e-12"
     _, res, how = solve_learned_with_fallback(net, dmf, B[0], tol=TOL)
     out["fallback_check"] = {"path": how, "converged": res.status == hip.CC_CONVERGED}

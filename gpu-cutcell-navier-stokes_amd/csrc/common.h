@@ -102,6 +102,14 @@ int cg_operator_mixed_multi(int n, int k, const std::function<int(const double*,
                             int max_outer, int precond, int check_every, void* ws, size_t wsb,
                             cc_pcg_result* results, hipStream_t s);
 size_t cg_operator_mixed_multi_bytes(size_t n, int k);
+// Iterative refinement around an inner solver (k in {1, 2, 4, 8, 16}):
+// residual as above; inner(Bs, Z, maxiter_left, its) solves for Z from the
+// scaled float64 residual Bs (columns not running have Bs = 0) and writes
+// each column's iteration count to its (host). X is zeroed first (pcg.hip).
+int refine_operator_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& residual,
+                          const std::function<int(const double*, double*, int, int*)>& inner, double* X, double tol,
+                          int maxiter, int max_outer, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s);
+size_t refine_operator_multi_bytes(size_t n, int k);
 // Y = A X (CSR, float64, k in {1, 2, 4, 8, 16}) with the autotuned launch
 // shape; partial[block][c] gets per-block X.Y sums when not null (pcg.hip).
 int csr_spmm_f64(int n, int k, double avg_row, const int* indptr, const int* indices, const double* vals,

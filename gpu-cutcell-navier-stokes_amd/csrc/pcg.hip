@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -1496,6 +1497,130 @@ static int mixed_operator_k(int n, const std::function<int(const double*, double
     results[c].true_rel = ho[c].rel;
   }
   return 0;
+}
+
+// Bs = R * inv_scale[c] (zero for the columns not running: inv_scale 0).
+template <int K>
+__global__ __launch_bounds__(256) void k_outer_scale(int n, const double* __restrict__ R,
+                                                     const double* __restrict__ inv_scale, double* __restrict__ Bs) {
+  const size_t total = (size_t)n * K;
+  const double sc = inv_scale[threadIdx.x % K];
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    Bs[e] = R[e] * sc;
+}
+
+// X += scale[c] * Z for the columns of this sweep (float64 correction).
+template <int K>
+__global__ __launch_bounds__(256) void k_outer_correct64(int n, double* __restrict__ X, const double* __restrict__ Z,
+                                                         const OuterState* __restrict__ ost,
+                                                         const int* __restrict__ run_mask) {
+  const size_t total = (size_t)n * K;
+  const int c = threadIdx.x % K;
+  if (!run_mask[c]) return;
+  const double sc = ost[c].scale;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x)
+    X[e] += sc * Z[e];
+}
+
+template <int K>
+static size_t refine_bytes(size_t n) {
+  size_t b = 0;
+  auto add = [&](size_t bytes) { b = ((b + 255) & ~size_t(255)) + bytes; };
+  for (int i = 0; i < 3; ++i) add(n * K * sizeof(double));  // R64, Bs, Z
+  add((size_t)kMaxParts * K * sizeof(double));
+  add(K * sizeof(OuterState));
+  add(K * sizeof(int));
+  add(K * sizeof(double));
+  return b + 256;
+}
+
+// Iterative refinement around an inner solver: the outer loop as in
+// mixed_operator_k (float64 residual, per-column stop and scale), the inner
+// solve inner(Bs, Z, maxiter_left, its) on the scaled residual Bs, with its
+// per-column iteration counts written to its (host).
+template <int K>
+static int refine_operator_k(int n, const std::function<int(const double*, double*, double*, int*)>& residual,
+                             const std::function<int(const double*, double*, int, int*)>& inner, double* X,
+                             double tol, int maxiter, int max_outer, void* ws, size_t wsb, cc_pcg_result* results,
+                             hipStream_t s) {
+  Carve cv{(char*)ws, 0, wsb};
+  double* R64 = cv.take<double>((size_t)n * K);
+  double* Bs = cv.take<double>((size_t)n * K);
+  double* Z = cv.take<double>((size_t)n * K);
+  double* pr = cv.take<double>((size_t)kMaxParts * K);
+  OuterState* ost = cv.take<OuterState>(K);
+  int* run_mask = cv.take<int>(K);
+  double* inv_scale = cv.take<double>(K);
+  if (!cv.ok()) return CC_E_WORKSPACE;
+  const int sg = stream_grid((size_t)n * K);
+  hipLaunchKernelGGL(k_zero, dim3(sg), dim3(kBlock), 0, s, X, (size_t)n * K);
+  int gs = 0;
+  CC_CHECK(residual(X, R64, pr, &gs));
+  if (gs < 1 || gs > kMaxParts) return CC_E_ARG;
+  hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 1);
+  CC_CHECK(hipGetLastError());
+  std::vector<OuterState> ho(K);
+  std::vector<int> outer(K, 0), inner_total(K, 0), its(K, 0);
+  int used = 0;
+  for (int sweep = 0; sweep < max_outer; ++sweep) {
+    CC_CHECK(hipMemcpyAsync(ho.data(), ost, K * sizeof(OuterState), hipMemcpyDeviceToHost, s));
+    CC_CHECK(hipStreamSynchronize(s));
+    bool any = false;
+    for (int c = 0; c < K; ++c) any |= ho[c].status == CC_RUNNING;
+    if (!any || used >= maxiter) break;
+    hipLaunchKernelGGL((k_outer_scale<K>), dim3(sg), dim3(kBlock), 0, s, n, R64, inv_scale, Bs);
+    CC_CHECK(hipGetLastError());
+    std::fill(its.begin(), its.end(), 0);
+    CC_CHECK(inner(Bs, Z, maxiter - used, its.data()));
+    int most = 0;
+    for (int c = 0; c < K; ++c)
+      if (ho[c].status == CC_RUNNING) {
+        outer[c] += 1;
+        inner_total[c] += its[c];
+        most = its[c] > most ? its[c] : most;
+      }
+    used += most > 0 ? most : 1;
+    hipLaunchKernelGGL((k_outer_correct64<K>), dim3(sg), dim3(kBlock), 0, s, n, X, Z, ost, run_mask);
+    CC_CHECK(residual(X, R64, pr, &gs));
+    if (gs < 1 || gs > kMaxParts) return CC_E_ARG;
+    hipLaunchKernelGGL((k_outer_finish<K>), dim3(1), dim3(kBlock), 0, s, pr, gs, ost, run_mask, inv_scale, tol, 0);
+    CC_CHECK(hipGetLastError());
+  }
+  CC_CHECK(hipMemcpyAsync(ho.data(), ost, K * sizeof(OuterState), hipMemcpyDeviceToHost, s));
+  CC_CHECK(hipStreamSynchronize(s));
+  for (int c = 0; c < K; ++c) {
+    results[c].iterations = inner_total[c];
+    results[c].status = ho[c].status == CC_CONVERGED ? CC_CONVERGED : CC_MAXITER;
+    results[c].outer = outer[c];
+    results[c].inner_total = inner_total[c];
+    results[c].recursive_rel = ho[c].rel;
+    results[c].true_rel = ho[c].rel;
+  }
+  return 0;
+}
+
+int refine_operator_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& residual,
+                          const std::function<int(const double*, double*, int, int*)>& inner, double* X, double tol,
+                          int maxiter, int max_outer, void* ws, size_t wsb, cc_pcg_result* results, hipStream_t s) {
+  switch (k) {
+    case 1: return refine_operator_k<1>(n, residual, inner, X, tol, maxiter, max_outer, ws, wsb, results, s);
+    case 2: return refine_operator_k<2>(n, residual, inner, X, tol, maxiter, max_outer, ws, wsb, results, s);
+    case 4: return refine_operator_k<4>(n, residual, inner, X, tol, maxiter, max_outer, ws, wsb, results, s);
+    case 8: return refine_operator_k<8>(n, residual, inner, X, tol, maxiter, max_outer, ws, wsb, results, s);
+    case 16: return refine_operator_k<16>(n, residual, inner, X, tol, maxiter, max_outer, ws, wsb, results, s);
+    default: return CC_E_ARG;
+  }
+}
+
+size_t refine_operator_multi_bytes(size_t n, int k) {
+  switch (k) {
+    case 1: return refine_bytes<1>(n);
+    case 2: return refine_bytes<2>(n);
+    case 4: return refine_bytes<4>(n);
+    case 8: return refine_bytes<8>(n);
+    case 16: return refine_bytes<16>(n);
+    default: return 0;
+  }
 }
 
 int cg_operator_mixed_multi(int n, int k, const std::function<int(const double*, double*, double*, int*)>& residual,

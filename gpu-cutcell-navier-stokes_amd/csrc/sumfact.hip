@@ -795,6 +795,16 @@ __global__ __launch_bounds__(256) void k_mfk_sub_colsq(int n, const double* __re
         (red[threadIdx.x] + red[K + threadIdx.x]) + (red[2 * K + threadIdx.x] + red[3 * K + threadIdx.x]);
 }
 
+// Precision conversions of a [n][K] block.
+__global__ __launch_bounds__(256) void k_d2f(size_t count, const double* __restrict__ a, float* __restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+__global__ __launch_bounds__(256) void k_f2d(size_t count, const float* __restrict__ a, double* __restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (size_t)gridDim.x * blockDim.x)
+    b[i] = (double)a[i];
+}
+
 template <typename T>
 struct MfBuf {
   T *u, *v, *W, *extra;
@@ -1268,6 +1278,91 @@ int cc_fcg_sumfact_ml_k(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32
   };
   return fcg_operator_ml_k(desc.n_red, k, apply, *ml, desc.diag, B, X, tol, maxiter, check_every, fws, fb, results,
                            s, residual);
+}
+
+// Workspace of cc_fcg_sumfact_ml_mixed_k: the K-column operator buffers, the
+// float32 direction and product blocks, the flexible CG's, the refinement's.
+static size_t ml_mixed_parts(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int k, size_t* off32, size_t* offf,
+                             size_t* offr) {
+  auto al = [](size_t x) { return (x + 511) & ~size_t(255); };
+  const size_t nk = (size_t)d->n_red * k;
+  size_t b = al(mfk_bytes(d, k));
+  *off32 = b;
+  b += al(2 * nk * sizeof(float));
+  *offf = b;
+  b += al(fcg_operator_ml_k_bytes((size_t)d->n_red, *ml, k));
+  *offr = b;
+  b += al(refine_operator_multi_bytes((size_t)d->n_red, k));
+  return b;
+}
+
+size_t cc_fcg_sumfact_ml_mixed_k_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32_t k) {
+  if (!desc_ok(d) || !ml || !ml_desc_ok(*ml) || !k_ok(k)) return 0;
+  size_t a, b, c;
+  return ml_mixed_parts(d, ml, k, &a, &b, &c);
+}
+
+int cc_fcg_sumfact_ml_mixed_k(const cc_sumfact_desc* d, const float* blocks32, const cc_mlcnn_desc* ml, int32_t k,
+                              const double* B, double* X, double tol, int32_t maxiter, double inner_tol,
+                              int32_t max_outer, int32_t check_every, void* workspace, size_t workspace_bytes,
+                              cc_pcg_result* results, cc_stream_t stream) {
+  if (!desc_ok(d) || !ml || !ml_desc_ok(*ml) || !k_ok(k) || !B || !X || !workspace || !results || maxiter < 0 ||
+      ml->net[0].n_red != d->n_red || max_outer < 1 || !(inner_tol > 0.0 && inner_tol < 1.0) ||
+      (d->n_corr > 0 && !blocks32))
+    return CC_E_ARG;
+  if (workspace_bytes < cc_fcg_sumfact_ml_mixed_k_workspace_bytes(d, ml, k)) return CC_E_WORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  char* base = (char*)workspace;
+  size_t off32, offf, offr;
+  const size_t total = ml_mixed_parts(d, ml, k, &off32, &offf, &offr);
+  const MfBuffers mb = carve_mfk(d, k, base);
+  const MfBuf<float> mb32 = as_f32(mb);
+  const size_t nk = (size_t)d->n_red * k;
+  float* P32 = reinterpret_cast<float*>(base + off32);
+  float* Q32 = P32 + nk;
+  char* fws = base + offf;
+  const size_t fb = offr - offf;
+  char* rws = base + offr;
+  const size_t rb = total - offr;
+  const cc_sumfact_desc desc = *d;
+  const int K = k;
+  const int gconv = grid1d(nk);
+  // The flexible CG's operator: its float64 direction rounded to float32,
+  // the float32 apply (with per-block partials of P32.Q32), the product
+  // widened back.
+  auto apply32 = [&](const double* P, double* Q, double* partial, int* gs) {
+    hipLaunchKernelGGL(k_d2f, dim3(gconv), dim3(256), 0, s, nk, P, P32);
+    CC_CHECK(mfk_apply_any<float>(desc, blocks32, K, mb32, P32, Q32, partial, gs, s));
+    hipLaunchKernelGGL(k_f2d, dim3(gconv), dim3(256), 0, s, nk, Q32, Q);
+    return (int)hipGetLastError();
+  };
+  // The inner solve reports recursive residuals only; the refinement takes
+  // the true ones.
+  auto no_residual = [&](const double*, double*, double* rel) {
+    for (int c = 0; c < K; ++c) rel[c] = 0.0;
+    return 0;
+  };
+  auto inner = [&](const double* Bs, double* Z, int imax, int* its) {
+    std::vector<cc_pcg_result> r(K);
+    CC_CHECK(fcg_operator_ml_k(desc.n_red, K, apply32, *ml, desc.diag, Bs, Z, inner_tol, imax, check_every, fws, fb,
+                               r.data(), s, no_residual));
+    for (int c = 0; c < K; ++c) its[c] = r[c].iterations;
+    return 0;
+  };
+  auto residual = [&](const double* Xv, double* R64, double* partial, int* gs) {
+    CC_CHECK(mfk_apply(desc, K, mb, Xv, R64, nullptr, nullptr, s));
+    const int g = grid1d(nk);
+    switch (K) {
+      case 1: hipLaunchKernelGGL((k_mfk_sub_colsq<1>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      case 2: hipLaunchKernelGGL((k_mfk_sub_colsq<2>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      case 4: hipLaunchKernelGGL((k_mfk_sub_colsq<4>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      case 8: hipLaunchKernelGGL((k_mfk_sub_colsq<8>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial); break;
+      default: hipLaunchKernelGGL((k_mfk_sub_colsq<16>), dim3(g), dim3(256), 0, s, desc.n_red, B, R64, partial);
+    }
+    *gs = g;
+    return (int)hipGetLastError();
+  };
+  return refine_operator_multi(desc.n_red, k, residual, inner, X, tol, maxiter, max_outer, rws, rb, results, s);
 }
 
 size_t cc_fcg_sumfact_workspace_bytes(const cc_sumfact_desc* d, const cc_cnn_desc* net) {

@@ -74,6 +74,7 @@ def _bind(lib):
         "cc_mlcnn_precondition": [P, P, P, P, P, SZ, P],
         "cc_fcg_sumfact_ml": [P, P, P, P, D, I, I, P, SZ, P, P, P],
         "cc_fcg_sumfact_ml_k": [P, P, I, P, P, D, I, I, P, SZ, P, P],
+        "cc_fcg_sumfact_ml_mixed_k": [P, P, P, I, P, P, D, I, D, I, I, P, SZ, P, P],
         "cc_mfma_probe": [P, P, P, P],
         "cc_mfma32_probe": [P, P, P, P],
         "cc_net_set_config": [I, I],
@@ -101,6 +102,7 @@ def _bind(lib):
         "cc_mlcnn_precondition_workspace_bytes": [P],
         "cc_fcg_sumfact_ml_workspace_bytes": [P, P],
         "cc_fcg_sumfact_ml_k_workspace_bytes": [P, P, I],
+        "cc_fcg_sumfact_ml_mixed_k_workspace_bytes": [P, P, I],
         "cc_net_level_workspace_bytes": [P, I],
     }.items():
         fn = getattr(lib, name)
@@ -620,6 +622,35 @@ class MultilevelCNN:
                                                 hip._ptr(X), float(tol), int(maxiter), int(check_every),
                                                 hip._ptr(ws), nb, ctypes.addressof(res), hip._stream()),
                    "fcg_sumfact_ml_k")
+        return X, list(res)
+
+
+    def fcg_mixed_batched(self, dmf, B, *, tol=1.0e-8, maxiter=5000, inner_tol=1.0e-4, max_outer=12,
+                          check_every=16, X=None):
+        """``fcg_batched`` inside float64 iterative refinement: each sweep
+        solves for the scaled residual to ``inner_tol`` with the operator
+        applied in float32, and the sweeps stop on the float64 residual
+        ``||B - A X|| / ||B|| <= tol``. Returns ``(X, results)``;
+        ``iterations`` counts inner iterations and ``outer`` the sweeps."""
+        torch = self.torch
+        d = self.desc()
+        B = B.contiguous()
+        k = B.shape[1]
+        nb = int(self.lib.cc_fcg_sumfact_ml_mixed_k_workspace_bytes(ctypes.addressof(dmf.desc), ctypes.addressof(d),
+                                                                     k))
+        if nb == 0:
+            raise ValueError("invalid descriptors or k")
+        ws = getattr(self, "_fcgk_mixed_ws", None)
+        if ws is None or ws.numel() < nb:
+            ws = torch.empty(nb, dtype=torch.uint8, device=self.device)
+            self._fcgk_mixed_ws = ws
+        X = torch.empty_like(B) if X is None else X
+        res = (hip.PcgResult * k)()
+        hip._check(self.lib.cc_fcg_sumfact_ml_mixed_k(ctypes.addressof(dmf.desc), dmf.blocks32, ctypes.addressof(d),
+                                                      k, hip._ptr(B), hip._ptr(X), float(tol), int(maxiter),
+                                                      float(inner_tol), int(max_outer), int(check_every),
+                                                      hip._ptr(ws), nb, ctypes.addressof(res), hip._stream()),
+                   "fcg_sumfact_ml_mixed_k")
         return X, list(res)
 
 

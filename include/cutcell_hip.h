@@ -655,6 +655,20 @@ int cc_fcg_sumfact_ml_k(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32
                         double tol, int32_t maxiter, int32_t check_every, void* workspace, size_t workspace_bytes,
                         cc_pcg_result* results, cc_stream_t stream);
 
+/* The batched learned flexible CG of cc_fcg_sumfact_ml_k inside float64
+ * iterative refinement: each sweep solves A z = r / ||r|| to inner_tol by
+ * flexible CG whose operator applies are float32 (blocks32 as in
+ * cc_pcg_sumfact_mixed_k; directions rounded to float32, products widened
+ * back), then x += ||r|| z and r = B - A x in float64; stops on that residual
+ * (<= tol), at most max_outer sweeps. results[c].iterations counts inner
+ * iterations, .outer the sweeps. No reference counterpart (the reference has
+ * Jacobi only, code/fast_solver_backends.py:74-87). */
+size_t cc_fcg_sumfact_ml_mixed_k_workspace_bytes(const cc_sumfact_desc* d, const cc_mlcnn_desc* ml, int32_t k);
+int cc_fcg_sumfact_ml_mixed_k(const cc_sumfact_desc* d, const float* blocks32, const cc_mlcnn_desc* ml, int32_t k,
+                              const double* B, double* X, double tol, int32_t maxiter, double inner_tol,
+                              int32_t max_outer, int32_t check_every, void* workspace, size_t workspace_bytes,
+                              cc_pcg_result* results, cc_stream_t stream);
+
 /* ---- Network pass of the multilevel preconditioner on gfx950 (no
  * reference counterpart). The preconditioners above evaluate every level's
  * C^T C in three launches that serve all levels and all grids of a batch:

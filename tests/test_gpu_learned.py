@@ -381,6 +381,46 @@ def test_ml_fcg_batched_matches_single_columns(lib):
         assert np.linalg.norm(X[:, c] - x1) / np.linalg.norm(x1) < 1e-6
 
 
+def test_ml_fcg_mixed_reaches_float64_residual(lib):
+    """The batched learned flexible CG inside float64 refinement, its operator
+    applied in float32: every column reaches the float64 residual 1e-8
+    (recomputed on the host with the explicit operator), the solutions agree
+    with the float64 batched solve to 1e-6, and a zero column stays x = 0
+    with no sweep. The iteration count is reported, not held: on this sphere
+    cut the float32 operator is off by up to 1e-3 of |A| |x| (large extension
+    weights of small dofs), each sweep gains less than inner_tol, and the
+    inner iterations exceed the float64 solve's (331 against 205 when
+    measured)."""
+    from cutcell.cut3d import SphereObstacle
+    from cutcell.grid3d import Axis, Grid3D
+    from cutcell.learned import MultilevelCNN
+    from cutcell.matfree import DeviceMatrixFree, assemble_reduced_csr, build_matrix_free
+
+    grid = Grid3D(tuple(Axis(12, 3) for _ in range(3)))
+    mf = build_matrix_free(grid, SphereObstacle.random(4))
+    A = assemble_reduced_csr(mf)
+    dmf = DeviceMatrixFree(mf)
+    net = MultilevelCNN(grid.nb, mf.interior, levels=3, seed=1)
+    B = np.random.default_rng(6).standard_normal((mf.n, 16))
+    B[:, 5] = 0.0
+    Bd = _dev(B)
+    X, res = net.fcg_mixed_batched(dmf, Bd, tol=1e-8, maxiter=4000)
+    X = X.cpu().numpy()
+    Xf, resf = net.fcg_batched(dmf, Bd, tol=1e-8, maxiter=4000)
+    Xf = Xf.cpu().numpy()
+    assert res[5].status == 1 and res[5].outer == 0 and not X[:, 5].any()
+    for c in range(16):
+        if c == 5:
+            continue
+        rel = np.linalg.norm(B[:, c] - A @ X[:, c]) / np.linalg.norm(B[:, c])
+        assert res[c].status == 1 and rel <= 1.05e-8, (c, res[c].status, rel)
+        assert abs(res[c].true_rel - rel) <= 1e-11
+        assert np.linalg.norm(X[:, c] - Xf[:, c]) <= 1e-6 * np.linalg.norm(Xf[:, c])
+        assert res[c].outer <= 12
+    print(f"mixed learned FCG: sweeps {sorted(set(r.outer for r in res))}, inner {res[0].iterations} "
+          f"against float64 {resf[0].iterations}", flush=True)
+
+
 def test_fcg_pairs_are_consistent(lib):
     """Pairs from the learned solver's own iterates: e_k = x* - x_k and
     r_k = b - A x_k, so A e_k = r_k up to x*'s tolerance and float32
