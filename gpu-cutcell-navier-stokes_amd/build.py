@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 INCLUDE = PKG.parent / "include"
 OUT = PKG / "cutcell" / "libcutcell_hip.so"
 
-SOURCES = ["pcg.hip", "rowblock.hip", "batched.hip", "sumfact.hip", "cnn.hip", "fcg.hip", "mg.hip", "ekte.hip", "gmres.hip", "asm2d.hip", "netpass.hip", "runs.hip"]
+SOURCES = ["pcg.hip", "rowblock.hip", "batched.hip", "sumfact.hip", "cnn.hip", "fcg.hip", "mg.hip", "ekte.hip", "gmres.hip", "asm2d.hip", "netpass.hip", "runs.hip", "cut3d.hip"]
 ARCH = os.environ.get("CUTCELL_OFFLOAD_ARCH", "gfx950")
 
 

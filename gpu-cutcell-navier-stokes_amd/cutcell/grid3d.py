@@ -16,6 +16,7 @@ active / interior / small sets, without assembling any global matrix.
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -229,13 +230,77 @@ def _gram_chunk_body(axes_spec, p, q, cells, sub, cell_of):
     return mass, stiff, frac
 
 
+class _Cut3dDesc(ctypes.Structure):
+    """``cc_cut3d_desc`` of the C ABI."""
+
+    _fields_ = [
+        ("n_cut", ctypes.c_int32),
+        ("degree", ctypes.c_int32),
+        ("ne", ctypes.c_int32 * 3),
+        ("n_knots", ctypes.c_int32 * 3),
+        ("length", ctypes.c_double * 3),
+        ("knots", ctypes.c_void_p * 3),
+        ("cells", ctypes.c_void_p),
+        ("corner", ctypes.c_void_p),
+        ("nq", ctypes.c_int32),
+        ("rule_pts", ctypes.c_void_p),
+        ("rule_w", ctypes.c_void_p),
+    ]
+
+
+def gram_blocks_device(grid: Grid3D, cells: np.ndarray, corner: np.ndarray, q: int):
+    """The cut cells' Gram blocks on the GPU (``cc_cut3d_gram``): ``cells``
+    (n, 3) as (ex, ey, ez), ``corner`` (n, 8) level-set values at corner
+    c = x + 2 y + 4 z. Returns host ``(mass, stiff, frac)`` as the host
+    integration does, in the order of ``cells``."""
+    import torch
+
+    from . import hip
+
+    lib = hip.require_gpu()
+    p1 = grid.degree + 1
+    P = p1**3
+    n = cells.shape[0]
+    ref_pts, ref_w = _tet_rule(q)
+    keep = []
+
+    def dev(a, dt):
+        t = torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to("cuda")
+        keep.append(t)
+        return t.data_ptr()
+
+    d = _Cut3dDesc()
+    d.n_cut = n
+    d.degree = grid.degree
+    for k, a in enumerate(grid.axes):
+        d.ne[k] = a.n_elements
+        d.n_knots[k] = a.knots.size
+        d.length[k] = a.length
+        d.knots[k] = dev(a.knots, np.float64)
+    d.cells = dev(cells, np.int32)
+    d.corner = dev(corner, np.float64)
+    d.nq = ref_w.size
+    d.rule_pts = dev(ref_pts, np.float64)
+    d.rule_w = dev(ref_w, np.float64)
+    mass = torch.empty((n, P, P), dtype=torch.float64, device="cuda")
+    stiff = torch.empty_like(mass)
+    frac = torch.empty(n, dtype=torch.float64, device="cuda")
+    hip._check(lib.cc_cut3d_gram(ctypes.addressof(d), hip._ptr(mass), hip._ptr(stiff), hip._ptr(frac), hip._stream()),
+               "cut3d_gram")
+    out = mass.cpu().numpy(), stiff.cpu().numpy(), frac.cpu().numpy()
+    del keep
+    return out
+
+
 def levelset_cut_data(grid: Grid3D, levelset, quadrature_order: int | None = None, chunk_cells: int = 32,
-                      jobs: int | None = None) -> CutData:
+                      jobs: int | None = None, device: bool = False) -> CutData:
     """Classify cells and integrate the cut cells' local Gram blocks.
 
     Cut cells are clipped all at once, sorted by their number of
     sub-tetrahedra (so chunks carry little padding) and integrated in a
-    process pool of ``jobs`` workers (default: the CPU count, at most 16).
+    process pool of ``jobs`` workers (default: the CPU count, at most 16);
+    with ``device`` the integration runs on the GPU instead, one workgroup
+    per cut cell (``gram_blocks_device``).
     """
     import os
     from concurrent.futures import ProcessPoolExecutor
@@ -259,7 +324,11 @@ def levelset_cut_data(grid: Grid3D, levelset, quadrature_order: int | None = Non
     cz, cy, cx = np.nonzero(cls == 2)
     hvec = np.array([ax.h, ay.h, az.h])
     nc_all = cz.size
-    if nc_all:
+    if nc_all and device:
+        cells = np.stack([cx, cy, cz], axis=1).astype(np.int64)
+        m, st, fr = gram_blocks_device(grid, cells, corner[cz, cy, cx], q)
+        masses, stiffs, fracs = [m], [st], [fr]
+    elif nc_all:
         base = np.stack([cx, cy, cz], axis=1) * hvec
         P = base[:, None, None, :] + hvec * _CORNERS[_KUHN][None, :, :, :]
         F = corner[cz, cy, cx][:, _KUHN]

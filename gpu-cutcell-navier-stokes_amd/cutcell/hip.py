@@ -82,6 +82,7 @@ _SIGNATURES = {
     "cc_sumfact_k_workspace_bytes": ([_P, _I32, _I32], _SZ),
     "cc_sumfact_apply_k": ([_P, _I32, _P, _P, _P, _SZ, _P], ctypes.c_int),
     "cc_pcg_sumfact_k": ([_P, _I32, _P, _P, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P], ctypes.c_int),
+    "cc_cut3d_gram": ([_P, _P, _P, _P, _P], ctypes.c_int),
     "cc_sumfact_mixed_k_workspace_bytes": ([_P, _I32], _SZ),
     "cc_sumfact_apply_k_f32": ([_P, _P, _I32, _P, _P, _P, _SZ, _P], ctypes.c_int),
     "cc_pcg_sumfact_mixed_k": ([_P, _P, _I32, _P, _P, _F64, _I32, _F64, _I32, _I32, _I32, _P, _SZ, _P, _P],

@@ -94,10 +94,12 @@ def _full_cell_blocks(grid: Grid3D, cells: np.ndarray, elem_m, elem_k, c: float)
 
 
 def build_matrix_free(grid: Grid3D, levelset, *, mass_coef: float = 1.0, support_threshold: float = 0.04,
-                      quadrature_order: int | None = None) -> MatrixFreeSystem:
+                      quadrature_order: int | None = None, device: bool = False) -> MatrixFreeSystem:
+    """The matrix-free system of a level-set cut; ``device`` integrates the
+    cut cells' Gram blocks on the GPU (``levelset_cut_data``)."""
     p = grid.degree
     p1 = p + 1
-    cut = levelset_cut_data(grid, levelset, quadrature_order)
+    cut = levelset_cut_data(grid, levelset, quadrature_order, device=device)
     shape = (grid.nb[2], grid.nb[1], grid.nb[0])
     poly = monomial_rows(grid, p)
     gre = greville_grid(grid)

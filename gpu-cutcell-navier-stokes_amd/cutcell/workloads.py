@@ -45,13 +45,14 @@ def c4_grid(n_elements: int = 64, degree: int = 3):
     return Grid3D(tuple(Axis(n_elements, degree) for _ in range(3)))
 
 
-def c4_operator(seed: int = 0, n_elements: int = 64, degree: int = 3):
-    """Sphere-obstacle box, matrix-free (``cutcell.matfree``)."""
+def c4_operator(seed: int = 0, n_elements: int = 64, degree: int = 3, device: bool = False):
+    """Sphere-obstacle box, matrix-free (``cutcell.matfree``); ``device``
+    integrates the cut cells on the GPU."""
     from .cut3d import SphereObstacle
     from .matfree import build_matrix_free
 
     lev = SphereObstacle.random(seed)
-    mf = build_matrix_free(c4_grid(n_elements, degree), lev)
+    mf = build_matrix_free(c4_grid(n_elements, degree), lev, device=device)
     mf.meta.update(kind="sphere3d", seed=seed, center=lev.center.tolist(), radius=lev.radius)
     return mf
 
@@ -64,13 +65,14 @@ def c5_grid(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3, lengths=
                    Axis(nz, degree, lengths[2], True)))
 
 
-def c5_operator(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3, mass_coef: float = 1.0):
+def c5_operator(nx: int = 128, ny: int = 64, nz: int = 64, degree: int = 3, mass_coef: float = 1.0,
+                device: bool = False):
     """Bump-channel operator ``E^T (K + c M) E``, matrix-free; ``mass_coef = 0``
     is the pressure-projection operator ``E^T K E`` (singular: the constants
     are its null space, so right-hand sides must sum to zero)."""
     from .matfree import build_matrix_free
 
-    mf = build_matrix_free(c5_grid(nx, ny, nz, degree), bump_levelset(), mass_coef=mass_coef)
+    mf = build_matrix_free(c5_grid(nx, ny, nz, degree), bump_levelset(), mass_coef=mass_coef, device=device)
     mf.meta.update(kind="bump_channel", shape=[nx, ny, nz], mass_coef=mass_coef)
     return mf
 

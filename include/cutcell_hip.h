@@ -194,6 +194,30 @@ int cc_gmres_jacobi_f64(int32_t n, const int32_t* indptr, const int32_t* indices
                         int32_t preconditioner, void* workspace, size_t workspace_bytes, cc_gmres_result* result,
                         cc_stream_t stream);
 
+/* Local Gram blocks of 3D level-set cut cells, the per-cut-cell integration
+ * of the 3D matrix-free systems' assembly (done on the host by
+ * cutcell/grid3d.py levelset_cut_data in a process pool; these geometries
+ * have no reference code). Each cut cell's six Kuhn tetrahedra are clipped
+ * to phi >= 0 of the linear interpolant of its corner values, split into
+ * tetrahedra as the host splits them, and integrated by the collapsed Gauss
+ * rule (rule_pts, rule_w: the host's tetrahedron rule); mass and stiff get the
+ * (p+1)^3 x (p+1)^3 blocks, local function (iz, iy, ix) x fastest, and frac
+ * the cell's fluid volume fraction. */
+typedef struct cc_cut3d_desc {
+  int32_t n_cut;
+  int32_t degree;           /* 1 .. 3 */
+  int32_t ne[3];            /* elements along x, y, z */
+  int32_t n_knots[3];
+  double length[3];
+  const double* knots[3];   /* parametric knots per axis (x / length) */
+  const int32_t* cells;     /* [n_cut][3]: ex, ey, ez */
+  const double* corner;     /* [n_cut][8]: phi at corner c (x bit 0, y bit 1, z bit 2) */
+  int32_t nq;
+  const double* rule_pts;   /* [nq][3] */
+  const double* rule_w;     /* [nq] */
+} cc_cut3d_desc;
+int cc_cut3d_gram(const cc_cut3d_desc* d, double* mass, double* stiff, double* frac, cc_stream_t stream);
+
 /* 2D cut-cell assembly on the device (replaces the per-cell loops of
  * assemble_oblique_cut_matrices, code/oblique_cut_sbp.py:271-433, for the
  * mass H and stiffness K): N x N cells of degree-p tensor B-splines on open
