@@ -284,7 +284,7 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
 
   // Weight operands (A = W[tap][ci][co] with row = co = j, column k = 8q + i).
   bf16x8 WA[9], WB1[3], WB2[3];
-  {
+  if constexpr (MODE != kLast) {
     const uint16_t* __restrict__ w = V.w;
     const int ci0 = 8 * (q & 1);
 #pragma unroll
@@ -308,6 +308,39 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       }
       WB1[dz] = pack8(h1);
       WB2[dz] = pack8(h2);
+    }
+  }
+  // Last pass (16 -> 1 channels): the MFMA's M dimension runs over targets
+  // m = 4 dz + r (tap plane dz, output row r; m >= 12 zero) instead of output
+  // channels, so one MFMA per operand covers every tap that operand feeds.
+  // Row m = j of A for the ra operand of halo row Y holds W[dz][Y - r][dx][ci][0]
+  // at k = 8q + i (ci = 8 (q % 2) + i, dx = q / 2), zero where Y - r is not a
+  // tap row; for the rb operand (dx = 2; rows Y and Y + 1 as q / 2 = 0, 1)
+  // output row r = Y takes taps dy = 0, 1 and r = Y - 1 takes dy = 2 from
+  // row Y + 1, as in the other passes.
+  bf16x8 LA[RW + 2], LB[RW + 1];
+  if constexpr (MODE == kLast) {
+    const uint16_t* __restrict__ w = V.w;
+    const int tdz = j >> 2, tr = j & 3, ci0 = 8 * (q & 1), s1 = q >> 1;
+#pragma unroll
+    for (int Y = 0; Y < RW + 2; ++Y) {
+      const int dy = Y - tr;
+      const bool ok = j < 12 && dy >= 0 && dy <= 2;
+      const int t = ok ? (tdz * 3 + dy) * 3 + s1 : 0;
+      uint16_t h[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) h[i] = ok ? w[(t * 16 + ci0 + i) * 16] : (uint16_t)0;
+      LA[Y] = pack8(h);
+    }
+#pragma unroll
+    for (int Y = 0; Y <= RW; ++Y) {
+      const int dy = tr == Y ? s1 : (tr == Y - 1 && s1 == 1 ? 2 : -1);
+      const bool ok = j < 12 && dy >= 0;
+      const int t = ok ? (tdz * 3 + dy) * 3 + 2 : 0;
+      uint16_t h[8];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) h[i] = ok ? w[(t * 16 + ci0 + i) * 16] : (uint16_t)0;
+      LB[Y] = pack8(h);
     }
   }
   // First pass: A = W0[tap = 8q + i][0][co = j] (taps past 26 zero); for
@@ -552,10 +585,13 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
     }
 
     f32x4 acc[PW][RW];
+    f32x4 lacc[PW + 2];  // last pass: D[4 dz + r][x] of each halo plane
 #pragma unroll
     for (int p = 0; p < PW; ++p)
 #pragma unroll
       for (int r = 0; r < RW; ++r) acc[p][r] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int hz = 0; hz < PW + 2; ++hz) lacc[hz] = f32x4{0.f, 0.f, 0.f, 0.f};
     // Per halo plane the wave's operands (ra of RW + 2 rows, rb of RW + 1)
     // are read together, then consumed by the plane's MFMAs: with two waves
     // per SIMD, a read issued just before its MFMA leaves the LDS latency
@@ -573,6 +609,14 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       }
       // Keeps the scheduler from sinking the reads next to their MFMAs.
       __builtin_amdgcn_sched_barrier(0);
+      if constexpr (MODE == kLast) {
+#pragma unroll
+        for (int Y = 0; Y < RW + 2; ++Y) {
+          lacc[hz] = mfma32(LA[Y], ra[Y], lacc[hz]);
+          if (Y <= RW) lacc[hz] = mfma32(LB[Y], rb[Y], lacc[hz]);
+        }
+        continue;
+      }
 #pragma unroll
       for (int Y = 0; Y < RW + 2; ++Y) {
 #pragma unroll
@@ -609,7 +653,15 @@ __global__ __launch_bounds__(NT, 2) void k_net_pass(const NetLaunch L) {
       for (int r = 0; r < RW; ++r) {
         const bool live = mp[(wy * RW + r + 1) * HX + j + 1] != 0;
         if (MODE == kLast) {
-          pend_f[p][r] = live ? acc[p][r][0] : 0.f;
+          // Output plane p, row r of voxel j: lane (j, q) holds tap plane
+          // dz = q of halo plane p + q; the three are summed across lanes.
+          float v = 0.f;
+          if (q == 0) v = lacc[p][r];
+          if (q == 1) v = lacc[p + 1][r];
+          if (q == 2) v = lacc[p + 2][r];
+          v += __shfl_xor(v, 16, 64);
+          v += __shfl_xor(v, 32, 64);
+          pend_f[p][r] = live ? v : 0.f;
         } else {
           pend[p][r].x = live ? pack_bf16(acc[p][r][0], acc[p][r][1]) : 0u;
           pend[p][r].y = live ? pack_bf16(acc[p][r][2], acc[p][r][3]) : 0u;
