@@ -94,8 +94,10 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const T*
   __shared__ T su[HY][HX];
   __shared__ T sa[HY][kSx];
   __shared__ T sb[HY][kSx];
-  __shared__ T rc[W][kSy][kSx];
-  __shared__ T rd[W][kSy][kSx];
+  // The y-pass results of the last W planes: each thread reads back only its
+  // own (x, y), so the ring lives in registers, indexed by plane mod W (the
+  // plane loop is unrolled by W so that every index is a constant).
+  T rc[W], rd[W];
   const int nbx = d.nb[0], nby = d.nb[1], nbz = d.nb[2];
   const size_t plane = (size_t)nbx * nby;
   // Blocks go to the 8 XCDs round-robin by linear id; each XCD is given one
@@ -154,53 +156,60 @@ __global__ __launch_bounds__(256) void k_mf_tensor_z(cc_sumfact_desc d, const T*
   }
   T val[PER];
   fetch(z_first, val);
-  for (int zi = z_first; zi < ze + p; ++zi) {
-    // Every thread finished the previous plane's x pass (it ended with the
-    // barrier after it), so su may be rewritten.
+  for (int zb = z_first; zb < ze + p; zb += W) {
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-      const int e = tid + 256 * k;
-      if (e < HALO) su[e / HX][e % HX] = val[k];
-    }
-    if (zi + 1 < ze + p) fetch(zi + 1, val);
-    __syncthreads();
-    // x pass: a = Mx u, b = (Kx + c Mx) u on the HY rows of the halo.
-    for (int i = tid; i < HY * kSx; i += 256) {
-      const int ly = i / kSx, lx = i % kSx;  // lx == tx
-      T a = 0, b = 0;
+    for (int u = 0; u < W; ++u) {
+      const int zi = zb + u;
+      if (zi >= ze + p) break;
+      // Every thread has finished the previous plane's x pass (the barrier
+      // after it) and its y pass (before the first barrier below), so su,
+      // and then sa and sb, may be rewritten.
 #pragma unroll
-      for (int k = 0; k < W; ++k) {
-        const T uu = su[ly][lx + k];
-        a += mx[k] * uu;
-        b += kx[k] * uu;
+      for (int k = 0; k < PER; ++k) {
+        const int e = tid + 256 * k;
+        if (e < HALO) su[e / HX][e % HX] = val[k];
       }
-      sa[ly][lx] = a;
-      sb[ly][lx] = b;
-    }
-    __syncthreads();
-    // y pass into the ring: c = My b + Ky a, d = My a.
-    const int slot = (zi - z_first) % W;
-    {
-      T cc_ = 0, dd = 0;
+      if (zi + 1 < ze + p) fetch(zi + 1, val);
+      __syncthreads();
+      // x pass: a = Mx u, b = (Kx + c Mx) u on the HY rows of the halo.
+      for (int i = tid; i < HY * kSx; i += 256) {
+        const int ly = i / kSx, lx = i % kSx;  // lx == tx
+        T a = 0, b = 0;
 #pragma unroll
-      for (int k = 0; k < W; ++k) {
-        cc_ += my[k] * sb[ty + k][tx] + ky[k] * sa[ty + k][tx];
-        dd += my[k] * sa[ty + k][tx];
+        for (int k = 0; k < W; ++k) {
+          const T uu = su[ly][lx + k];
+          a += mx[k] * uu;
+          b += kx[k] * uu;
+        }
+        sa[ly][lx] = a;
+        sb[ly][lx] = b;
       }
-      rc[slot][ty][tx] = cc_;
-      rd[slot][ty][tx] = dd;
-    }
-    __syncthreads();
-    // z pass: output plane zo = zi - p from ring planes zo - p .. zo + p.
-    const int zo = zi - p;
-    if (zo >= zs && gxo < nbx && gyo < nby) {
-      T out = 0;
+      __syncthreads();
+      // y pass into the ring slot of this plane ((zi - z_first) mod W = u):
+      // c = My b + Ky a, d = My a.
+      {
+        T cc_ = 0, dd = 0;
 #pragma unroll
-      for (int k = 0; k < W; ++k) {
-        const int sl = (zo - p + k - z_first) % W;
-        out += mz[zo - zs][k] * rc[sl][ty][tx] + kz[zo - zs][k] * rd[sl][ty][tx];
+        for (int k = 0; k < W; ++k) {
+          cc_ += my[k] * sb[ty + k][tx] + ky[k] * sa[ty + k][tx];
+          dd += my[k] * sa[ty + k][tx];
+        }
+        rc[u] = cc_;
+        rd[u] = dd;
       }
-      v[(size_t)zo * plane + (size_t)gyo * nbx + gxo] = out;
+      // z pass: output plane zo = zi - p from ring planes zo - p .. zo + p,
+      // slot (u - 2p + k) mod W for k = 0 .. 2p.
+      const int zo = zi - p;
+      if (zo >= zs && gxo < nbx && gyo < nby) {
+        T out = 0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) {
+          constexpr int kW = W;
+          const int sl = ((u - 2 * p + k) % kW + kW) % kW;
+          out += mz[zo - zs][k] * rc[sl] + kz[zo - zs][k] * rd[sl];
+        }
+        v[(size_t)zo * plane + (size_t)gyo * nbx + gxo] = out;
+      }
     }
   }
 }
