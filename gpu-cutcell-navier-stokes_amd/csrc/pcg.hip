@@ -52,6 +52,7 @@ struct Profiler {
   bool on = false;
   uint32_t classes = 0;  // bit c: class c is timed
   bool open = false;     // a timed class's first mark is down (pairs never nest)
+  int armed = -1;        // class whose single launch takes the next event pair
   std::vector<hipEvent_t> pool;
   size_t used = 0;
   std::vector<std::pair<int, size_t>> pending;
@@ -85,10 +86,45 @@ void prof_end(hipStream_t s) {
   g_prof.open = false;
 }
 
+void prof_arm(int cls) {
+  if (!g_prof.on || !(g_prof.classes >> cls & 1u)) return;
+  g_prof.armed = cls;
+}
+
+void prof_disarm() { g_prof.armed = -1; }
+
+void prof_take(hipEvent_t* start, hipEvent_t* stop) {
+  *start = *stop = nullptr;
+  if (g_prof.armed < 0) return;
+  while (g_prof.pool.size() < g_prof.used + 2) {
+    hipEvent_t e;
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return;
+    g_prof.pool.push_back(e);
+  }
+  g_prof.pending.push_back({g_prof.armed, g_prof.used});
+  *start = g_prof.pool[g_prof.used];
+  *stop = g_prof.pool[g_prof.used + 1];
+  g_prof.used += 2;
+  g_prof.armed = -1;
+}
+
+// Position in the list of timed intervals not yet collected.
+static size_t prof_pending_mark() { return g_prof.pending.size(); }
+
+// Of the intervals of class cls recorded since mark, keep the first `live`
+// and drop the rest (launches queued after every column had stopped, whose
+// kernels exit at once).
+static void prof_keep(int cls, size_t mark, long long live) {
+  long long seen = 0;
+  for (size_t i = mark; i < g_prof.pending.size(); ++i)
+    if (g_prof.pending[i].first == cls && seen++ >= live) g_prof.pending[i].first = -1;
+}
+
 // Call only after the stream has been synchronised.
 void prof_collect() {
   if (!g_prof.on) return;
   for (auto& pr : g_prof.pending) {
+    if (pr.first < 0) continue;
     float t = 0.f;
     if (pr.second + 1 < g_prof.used &&
         hipEventElapsedTime(&t, g_prof.pool[pr.second], g_prof.pool[pr.second + 1]) == hipSuccess) {
@@ -961,20 +997,20 @@ static int spmm_launch_nz(int n, int unroll, int grid_req, int stream, int nsub,
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
   if (unroll == 8 && stream)
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 8, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
-                       vals, X, Y, B, partial, st, nsub);
+    launch_timed(k_spmm<MT, VT, K, NZ, MODE, 8, CPL, true>, dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
+                 vals, X, Y, B, partial, st, nsub);
   else if (unroll == 8)
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 8, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
-                       indices, vals, X, Y, B, partial, st, nsub);
+    launch_timed(k_spmm<MT, VT, K, NZ, MODE, 8, CPL, false>, dim3(grid), dim3(kBlock), 0, s, n, indptr,
+                 indices, vals, X, Y, B, partial, st, nsub);
   else if (unroll >= 4 && stream)
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, true>), dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
-                       vals, X, Y, B, partial, st, nsub);
+    launch_timed(k_spmm<MT, VT, K, NZ, MODE, 4, CPL, true>, dim3(grid), dim3(kBlock), 0, s, n, indptr, indices,
+                 vals, X, Y, B, partial, st, nsub);
   else if (unroll >= 4)
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 4, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
-                       indices, vals, X, Y, B, partial, st, nsub);
+    launch_timed(k_spmm<MT, VT, K, NZ, MODE, 4, CPL, false>, dim3(grid), dim3(kBlock), 0, s, n, indptr,
+                 indices, vals, X, Y, B, partial, st, nsub);
   else
-    hipLaunchKernelGGL((k_spmm<MT, VT, K, NZ, MODE, 1, CPL, false>), dim3(grid), dim3(kBlock), 0, s, n, indptr,
-                       indices, vals, X, Y, B, partial, st, nsub);
+    launch_timed(k_spmm<MT, VT, K, NZ, MODE, 1, CPL, false>, dim3(grid), dim3(kBlock), 0, s, n, indptr,
+                 indices, vals, X, Y, B, partial, st, nsub);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
@@ -1015,11 +1051,11 @@ static int spmv_win_launch(int n, double avg_row, int grid_req, int stream, int 
   if (grid >= 2 * kXcds) grid -= grid % kXcds;
   if (grid < 1) grid = 1;
   if (stream)
-    hipLaunchKernelGGL((k_spmv_win<MT, VT, MODE, true>), dim3(grid), dim3(kWinBlock), 0, s, n, indptr, indices, vals,
-                       X, Y, B, partial, st, nsub);
+    launch_timed(k_spmv_win<MT, VT, MODE, true>, dim3(grid), dim3(kWinBlock), 0, s, n, indptr, indices, vals,
+                 X, Y, B, partial, st, nsub);
   else
-    hipLaunchKernelGGL((k_spmv_win<MT, VT, MODE, false>), dim3(grid), dim3(kWinBlock), 0, s, n, indptr, indices, vals,
-                       X, Y, B, partial, st, nsub);
+    launch_timed(k_spmv_win<MT, VT, MODE, false>, dim3(grid), dim3(kWinBlock), 0, s, n, indptr, indices, vals,
+                 X, Y, B, partial, st, nsub);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
@@ -1126,7 +1162,7 @@ template <typename VT, int K, typename ApplyFn>
 static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_scale, const int* run_mask,
                    CgBuffers<VT>& b, int precond, double tol, int maxiter, int check_every, hipStream_t s,
                    std::vector<ColState>& host_st, double* hist,
-                   const std::function<int(int)>* after_iter = nullptr) {
+                   const std::function<int(int)>* after_iter = nullptr, bool one_launch = false) {
   const int sg = stream_grid((size_t)n * K);
   hipLaunchKernelGGL((k_cg_setup<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, src, inv_scale, b.X, b.R, b.Z, b.P, b.diag,
                      precond, b.pa);
@@ -1134,11 +1170,19 @@ static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_
   CC_CHECK(hipGetLastError());
   if (check_every < 1) check_every = 1;
   host_st.resize(K);
+  const size_t pmark = prof_pending_mark();
   for (int it = 1; it <= maxiter; ++it) {
     int gs = 0;
-    prof_begin(kProfSpmm, s);
-    CC_CHECK(apply(b.P, b.Q, b.pa, &gs));
-    prof_end(s);
+    // A product that is one launch is timed by that launch's own events.
+    if (one_launch) {
+      prof_arm(kProfSpmm);
+      CC_CHECK(apply(b.P, b.Q, b.pa, &gs));
+      prof_disarm();
+    } else {
+      prof_begin(kProfSpmm, s);
+      CC_CHECK(apply(b.P, b.Q, b.pa, &gs));
+      prof_end(s);
+    }
     hipLaunchKernelGGL((k_cg_alpha<K>), dim3(1), dim3(kFinThreads), 0, s, b.pa, gs, b.st, it);
     prof_begin(kProfUpdate, s);
     hipLaunchKernelGGL((k_cg_update<VT, K>), dim3(sg), dim3(kBlock), 0, s, n, b.X, b.R, b.Z, b.P, b.Q, b.diag,
@@ -1153,14 +1197,18 @@ static int cg_loop(int n, ApplyFn&& apply, const double* src, const double* inv_
     if (it % check_every == 0 || it == maxiter) {
       CC_CHECK(hipMemcpyAsync(host_st.data(), b.st, K * sizeof(ColState), hipMemcpyDeviceToHost, s));
       CC_CHECK(hipStreamSynchronize(s));
-      prof_collect();
       bool any = false;
       for (int c = 0; c < K; ++c) any |= host_st[c].status == CC_RUNNING;
-      if (!any) return 0;
+      if (!any) break;
     }
   }
   CC_CHECK(hipMemcpyAsync(host_st.data(), b.st, K * sizeof(ColState), hipMemcpyDeviceToHost, s));
   CC_CHECK(hipStreamSynchronize(s));
+  // Iteration it's product ran while some column was running: it <= the
+  // last column's count.
+  int live = 0;
+  for (int c = 0; c < K; ++c) live = std::max(live, (int)host_st[c].iterations);
+  prof_keep(kProfSpmm, pmark, live);
   prof_collect();
   return 0;
 }
@@ -1189,7 +1237,8 @@ static int cg_run(int n, double avg_row, const int* indptr, const int* indices, 
     if (rb) return rb_spmm<MT, VT, K>(*rb, P, Q, partial, st, s, gs);
     return spmm_launch<MT, VT, K, 0>(n, avg_row, indptr, indices, vals, P, Q, nullptr, partial, st, s, gs);
   };
-  return cg_loop<VT, K>(n, apply, src, inv_scale, run_mask, b, precond, tol, maxiter, check_every, s, host_st, hist);
+  return cg_loop<VT, K>(n, apply, src, inv_scale, run_mask, b, precond, tol, maxiter, check_every, s, host_st, hist,
+                        nullptr, true);
 }
 
 // Generic float64 single-right-hand-side CG over an operator (used by the

@@ -550,24 +550,24 @@ static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y
   constexpr bool kTeamIsWave = T == 64;
   if (rb.pairs) {
     if (unroll == 4)
-      hipLaunchKernelGGL((k_spmm_rb8<MT, VT, K, NZ, CPL, 4>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
-                         rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
+      launch_timed(k_spmm_rb8<MT, VT, K, NZ, CPL, 4>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                   rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
     else
-      hipLaunchKernelGGL((k_spmm_rb8<MT, VT, K, NZ, CPL, 1>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
-                         rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
+      launch_timed(k_spmm_rb8<MT, VT, K, NZ, CPL, 1>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                   rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
   } else if (kTeamIsWave && rb.grun && rb.grptr) {
     if (unroll == 4)
-      hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 4, kTeamIsWave>), dim3(grid), dim3(kBlock), 0, s, rb.n,
-                         rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
+      launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 4, kTeamIsWave>, dim3(grid), dim3(kBlock), 0, s, rb.n,
+                   rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
     else
-      hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 1, kTeamIsWave>), dim3(grid), dim3(kBlock), 0, s, rb.n,
-                         rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
+      launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 1, kTeamIsWave>, dim3(grid), dim3(kBlock), 0, s, rb.n,
+                   rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
   } else if (unroll == 4) {
-    hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 4, false>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
-                       rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
+    launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 4, false>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                 rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
   } else {
-    hipLaunchKernelGGL((k_spmm_rb<MT, VT, K, NZ, CPL, 1, false>), dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
-                       rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
+    launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 1, false>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
+                 rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
   }
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();

@@ -233,8 +233,8 @@ static int spmv_runs_launch(int n, const int* rptr, const uint32_t* runs, const 
   int grid = (n + kRunWaves - 1) / kRunWaves;
   const int cap = partial ? kMaxParts : 256 * 16;
   if (grid > cap) grid = cap;
-  hipLaunchKernelGGL((k_spmv_runs<MT, VT>), dim3(grid), dim3(64 * kRunWaves), 0, s, n, rptr, runs, indptr, vals, X,
-                     Y, partial, st);
+  launch_timed(k_spmv_runs<MT, VT>, dim3(grid), dim3(64 * kRunWaves), 0, s, n, rptr, runs, indptr, vals, X,
+               Y, partial, st);
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
 }
