@@ -82,6 +82,74 @@ struct MgDims {
   MgAxis a[3];
 };
 
+// Prolongation sources of fine point f: count 1 or 2, coarse indices and weights.
+__device__ inline int pro_src(int f, const MgAxis& ax, int* c, float* w) {
+  if ((f & 1) == 0) {
+    c[0] = f >> 1;
+    w[0] = 1.0f;
+    return 1;
+  }
+  c[0] = (f - 1) >> 1;
+  w[0] = 0.5f;
+  const int c1 = c[0] + 1;
+  if (c1 < ax.nc) {
+    c[1] = c1;
+    w[1] = 0.5f;
+    return 2;
+  }
+  if (ax.per) {
+    c[1] = 0;
+    w[1] = 0.5f;
+    return 2;
+  }
+  w[0] = 1.0f;
+  return 1;
+}
+
+// The 8 prolongation taps of fine point (fx, fy, fz): coarse point offsets
+// (x fastest) and weights wz * wy * wx. A one-source axis repeats its source
+// at weight 0, so the 8 loads of a sum are independent; adding 0 * c leaves
+// every sum as the taps-only loop has it. The weights are 0, 1/8 ... 1, so
+// their products are exact.
+__device__ inline void prolong_taps(const MgDims& g, int fx, int fy, int fz, size_t* off, float* w) {
+  int cx[2], cy[2], cz[2];
+  float wx[2], wy[2], wz[2];
+  if (pro_src(fx, g.a[0], cx, wx) == 1) {
+    cx[1] = cx[0];
+    wx[1] = 0.0f;
+  }
+  if (pro_src(fy, g.a[1], cy, wy) == 1) {
+    cy[1] = cy[0];
+    wy[1] = 0.0f;
+  }
+  if (pro_src(fz, g.a[2], cz, wz) == 1) {
+    cz[1] = cz[0];
+    wz[1] = 0.0f;
+  }
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int k = t >> 2, j = (t >> 1) & 1, i = t & 1;
+    off[t] = ((size_t)cz[k] * g.a[1].nc + cy[j]) * g.a[0].nc + cx[i];
+    w[t] = wz[k] * wy[j] * wx[i];
+  }
+}
+
+// acc + (P coarse)(fx, fy, fz), coarse a single-channel grid with element
+// stride cs: the 8 loads issued together, then the terms added in tap order.
+// Every kernel that prolongates sums this way, so the results agree bit for
+// bit.
+__device__ inline float prolong_value(const MgDims& g, const float* __restrict__ coarse, int cs, float acc, int fx,
+                                      int fy, int fz) {
+  size_t off[8];
+  float w[8], cv[8];
+  prolong_taps(g, fx, fy, fz, off, w);
+#pragma unroll
+  for (int t = 0; t < 8; ++t) cv[t] = coarse[off[t] * cs];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) acc += w[t] * cv[t];
+  return acc;
+}
+
 }  // namespace cc
 
 #include <functional>

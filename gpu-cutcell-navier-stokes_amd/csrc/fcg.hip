@@ -740,30 +740,71 @@ __global__ __launch_bounds__(256) void k_preck_in(int n, const int* __restrict__
   }
 }
 
-// z = D^-1 r + S y (y: column c's grid at c * ybs, element stride ys);
-// partials z.q per column.
+// z = D^-1 r + S y0; partials z.q per column. Without `coarse`, y0 is
+// column c's fine grid y at c * ybs (element stride ys). With `coarse`, the
+// last prolongation is done here: y0 = y + P coarse_c (y may be null),
+// coarse_c the next level's grid at c * cbs and P that of g, summed as
+// k_mg_prolong sums it, so y0 never goes through memory.
 template <int K>
 __global__ __launch_bounds__(256) void k_preck_out(int n, const int* __restrict__ red2full, const double* __restrict__ r,
                                                    const double* __restrict__ diag, const float* __restrict__ y,
-                                                   int ys, size_t ybs, double* __restrict__ z,
-                                                   const double* __restrict__ q, double* __restrict__ part) {
+                                                   int ys, size_t ybs, const float* __restrict__ coarse, MgDims g,
+                                                   size_t cbs, double* __restrict__ z, const double* __restrict__ q,
+                                                   double* __restrict__ part) {
   // Through an LDS tile of 1024 / K unknowns x K columns: the reads of y are
   // runs of one column's grid, the rows of r, q and z whole K-column rows.
   constexpr int TJ = 1024 / K;
   __shared__ float tile[TJ][K + 1];
   __shared__ double red[4 * K];
+  // Per unknown of the tile, the prolongation's taps (shared by the K
+  // columns): coarse offsets and weights, tap-major so that a wave's lanes
+  // (consecutive unknowns) read consecutive words.
+  __shared__ int s_off[8][TJ];
+  __shared__ float s_w[8][TJ];
+  __shared__ int s_v[TJ];
   const int c = threadIdx.x % K;  // the column of every element this thread writes
   const size_t ntiles = ((size_t)n + TJ - 1) / TJ;
   double zq = 0.0;
   for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
     const size_t j0 = tb * TJ;
     __syncthreads();  // the previous tile's rows are done
+    if (coarse) {
+      if (threadIdx.x < TJ && j0 + threadIdx.x < (size_t)n) {
+        const int v = red2full[j0 + threadIdx.x];
+        const int nfx = g.a[0].nf, nfy = g.a[1].nf;
+        const int fx = v % nfx, row = v / nfx, fy = row % nfy, fz = row / nfy;
+        size_t off[8];
+        float w[8];
+        prolong_taps(g, fx, fy, fz, off, w);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          s_off[t][threadIdx.x] = (int)off[t];
+          s_w[t][threadIdx.x] = w[t];
+        }
+        s_v[threadIdx.x] = v;
+      }
+      __syncthreads();
+    }
 #pragma unroll
     for (int qq = 0; qq < 4; ++qq) {
       const int k = threadIdx.x + 256 * qq;
       const int cc = k / TJ, ji = k % TJ;
       const size_t j = j0 + ji;
-      if (j < (size_t)n) tile[ji][cc] = y[(size_t)cc * ybs + (size_t)red2full[j] * ys];
+      if (j < (size_t)n) {
+        if (coarse) {
+          // As prolong_value sums: the add term, then the 8 taps in order.
+          const float* cb = coarse + (size_t)cc * cbs;
+          float acc = y ? y[(size_t)cc * ybs + (size_t)s_v[ji] * ys] : 0.0f;
+          float cv[8];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) cv[t] = cb[s_off[t][ji]];
+#pragma unroll
+          for (int t = 0; t < 8; ++t) acc += s_w[t][ji] * cv[t];
+          tile[ji][cc] = acc;
+        } else {
+          tile[ji][cc] = y[(size_t)cc * ybs + (size_t)red2full[j] * ys];
+        }
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -871,20 +912,26 @@ static int ml_prec_k(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double*
   // One block per 1024-element tile (no partials).
   hipLaunchKernelGGL((k_preck_in<K>), dim3((unsigned)(((size_t)n * K + 1023) / 1024)), dim3(256), 0, s, n, f.red2full,
                      r, diag, mb.u[0], vox[0]);
-  MgDims g[CC_ML_MAX_LEVELS];
+  MgDims g[CC_ML_MAX_LEVELS] = {};
   for (int l = 1; l < L; ++l) {
     mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
     CC_CHECK(mg_restrict(g[l], mb.u[l - 1], 1, mb.u[l], 1, s, K, vox[l - 1], vox[l]));
   }
   CC_CHECK(level_networks(ml, mb, K, s));
   const float* src = mb.y[L - 1];
-  for (int l = L - 1; l >= 1; --l) {
+  for (int l = L - 1; l >= 2; --l) {
     CC_CHECK(mg_prolong(g[l], src, 1, ml.has_net[l - 1] ? mb.y[l - 1] : nullptr, 1, mb.t[l - 1], s, K, vox[l],
                         vox[l - 1], vox[l - 1]));
     src = mb.t[l - 1];
   }
-  hipLaunchKernelGGL((k_preck_out<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, src, 1, vox[0], z, q,
-                     part);
+  // The prolongation to the finest grid is done by the output kernel, at
+  // the active voxels only.
+  if (L >= 2)
+    hipLaunchKernelGGL((k_preck_out<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag,
+                       ml.has_net[0] ? mb.y[0] : nullptr, 1, vox[0], src, g[1], vox[1], z, q, part);
+  else
+    hipLaunchKernelGGL((k_preck_out<K>), dim3(gk(n, K)), dim3(256), 0, s, n, f.red2full, r, diag, src, 1, vox[0],
+                       nullptr, g[0], (size_t)0, z, q, part);
   return (int)hipGetLastError();
 }
 

@@ -44,30 +44,6 @@ __device__ inline bool rest_tap(int c, int d, const MgAxis& ax, int* f, float* w
   return true;
 }
 
-// Prolongation sources of fine point f: count 1 or 2, coarse indices and weights.
-__device__ inline int pro_src(int f, const MgAxis& ax, int* c, float* w) {
-  if ((f & 1) == 0) {
-    c[0] = f >> 1;
-    w[0] = 1.0f;
-    return 1;
-  }
-  c[0] = (f - 1) >> 1;
-  w[0] = 0.5f;
-  const int c1 = c[0] + 1;
-  if (c1 < ax.nc) {
-    c[1] = c1;
-    w[1] = 0.5f;
-    return 2;
-  }
-  if (ax.per) {
-    c[1] = 0;
-    w[1] = 0.5f;
-    return 2;
-  }
-  w[0] = 1.0f;
-  return 1;
-}
-
 // Batched over blockIdx.y: grid b of a batch starts b * fbs (fine) and
 // b * cbs (coarse) elements in.
 __global__ __launch_bounds__(256) void k_mg_restrict(MgDims g, const float* __restrict__ fine, int fs,
@@ -127,36 +103,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgDims g, const float* __res
   for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
        v += (long long)gridDim.x * blockDim.x) {
     const int fx = (int)(v % nfx), fy = (int)((v / nfx) % nfy), fz = (int)(v / ((long long)nfx * nfy));
-    int cx[2], cy[2], cz[2];
-    float wx[2], wy[2], wz[2];
-    // A one-source axis repeats its source at weight 0, so the 8 loads are
-    // independent and issued together; adding 0 * c leaves every sum as the
-    // taps-only loop computes it.
-    if (pro_src(fx, g.a[0], cx, wx) == 1) {
-      cx[1] = cx[0];
-      wx[1] = 0.0f;
-    }
-    if (pro_src(fy, g.a[1], cy, wy) == 1) {
-      cy[1] = cy[0];
-      wy[1] = 0.0f;
-    }
-    if (pro_src(fz, g.a[2], cz, wz) == 1) {
-      cz[1] = cz[0];
-      wz[1] = 0.0f;
-    }
-    float cv[8];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int k = t >> 2, j = (t >> 1) & 1, i = t & 1;
-      cv[t] = coarse[(((size_t)cz[k] * g.a[1].nc + cy[j]) * g.a[0].nc + cx[i]) * cs];
-    }
-    float acc = add ? add[(size_t)v * as] : 0.0f;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int k = t >> 2, j = (t >> 1) & 1, i = t & 1;
-      acc += (wz[k] * wy[j] * wx[i]) * cv[t];
-    }
-    fine[v] = acc;
+    fine[v] = prolong_value(g, coarse, cs, add ? add[(size_t)v * as] : 0.0f, fx, fy, fz);
   }
 }
 
