@@ -656,26 +656,6 @@ __global__ __launch_bounds__(kFinK) void k_fcgk_alpha(const double* __restrict__
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_fcgk_update(int n, double* __restrict__ x, double* __restrict__ r,
-                                                     const double* __restrict__ p, const double* __restrict__ q,
-                                                     double* __restrict__ part, const FcgState* __restrict__ st) {
-  __shared__ double red[4 * K];
-  const int c = threadIdx.x % K;
-  const bool run = st[c].status == CC_RUNNING;
-  const double a = st[c].alpha;
-  double rr = 0.0;
-  const size_t total = (size_t)n * K;
-  if (run)
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (size_t)gridDim.x * blockDim.x) {
-      x[e] += a * p[e];
-      const double rn = r[e] - a * q[e];
-      r[e] = rn;
-      rr += rn * rn;
-    }
-  block_cols<K>(rr, part + (size_t)blockIdx.x * K, red);
-}
-
-template <int K>
 __global__ __launch_bounds__(kFinK) void k_fcgk_rel(const double* __restrict__ part, int nparts, FcgState* st, int iter,
                                                   double tol, int maxiter) {
   __shared__ double smem[kFinK];
@@ -738,6 +718,58 @@ __global__ __launch_bounds__(256) void k_preck_in(int n, const int* __restrict__
       if (j < (size_t)n) u0[(size_t)c * vox + red2full[j]] = tile[c][ji];
     }
   }
+}
+
+// The flexible-CG update fused with the scaling into the grids: on running
+// columns x += alpha p and r -= alpha q (partials r.r per column, for the
+// stop test), then S r of every column into the grids u0[c][voxel], as
+// k_preck_in scales them, with one read of r. At most kMaxParts blocks
+// (grid-strided over the tiles).
+template <int K>
+__global__ __launch_bounds__(256) void k_fcgk_update_in(int n, const int* __restrict__ red2full,
+                                                        double* __restrict__ x, double* __restrict__ r,
+                                                        const double* __restrict__ p, const double* __restrict__ q,
+                                                        const FcgState* __restrict__ st,
+                                                        const double* __restrict__ diag, float* __restrict__ u0,
+                                                        size_t vox, double* __restrict__ part) {
+  constexpr int TJ = 1024 / K;
+  __shared__ float tile[K][TJ + 1];
+  __shared__ double red[4 * K];
+  const int c = threadIdx.x % K;  // the column of every element this thread updates
+  const bool run = st[c].status == CC_RUNNING;
+  const double a = st[c].alpha;
+  double rr = 0.0;
+  const size_t ntiles = ((size_t)n + TJ - 1) / TJ;
+  for (size_t tb = blockIdx.x; tb < ntiles; tb += gridDim.x) {
+    const size_t j0 = tb * TJ;
+    __syncthreads();  // the previous tile's runs are written
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int k = threadIdx.x + 256 * qq;
+      const int ji = k / K;
+      const size_t j = j0 + ji;
+      if (j < (size_t)n) {
+        const size_t e = j * K + c;
+        double rn = r[e];
+        if (run) {
+          x[e] += a * p[e];
+          rn = rn - a * q[e];
+          r[e] = rn;
+          rr += rn * rn;
+        }
+        tile[c][ji] = (float)(sym_scale(diag[j]) * rn);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int k = threadIdx.x + 256 * qq;
+      const int cc = k / TJ, ji = k % TJ;
+      const size_t j = j0 + ji;
+      if (j < (size_t)n) u0[(size_t)cc * vox + red2full[j]] = tile[cc][ji];
+    }
+  }
+  block_cols<K>(rr, part + (size_t)blockIdx.x * K, red);
 }
 
 // z = D^-1 r + S y0; partials z.q per column. Without `coarse`, y0 is
@@ -901,17 +933,19 @@ static MlBuffers carve_ml_k(const cc_mlcnn_desc& ml, char* base, int K) {
   return b;
 }
 
+// scaled_in: the grids u0 already hold S r (k_fcgk_update_in wrote them).
 template <int K>
 static int ml_prec_k(const cc_mlcnn_desc& ml, const MlBuffers& mb, const double* r, const double* diag,
-                     const double* q, double* z, double* part, hipStream_t s) {
+                     const double* q, double* z, double* part, hipStream_t s, bool scaled_in = false) {
   const cc_cnn_desc& f = ml.net[0];
   const int n = f.n_red;
   const int L = ml.levels;
   size_t vox[CC_ML_MAX_LEVELS];
   for (int l = 0; l < L; ++l) vox[l] = ml_vox(ml.net[l]);
   // One block per 1024-element tile (no partials).
-  hipLaunchKernelGGL((k_preck_in<K>), dim3((unsigned)(((size_t)n * K + 1023) / 1024)), dim3(256), 0, s, n, f.red2full,
-                     r, diag, mb.u[0], vox[0]);
+  if (!scaled_in)
+    hipLaunchKernelGGL((k_preck_in<K>), dim3((unsigned)(((size_t)n * K + 1023) / 1024)), dim3(256), 0, s, n,
+                       f.red2full, r, diag, mb.u[0], vox[0]);
   MgDims g[CC_ML_MAX_LEVELS] = {};
   for (int l = 1; l < L; ++l) {
     mg_level_dims(ml.net[l - 1], ml.net[l], &g[l]);
@@ -955,10 +989,13 @@ static int fcg_loop_ml_k(int n, const std::function<int(const double*, double*, 
     CC_CHECK(apply(b.p, b.q, b.pa, &np));
     prof_end(s);
     hipLaunchKernelGGL((k_fcgk_alpha<K>), dim3(1), dim3(kFinK), 0, s, b.pa, np, b.pb, gs, b.st, it);
-    hipLaunchKernelGGL((k_fcgk_update<K>), dim3(gs), dim3(256), 0, s, n, x, b.r, b.p, b.q, b.pc, b.st);
+    // The update also scales r into the finest grids (the preconditioner's
+    // first step), so the timed preconditioner starts at the restrictions.
+    hipLaunchKernelGGL((k_fcgk_update_in<K>), dim3(gs), dim3(256), 0, s, n, ml.net[0].red2full, x, b.r, b.p, b.q,
+                       b.st, diag, mb.u[0], ml_vox(ml.net[0]), b.pc);
     hipLaunchKernelGGL((k_fcgk_rel<K>), dim3(1), dim3(kFinK), 0, s, b.pc, gs, b.st, it, tol, maxiter);
     prof_begin(kProfLearned, s);
-    CC_CHECK(ml_prec_k<K>(ml, mb, b.r, diag, b.q, b.z, b.pa, s));
+    CC_CHECK(ml_prec_k<K>(ml, mb, b.r, diag, b.q, b.z, b.pa, s, true));
     prof_end(s);
     hipLaunchKernelGGL((k_fcgk_beta<K>), dim3(1), dim3(kFinK), 0, s, b.pa, gs, b.st);
     hipLaunchKernelGGL((k_fcgk_dir<K>), dim3(gs), dim3(256), 0, s, n, b.p, b.z, b.r, b.pb, b.st, 0);
