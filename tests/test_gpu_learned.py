@@ -138,7 +138,7 @@ def test_preconditioner_matches_oracle_and_is_symmetric(lib):
 
 def test_learned_fcg_trains_and_solves(lib):
     """Pairs from the solver's own iterates, Adam on the GPU (loss falls),
-    flexible CG to 1e-8 with the float64 true residual as the judge, and the
+    flexible CG to 1e-8 with the float64 true residual deciding the stop, and the
     gain: the trained preconditioner needs fewer iterations than Jacobi-PCG
     on the same cut system (a float64 simulation of this recipe goes from
     236 to about 100)."""
