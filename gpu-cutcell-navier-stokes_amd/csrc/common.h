@@ -234,10 +234,11 @@ size_t fcg_operator_ml_k_bytes(size_t n, const cc_mlcnn_desc& ml, int k);
 struct cc_rb_desc;
 namespace cc {
 // Row-block product Y = A X (rowblock.hip); partial[block][c] gets X.Y sums
-// when not NULL. MT = VT = double or float; K in {1, 2, 4, 8, 16}.
+// when not NULL. With B (not for the paired form): the residual Y = B - A X
+// and partials Y.Y. MT = VT = double or float; K in {1, 2, 4, 8, 16}.
 template <typename MT, typename VT, int K>
 int rb_spmm(const cc_rb_desc& rb, const VT* X, VT* Y, double* partial, const ColState* st, hipStream_t s,
-            int* grid_out);
+            int* grid_out, const VT* B = nullptr);
 bool rb_desc_ok(const cc_rb_desc* rb, int n);
 }  // namespace cc
 

@@ -232,7 +232,8 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __res
                                                  const int* __restrict__ gcol, const MT* __restrict__ gval,
                                                  const VT* __restrict__ X, VT* __restrict__ Y,
                                                  double* __restrict__ partial, const ColState* __restrict__ st,
-                                                 const int* __restrict__ grptr, const uint32_t* __restrict__ grun) {
+                                                 const int* __restrict__ grptr, const uint32_t* __restrict__ grun,
+                                                 const VT* __restrict__ B) {
   constexpr int KL = K / CPL;
   constexpr int T = KL * NZ;
   constexpr int TPW = 64 / T;
@@ -371,14 +372,25 @@ __global__ __launch_bounds__(256) void k_spmm_rb(int n, int ng, const int* __res
         if (row < n) {
           const size_t e = (size_t)row * K + c0;
           VT yv[CPL];
+          if (B) {
+            // Residual form: Y = B - A X, partials Y.Y.
+            VT bv[CPL];
+            vload<VT, CPL>(B + e, bv);
 #pragma unroll
-          for (int m = 0; m < CPL; ++m) yv[m] = (VT)acc[r][m];
-          vstore<VT, CPL>(Y + e, yv);
-          if (partial) {
-            VT xv[CPL];
-            vload<VT, CPL>(X + e, xv);
+            for (int m = 0; m < CPL; ++m) yv[m] = (VT)((AT)bv[m] - acc[r][m]);
+            vstore<VT, CPL>(Y + e, yv);
 #pragma unroll
-            for (int m = 0; m < CPL; ++m) dot[m] += (double)xv[m] * (double)yv[m];
+            for (int m = 0; m < CPL; ++m) dot[m] += (double)yv[m] * (double)yv[m];
+          } else {
+#pragma unroll
+            for (int m = 0; m < CPL; ++m) yv[m] = (VT)acc[r][m];
+            vstore<VT, CPL>(Y + e, yv);
+            if (partial) {
+              VT xv[CPL];
+              vload<VT, CPL>(X + e, xv);
+#pragma unroll
+              for (int m = 0; m < CPL; ++m) dot[m] += (double)xv[m] * (double)yv[m];
+            }
           }
         }
       }
@@ -539,7 +551,7 @@ static RbConfig g_rb_cfg[2][17] = {};
 
 template <typename MT, typename VT, int K, int NZ, int CPL>
 static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y, double* partial, const ColState* st,
-                        int grid_req, int unroll, hipStream_t s, int* grid_out) {
+                        int grid_req, int unroll, hipStream_t s, int* grid_out, const VT* B) {
   constexpr int T = (K / CPL) * NZ;
   const int teams_per_block = (kBlock / 64) * (64 / T);
   int grid = grid_req;
@@ -549,6 +561,7 @@ static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y
   if (grid < 1) grid = 1;
   constexpr bool kTeamIsWave = T == 64;
   if (rb.pairs) {
+    if (B) return CC_E_ARG;  // the paired form has no residual form
     if (unroll == 4)
       launch_timed(k_spmm_rb8<MT, VT, K, NZ, CPL, 4>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
                    rb.pairs, rb.gptr, rb.gcol, gval, X, Y, partial, st);
@@ -558,16 +571,16 @@ static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y
   } else if (kTeamIsWave && rb.grun && rb.grptr) {
     if (unroll == 4)
       launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 4, kTeamIsWave>, dim3(grid), dim3(kBlock), 0, s, rb.n,
-                   rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
+                   rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun, B);
     else
       launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 1, kTeamIsWave>, dim3(grid), dim3(kBlock), 0, s, rb.n,
-                   rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun);
+                   rb.ngroups, rb.gptr, rb.gcol, gval, X, Y, partial, st, rb.grptr, rb.grun, B);
   } else if (unroll == 4) {
     launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 4, false>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
-                 rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
+                 rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr, B);
   } else {
     launch_timed(k_spmm_rb<MT, VT, K, NZ, CPL, 1, false>, dim3(grid), dim3(kBlock), 0, s, rb.n, rb.ngroups,
-                 rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr);
+                 rb.gptr, rb.gcol, gval, X, Y, partial, st, nullptr, nullptr, B);
   }
   if (grid_out) *grid_out = grid;
   return (int)hipGetLastError();
@@ -575,7 +588,7 @@ static int rb_launch_nz(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y
 
 template <typename MT, typename VT, int K, int CPL>
 static int rb_launch_cpl(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* Y, double* partial,
-                         const ColState* st, const RbConfig& cfg, hipStream_t s, int* grid_out) {
+                         const ColState* st, const RbConfig& cfg, hipStream_t s, int* grid_out, const VT* B) {
   constexpr int KL = K / CPL;
   // Default: 64-lane teams, at most 16 slots.
   int nz = cfg.nz > 0 && KL * cfg.nz <= 64 ? cfg.nz : (64 / KL < 16 ? 64 / KL : 16);
@@ -583,7 +596,7 @@ static int rb_launch_cpl(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* 
 #define CC_RB_NZ(NZV)                                                                                         \
   if (nz == NZV && KL * NZV <= 64)                                                                            \
     return rb_launch_nz<MT, VT, K, (KL * NZV <= 64 ? NZV : 1), CPL>(rb, gval, X, Y, partial, st, cfg.grid, \
-                                                                    unroll, s, grid_out);
+                                                                    unroll, s, grid_out, B);
   CC_RB_NZ(64)
   CC_RB_NZ(32)
   CC_RB_NZ(16)
@@ -597,7 +610,7 @@ static int rb_launch_cpl(const cc_rb_desc& rb, const MT* gval, const VT* X, VT* 
 
 template <typename MT, typename VT, int K>
 int rb_spmm(const cc_rb_desc& rb, const VT* X, VT* Y, double* partial, const ColState* st, hipStream_t s,
-            int* grid_out) {
+            int* grid_out, const VT* B) {
   const MT* gval = nullptr;
   if constexpr (sizeof(MT) == 8)
     gval = rb.gval;
@@ -610,17 +623,17 @@ int rb_spmm(const cc_rb_desc& rb, const VT* X, VT* Y, double* partial, const Col
   if (cpl > K || cpl * (int)sizeof(VT) > 16) cpl = K >= wide ? wide : K;
   if (cpl >= 4 && sizeof(VT) == 4 && K >= 4)
     return rb_launch_cpl<MT, VT, K, (sizeof(VT) == 4 && K >= 4 ? 4 : 1)>(rb, gval, X, Y, partial, st, cfg, s,
-                                                                        grid_out);
+                                                                        grid_out, B);
   if (cpl >= 2 && K >= 2)
-    return rb_launch_cpl<MT, VT, K, (K >= 2 ? 2 : 1)>(rb, gval, X, Y, partial, st, cfg, s, grid_out);
-  return rb_launch_cpl<MT, VT, K, 1>(rb, gval, X, Y, partial, st, cfg, s, grid_out);
+    return rb_launch_cpl<MT, VT, K, (K >= 2 ? 2 : 1)>(rb, gval, X, Y, partial, st, cfg, s, grid_out, B);
+  return rb_launch_cpl<MT, VT, K, 1>(rb, gval, X, Y, partial, st, cfg, s, grid_out, B);
 }
 
 #define CC_RB_INST(K)                                                                                          \
   template int rb_spmm<double, double, K>(const cc_rb_desc&, const double*, double*, double*, const ColState*, \
-                                          hipStream_t, int*);                                                  \
+                                          hipStream_t, int*, const double*);                                   \
   template int rb_spmm<float, float, K>(const cc_rb_desc&, const float*, float*, double*, const ColState*,     \
-                                        hipStream_t, int*);
+                                        hipStream_t, int*, const float*);
 CC_RB_INST(1)
 CC_RB_INST(2)
 CC_RB_INST(4)
