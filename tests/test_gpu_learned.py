@@ -184,7 +184,12 @@ def test_learned_fcg_trains_and_solves(lib):
 ML_GRIDS = [((19, 9, 7), (False, False, False)), ((20, 9, 8), (True, False, True))]
 
 
-@pytest.mark.parametrize("dims,periodic", ML_GRIDS)
+# Larger grids: several hundred coarse points per plane, every axis longer
+# than the smaller grids', clamped with odd and even lengths and all-periodic.
+MG_TILE_GRIDS = [((70, 11, 7), (False, False, False)), ((72, 12, 10), (True, True, True))]
+
+
+@pytest.mark.parametrize("dims,periodic", ML_GRIDS + MG_TILE_GRIDS)
 def test_mg_transfers_match_oracle(lib, dims, periodic):
     """cc_mg_restrict = P^T and cc_mg_prolong = P (+ add) of the oracle's
     entry-by-entry interpolation matrix, with element strides 1 and 16."""
