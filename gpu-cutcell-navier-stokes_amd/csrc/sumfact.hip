@@ -941,6 +941,36 @@ static MfBuffers carve_mfk(const cc_sumfact_desc* d, int K, char* base) {
 
 // One K-column apply in value type T; blocks: the cut-cell corrections in T
 // (the descriptor's float64 blocks, or a float32 copy).
+// A second stream and two events per host thread and device, made on first
+// use: the K-column apply runs its shell corrections there, beside the
+// tensor product on the caller's stream. The two read the same u and write
+// different arrays; the touch kernel waits for both. The tensor product is
+// bound by its own issue rate and the corrections by memory, so they share
+// the CUs well. Default priority; never destroyed (one per thread).
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+static SideStream* side_stream() {
+  constexpr int kDevices = 16;
+  thread_local SideStream per_dev[kDevices];
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDevices) return nullptr;
+  SideStream& ss = per_dev[dev];
+  if (!ss.s) {
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (hipEventCreateWithFlags(&e0, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&e1, hipEventDisableTiming) != hipSuccess) return nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    ss.fork = e0;
+    ss.join = e1;
+    ss.s = st;
+  }
+  return &ss;
+}
+
 template <typename T, int P1, int K>
 static int mfk_apply_t(const cc_sumfact_desc& d, const T* blocks, const MfBuf<T>& b, const T* x, T* y,
                        double* partial, int* nparts, hipStream_t s) {
@@ -950,10 +980,23 @@ static int mfk_apply_t(const cc_sumfact_desc& d, const T* blocks, const MfBuf<T>
   hipLaunchKernelGGL((k_mfk_gather<T, K>), dim3(grid_items(n_full * K, 1024)), dim3(256), 0, s, d, x, b.u);
   if (d.n_small > 0)
     hipLaunchKernelGGL((k_mfk_gather_small<T, K>), dim3((d.n_small + 3) / 4), dim3(256), 0, s, d, x, b.u);
+  // The shell corrections (u -> W) beside the tensor product (u -> v).
+  SideStream* ss = d.n_corr > 0 && K > 1 ? side_stream() : nullptr;
+  const dim3 shell_grid(d.n_corr), shell_block(64 * (K < 8 ? 1 : K / 8));
+  // The tensor product is queued first, so that its blocks are dispatched
+  // before the corrections' many blocks take the CUs.
+  if (ss) CC_CHECK(hipEventRecord(ss->fork, s));
   tensor_launch<T, P1>(d, b.u, b.v, K, s);
+  if (ss) {
+    CC_CHECK(hipStreamWaitEvent(ss->s, ss->fork, 0));
+    hipLaunchKernelGGL((k_mfk_shell<T, P1, K>), shell_grid, shell_block, 0, ss->s, d, blocks, b.u, b.W);
+    CC_CHECK(hipEventRecord(ss->join, ss->s));
+  }
   if (d.n_corr > 0) {
-    hipLaunchKernelGGL((k_mfk_shell<T, P1, K>), dim3(d.n_corr), dim3(64 * (K < 8 ? 1 : K / 8)), 0, s, d, blocks, b.u,
-                       b.W);
+    if (ss)
+      CC_CHECK(hipStreamWaitEvent(s, ss->join, 0));
+    else
+      hipLaunchKernelGGL((k_mfk_shell<T, P1, K>), shell_grid, shell_block, 0, s, d, blocks, b.u, b.W);
     hipLaunchKernelGGL((k_mfk_touch<T, K>), dim3(grid_items(d.n_touch, 4)), dim3(256), 0, s, d, b.W, b.v, PP);
   }
   if (d.n_et_rows > 0)
